@@ -1,0 +1,491 @@
+// Host engine: device context, resident protein spectra, ligand tables and
+// the two search pipelines.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "kernels.hpp"
+
+namespace cdk {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) throw Error(kErrNoMem, msg);
+  throw Error(kErrHip, msg);
+}
+
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+void DevBuf::ensure(size_t bytes) {
+  if (bytes <= cap) return;
+  release();
+  if (bytes == 0) return;
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  cap = bytes;
+}
+
+Context::Context(int dev) : device(dev) {
+  int n = 0;
+  hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+  if (dev < 0 || dev >= n) throw Error(kErrInvalid, "device index out of range");
+  hip_check(hipSetDevice(dev), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+Context::~Context() {
+  (void)hipSetDevice(device);
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& kv : twiddles) delete kv.second;
+  if (h_keys) (void)hipHostFree(h_keys);
+  if (h_meta) (void)hipHostFree(h_meta);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+const float2* Context::twiddle(int N) {
+  auto it = twiddles.find(N);
+  if (it != twiddles.end()) return it->second->as<float2>();
+  std::vector<float2> t(N);
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int m = 0; m < N; ++m) {
+    const double a = two_pi * (double)m / (double)N;
+    t[m] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+  auto* b = new DevBuf();
+  b->ensure(N * sizeof(float2));
+  hip_check(hipMemcpy(b->p, t.data(), N * sizeof(float2), hipMemcpyHostToDevice), "twiddle upload");
+  twiddles[N] = b;
+  return b->as<float2>();
+}
+
+void Context::ensure_host(size_t nslots) {
+  if (nslots <= h_cap) return;
+  if (h_keys) (void)hipHostFree(h_keys);
+  if (h_meta) (void)hipHostFree(h_meta);
+  h_keys = h_meta = nullptr;
+  hip_check(hipHostMalloc(&h_keys, nslots * sizeof(unsigned long long)), "hipHostMalloc");
+  hip_check(hipHostMalloc(&h_meta, nslots * sizeof(RotMeta)), "hipHostMalloc");
+  h_cap = nslots;
+}
+
+void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const float* len, const float* radii,
+                    const float* values) {
+  if (nch < 2 || nch > kMaxCh) throw Error(kErrInvalid, "number of grids must be in [2, 16]");
+  if (!grid_edge_supported(N)) throw Error(kErrUnsupported, "grid edge must be one of 32, 48, 64, 96, 128");
+  if (!(h > 0.f)) throw Error(kErrInvalid, "grid spacing must be positive");
+  Context& ctx = *p.ctx;
+  p.nch = nch;
+  p.N = N;
+  p.h = h;
+  for (int k = 0; k < 3; ++k) {
+    p.mid[k] = mid ? mid[k] : 0.f;
+    p.len[k] = len ? len[k] : (float)(N - 1) * h;
+  }
+  p.radii.assign(radii, radii + (nch - 1));
+  const size_t G = p.G(), H = p.H();
+  p.grid.ensure((size_t)nch * G * sizeof(float));
+  p.spec.ensure((size_t)nch * H * sizeof(float2));
+  hip_check(hipMemcpyAsync(p.grid.p, values, (size_t)nch * G * sizeof(float), hipMemcpyHostToDevice, ctx.stream),
+            "protein upload");
+  hip_check(launch_r2c_3d(ctx.stream, N, p.grid.as<float>(), p.spec.as<float2>(), nch, ctx.twiddle(N)),
+            "protein R2C");
+  hip_check(hipStreamSynchronize(ctx.stream), "protein R2C sync");
+}
+
+std::unique_ptr<Ligand> make_ligand(const Protein& p, int natom, const float* charge, const float* eps,
+                                    const float* radius) {
+  if (natom <= 0 || natom > kMaxAtoms) throw Error(kErrInvalid, "ligand atom count must be in [1, 512]");
+  auto L = std::make_unique<Ligand>();
+  L->natom = natom;
+  L->charge.assign(charge, charge + natom);
+  L->eps.assign(eps, eps + natom);
+  L->radius.assign(radius, radius + natom);
+  const int nv = p.nch - 1;
+  // GetIdxOfAtomsForVdwRadius.cpp:9-23: nearest probe radius, strict '<' so
+  // the first of equal distances wins; float difference, double running min.
+  L->atoms_of_radius.assign(nv, {});
+  L->atom_channel.assign(natom, 0);
+  std::vector<int> radius_of_atom(natom);
+  for (int i = 0; i < natom; ++i) {
+    int idx = 0;
+    double diff = std::abs(radius[i] - p.radii[0]);
+    for (int j = 1; j < nv; ++j) {
+      if (std::abs(radius[i] - p.radii[j]) < diff) {
+        idx = j;
+        diff = std::abs(radius[i] - p.radii[j]);
+      }
+    }
+    L->atoms_of_radius[idx].push_back(i);
+    radius_of_atom[i] = idx;
+  }
+  // GetIdxOfAtomsForVdwRadius.cpp:26-34: used radii in ascending order.
+  for (int j = 0; j < nv; ++j)
+    if (!L->atoms_of_radius[j].empty()) L->idx_vdw_used.push_back(j);
+  L->num_vdw_used = (int)L->idx_vdw_used.size();
+  const int nch = L->num_vdw_used + 1;
+  if (nch > kMaxCh) throw Error(kErrInvalid, "too many channels");
+  LigandDesc& d = L->desc;
+  std::memset(&d, 0, sizeof(d));
+  d.natom = natom;
+  d.nch = nch;
+  d.w_off = 0;
+  d.list_off = 0;
+  int pos = 0;
+  for (int c = 0; c < L->num_vdw_used; ++c) {
+    d.ch_start[c] = pos;
+    d.pmap[c] = L->idx_vdw_used[c];
+    for (int a : L->atoms_of_radius[L->idx_vdw_used[c]]) {
+      L->chan_atoms.push_back(a);
+      L->atom_channel[a] = c;
+    }
+    pos = (int)L->chan_atoms.size();
+  }
+  // Electrostatic channel: every atom, elec protein grid is the last one
+  // (TranRotaConfSearchNoMini.cu:143-146).
+  d.ch_start[L->num_vdw_used] = pos;
+  d.pmap[L->num_vdw_used] = p.nch - 1;
+  for (int a = 0; a < natom; ++a) L->chan_atoms.push_back(a);
+  d.ch_start[nch] = (int)L->chan_atoms.size();
+  L->wv.resize(natom);
+  L->wq.resize(natom);
+  for (int a = 0; a < natom; ++a) {
+    L->wv[a] = std::sqrt(eps[a]);  // float overload, as FillLigandGrid.cpp:45
+    L->wq[a] = charge[a];
+  }
+  return L;
+}
+
+float key_energy(unsigned long long key) {
+  if (key == ~0ull) return std::numeric_limits<float>::infinity();
+  unsigned u = (unsigned)(key >> 32);
+  u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+unsigned key_index(unsigned long long key) { return (unsigned)(key & 0xffffffffu); }
+
+namespace {
+
+// Upper bound of the occupied box edge of a conformer over all rotations:
+// the extent along any axis is at most the largest pairwise distance.
+int box_bound(const float* xyz, int natom, float h, int N, double qscale) {
+  double d2 = 0;
+  for (int i = 0; i < natom; ++i)
+    for (int j = i + 1; j < natom; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) {
+        const double d = (double)xyz[i * 3 + k] - (double)xyz[j * 3 + k];
+        s += d * d;
+      }
+      d2 = std::max(d2, s);
+    }
+  const int b = (int)std::floor(std::sqrt(d2) * qscale / (double)h) + 3;
+  return std::min(N, std::max(b, 2));
+}
+
+// Rotation scales lengths by |q|^2; the bound must hold for non-unit input.
+double quat_scale(const float* q, size_t n) {
+  double m = 1.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double s = (double)q[i * 4] * q[i * 4] + (double)q[i * 4 + 1] * q[i * 4 + 1] +
+                     (double)q[i * 4 + 2] * q[i * 4 + 2] + (double)q[i * 4 + 3] * q[i * 4 + 3];
+    m = std::max(m, s);
+  }
+  return m * (1.0 + 1e-5);
+}
+
+struct Tables {
+  std::vector<LigandDesc> ligs;
+  std::vector<int> chan;
+  std::vector<float> wv, wq, coords;
+  std::vector<PoseSlot> slots;
+  std::vector<size_t> job_first;  // first slot of each job
+  int astride = 0, cmax = 0, boxmax = 2;
+  int dummy_lig = 0;
+};
+
+void build_tables(const Protein& prot, const std::vector<Job>& jobs, Tables& t) {
+  std::map<const Ligand*, int> lig_index;
+  for (const Job& j : jobs) {
+    if (!j.lig) throw Error(kErrInvalid, "job without ligand");
+    if (j.nconf < 0 || j.nquat < 0) throw Error(kErrInvalid, "negative conformer or quaternion count");
+    if (lig_index.count(j.lig)) continue;
+    const Ligand& L = *j.lig;
+    LigandDesc d = L.desc;
+    d.list_off = (int)t.chan.size();
+    d.w_off = (int)t.wv.size();
+    t.chan.insert(t.chan.end(), L.chan_atoms.begin(), L.chan_atoms.end());
+    t.wv.insert(t.wv.end(), L.wv.begin(), L.wv.end());
+    t.wq.insert(t.wq.end(), L.wq.begin(), L.wq.end());
+    for (int c = 0; c < d.nch; ++c)
+      if (d.pmap[c] < 0 || d.pmap[c] >= prot.nch) throw Error(kErrInvalid, "ligand built for another protein");
+    lig_index[j.lig] = (int)t.ligs.size();
+    t.ligs.push_back(d);
+    t.astride = std::max(t.astride, L.natom);
+    t.cmax = std::max(t.cmax, d.nch);
+  }
+  // Padding ligand: zero atoms, so its poses are never valid.
+  LigandDesc dummy;
+  std::memset(&dummy, 0, sizeof(dummy));
+  dummy.nch = 1;
+  t.dummy_lig = (int)t.ligs.size();
+  t.ligs.push_back(dummy);
+  if (t.chan.empty()) t.chan.push_back(0);
+  if (t.wv.empty()) {
+    t.wv.push_back(0.f);
+    t.wq.push_back(0.f);
+  }
+  size_t nslots = 0;
+  for (const Job& j : jobs) nslots += j.npose();
+  if (nslots > (size_t)std::numeric_limits<int>::max()) throw Error(kErrInvalid, "too many poses in one call");
+  t.slots.reserve(nslots);
+  for (const Job& j : jobs) {
+    t.job_first.push_back(t.slots.size());
+    const size_t np = j.npose();
+    if (np == 0) continue;
+    const int li = lig_index[j.lig];
+    const int na = j.lig->natom;
+    const size_t nq_total = (size_t)(j.shared_quats ? 1 : j.nconf) * j.nquat;
+    for (size_t i = 0; i < nq_total * 4; ++i)
+      if (!std::isfinite(j.quats[i])) throw Error(kErrInvalid, "non-finite quaternion");
+    const double qscale = quat_scale(j.quats, nq_total);
+    int cur_conf = -1;
+    int off = 0;
+    for (size_t pose = j.pose_begin; pose < j.pose_begin + np; ++pose) {
+      const int c = (int)(pose / (size_t)j.nquat), r = (int)(pose % (size_t)j.nquat);
+      if (c != cur_conf) {
+        cur_conf = c;
+        if (t.coords.size() + (size_t)na * 3 > (size_t)std::numeric_limits<int>::max())
+          throw Error(kErrInvalid, "coordinate table too large");
+        off = (int)t.coords.size();
+        const float* xyz = j.coords + (size_t)c * na * 3;
+        for (int a = 0; a < na * 3; ++a)
+          if (!std::isfinite(xyz[a])) throw Error(kErrInvalid, "non-finite ligand coordinate");
+        t.coords.insert(t.coords.end(), xyz, xyz + (size_t)na * 3);
+        t.boxmax = std::max(t.boxmax, box_bound(xyz, na, prot.h, prot.N, qscale));
+      }
+      const float* q = j.shared_quats ? j.quats : j.quats + (size_t)c * j.nquat * 4;
+      PoseSlot sl;
+      std::memset(&sl, 0, sizeof(sl));
+      for (int k = 0; k < 4; ++k) sl.q[k] = q[r * 4 + k];
+      sl.lig = li;
+      sl.coord_off = off;
+      t.slots.push_back(sl);
+    }
+  }
+  if (t.coords.empty()) t.coords.resize(3, 0.f);
+  t.astride = std::max(t.astride, 1);
+  t.cmax = std::max(t.cmax, 1);
+}
+
+template <class T>
+void upload(Context& ctx, DevBuf& b, const std::vector<T>& v) {
+  b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (!v.empty())
+    hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx.stream), "upload");
+}
+
+struct EventTimer {
+  Context& ctx;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  explicit EventTimer(Context& c) : ctx(c) {}
+  ~EventTimer() {
+    for (auto& e : ev) {
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+  }
+  hipEvent_t begin(int stage) {
+    if (!ctx.time_stages) return nullptr;
+    hipEvent_t a, b;
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    ev.push_back({stage, {a, b}});
+    hip_check(hipEventRecord(a, ctx.stream), "hipEventRecord");
+    return b;
+  }
+  void end(hipEvent_t b) {
+    if (b) hip_check(hipEventRecord(b, ctx.stream), "hipEventRecord");
+  }
+  void collect() {
+    for (auto& e : ev) {
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, e.second.first, e.second.second), "hipEventElapsedTime");
+      ctx.stage_times.ms[e.first] += ms;
+      ctx.stage_times.launches[e.first] += 1;
+    }
+  }
+};
+
+}  // namespace
+
+void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,
+                std::vector<PoseResults>& out) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  Tables t;
+  build_tables(prot, jobs, t);
+  const int N = prot.N;
+  const size_t G = prot.G(), H = prot.H();
+  const int ZH = N / 2 + 1;
+  const float2* tw = ctx.twiddle(N);
+  const size_t total = t.slots.size();
+  out.assign(jobs.size(), PoseResults());
+  if (total == 0) return;
+
+  upload(ctx, ctx.d_ligs, t.ligs);
+  upload(ctx, ctx.d_chan, t.chan);
+  upload(ctx, ctx.d_wv, t.wv);
+  upload(ctx, ctx.d_wq, t.wq);
+  upload(ctx, ctx.d_coords, t.coords);
+  upload(ctx, ctx.d_slots, t.slots);
+  ctx.d_keys.ensure(total * sizeof(unsigned long long));
+  ctx.d_meta.ensure(total * sizeof(RotMeta));
+  const PoseSlot* d_slots = ctx.d_slots.as<PoseSlot>();
+  const LigandDesc* d_ligs = ctx.d_ligs.as<LigandDesc>();
+  unsigned long long* d_keys = ctx.d_keys.as<unsigned long long>();
+  RotMeta* d_meta = ctx.d_meta.as<RotMeta>();
+  EventTimer timer(ctx);
+
+  if (engine == kEngineReference) {
+    // One launch sequence per batch of B rotations of one conformer
+    // (TranRotaConfSearchNoMini.cu:297-464).
+    const int B = std::max(1, ctx.ref_batch);
+    const int C = t.cmax;
+    ctx.d_rot.ensure((size_t)B * t.astride * 3 * sizeof(float));
+    ctx.d_ag.ensure((size_t)B * t.astride * sizeof(AtomGrid));
+    ctx.d_L.ensure((size_t)B * C * G * sizeof(float));
+    ctx.d_Lhat.ensure((size_t)B * C * H * sizeof(float2));
+    ctx.d_S.ensure((size_t)B * H * sizeof(float2));
+    ctx.d_E.ensure((size_t)B * G * sizeof(float));
+    size_t s = 0;
+    while (s < total) {
+      // Batch: consecutive poses of one conformer, at most B of them.
+      const int lig_idx = t.slots[s].lig;
+      const int coff = t.slots[s].coord_off;
+      size_t e2 = s + 1;
+      while (e2 < total && (int)(e2 - s) < B && t.slots[e2].coord_off == coff && t.slots[e2].lig == lig_idx) ++e2;
+      const int n = (int)(e2 - s);
+      const int nch = t.ligs[lig_idx].nch;
+      const int* pmap = reinterpret_cast<const int*>(reinterpret_cast<const char*>(d_ligs + lig_idx) +
+                                                     offsetof(LigandDesc, pmap));
+      const PoseSlot* sl = d_slots + s;
+      hipEvent_t e;
+      e = timer.begin(0);
+      hip_check(launch_prep(ctx.stream, sl, n, d_ligs, ctx.d_coords.as<float>(), N, prot.h, t.astride, N,
+                            ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
+                "prep");
+      timer.end(e);
+      e = timer.begin(1);
+      hip_check(launch_spread_full(ctx.stream, sl, n, d_ligs, ctx.d_chan.as<int>(), ctx.d_wv.as<float>(),
+                                   ctx.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, N, nch,
+                                   ctx.d_L.as<float>()),
+                "spread");
+      timer.end(e);
+      e = timer.begin(2);
+      hip_check(launch_r2c_3d(ctx.stream, N, ctx.d_L.as<float>(), ctx.d_Lhat.as<float2>(), n * nch, tw),
+                "ligand R2C");
+      timer.end(e);
+      e = timer.begin(3);
+      hip_check(launch_conj_mult(ctx.stream, ctx.d_Lhat.as<float2>(), prot.spec.as<float2>(), pmap, nch,
+                                 (long long)H, n),
+                "ConjMult");
+      timer.end(e);
+      e = timer.begin(4);
+      hip_check(launch_sum_grids(ctx.stream, ctx.d_Lhat.as<float2>(), ctx.d_S.as<float2>(), nch, (long long)H, n,
+                                 (float)G),
+                "SumGrids");
+      timer.end(e);
+      e = timer.begin(5);
+      hip_check(launch_c2r_3d(ctx.stream, N, ctx.d_S.as<float2>(), ctx.d_E.as<float>(), n, tw), "C2R");
+      timer.end(e);
+      e = timer.begin(6);
+      hip_check(launch_argmin_full(ctx.stream, ctx.d_E.as<float>(), d_meta + s, d_keys + s, N, n), "argmin");
+      timer.end(e);
+      s = e2;
+    }
+  } else {
+    const int cmax = t.cmax, boxmax = t.boxmax;
+    const size_t plane = (size_t)N * ZH;
+    const size_t per_slot = (size_t)cmax * boxmax * plane * sizeof(float2) + (size_t)N * plane * sizeof(float2) +
+                            (size_t)t.astride * (3 * sizeof(float) + sizeof(AtomGrid));
+    size_t chunk = ctx.fused_chunk > 0 ? (size_t)ctx.fused_chunk : std::max<size_t>(1, ctx.ws_budget / per_slot);
+    chunk = std::min(chunk, total);
+    ctx.d_rot.ensure(chunk * t.astride * 3 * sizeof(float));
+    ctx.d_ag.ensure(chunk * t.astride * sizeof(AtomGrid));
+    ctx.d_A.ensure(chunk * cmax * boxmax * plane * sizeof(float2));
+    ctx.d_T.ensure(chunk * N * plane * sizeof(float2));
+    const int group = std::max(1, ctx.mid_group);
+    for (size_t s = 0; s < total; s += chunk) {
+      const int n = (int)std::min(chunk, total - s);
+      const PoseSlot* sl = d_slots + s;
+      hipEvent_t e;
+      e = timer.begin(0);
+      hip_check(launch_prep(ctx.stream, sl, n, d_ligs, ctx.d_coords.as<float>(), N, prot.h, t.astride, boxmax,
+                            ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
+                "prep");
+      timer.end(e);
+      e = timer.begin(1);
+      hip_check(launch_fwd_planes(ctx.stream, N, sl, n, d_ligs, ctx.d_chan.as<int>(), ctx.d_wv.as<float>(),
+                                  ctx.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, cmax, boxmax,
+                                  ctx.d_A.as<float2>(), tw),
+                "fwd planes");
+      timer.end(e);
+      e = timer.begin(2);
+      hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), cmax, boxmax,
+                           prot.spec.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+                "mid");
+      timer.end(e);
+      e = timer.begin(3);
+      hip_check(launch_inv_argmin(ctx.stream, N, d_meta + s, n, N - 2, ctx.d_T.as<float2>(), d_keys + s, tw),
+                "inverse + argmin");
+      timer.end(e);
+    }
+  }
+  ctx.ensure_host(total);
+  hip_check(hipMemcpyAsync(ctx.h_keys, d_keys, total * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           ctx.stream),
+            "keys D2H");
+  hip_check(hipMemcpyAsync(ctx.h_meta, d_meta, total * sizeof(RotMeta), hipMemcpyDeviceToHost, ctx.stream),
+            "meta D2H");
+  hip_check(hipStreamSynchronize(ctx.stream), "search sync");
+  timer.collect();
+  const unsigned long long* keys = static_cast<const unsigned long long*>(ctx.h_keys);
+  const RotMeta* meta = static_cast<const RotMeta*>(ctx.h_meta);
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const size_t first = t.job_first[j];
+    const size_t n = jobs[j].npose();
+    PoseResults& r = out[j];
+    r.energy.resize(n);
+    r.index.resize(n * 3);
+    r.mincoor.resize(n * 3);
+    for (size_t i = 0; i < n; ++i) {
+      const unsigned long long k = keys[first + i];
+      const RotMeta& m = meta[first + i];
+      if (k == ~0ull || !m.valid) {
+        r.energy[i] = std::numeric_limits<float>::infinity();
+        r.index[i * 3 + 0] = r.index[i * 3 + 1] = r.index[i * 3 + 2] = -1;
+      } else {
+        const unsigned lin = key_index(k);
+        r.energy[i] = key_energy(k);
+        r.index[i * 3 + 0] = (int)(lin / (unsigned)(N * N));
+        r.index[i * 3 + 1] = (int)((lin / (unsigned)N) % (unsigned)N);
+        r.index[i * 3 + 2] = (int)(lin % (unsigned)N);
+      }
+      for (int k3 = 0; k3 < 3; ++k3) r.mincoor[i * 3 + k3] = m.minc[k3];
+    }
+  }
+}
+
+}  // namespace cdk

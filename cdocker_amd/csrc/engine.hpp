@@ -1,0 +1,161 @@
+// Host engine: device context, resident protein spectra, ligand tables and
+// the two search pipelines (reference dataflow and fused/pruned).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cdk_types.hpp"
+
+namespace cdk {
+
+// Error carrying a C-ABI status code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status {
+  kOk = 0,
+  kErrInvalid = 1,
+  kErrHip = 2,
+  kErrNoMem = 3,
+  kErrUnsupported = 4,
+  kErrPending = 5,
+  kErrBudget = 6,
+  kErrCancelled = 7,
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void ensure(size_t bytes);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Context;
+
+// Resident protein: potential grids and their forward spectra, all channels,
+// FP32, [channel][X][Y][Z] and [channel][X][Y][Z/2+1] (NoMini:177-199).
+struct Protein {
+  Context* ctx = nullptr;
+  int nch = 0;  // num grids: vdW channels + 1 elec (ReadGrids.cpp:27)
+  int N = 0;
+  float h = 0.f;
+  float mid[3] = {0, 0, 0};
+  float len[3] = {0, 0, 0};
+  std::vector<float> radii;  // nch - 1 probe radii
+  DevBuf grid;               // float [nch][G]
+  DevBuf spec;               // float2 [nch][H]
+  size_t G() const { return (size_t)N * N * N; }
+  size_t H() const { return (size_t)N * N * (N / 2 + 1); }
+  // gridMin = mid - len / 2 evaluated in float, then widened (NoMini:149-151).
+  double grid_min(int k) const { return (double)(mid[k] - len[k] / 2); }
+};
+
+// Ligand with its channel assignment against a protein's probe radii
+// (GetIdxOfAtomsForVdwRadius.cpp:4-35).
+struct Ligand {
+  int natom = 0;
+  std::vector<float> charge, eps, radius;
+  int num_vdw_used = 0;
+  std::vector<int> idx_vdw_used;               // used probe indices, ascending
+  std::vector<std::vector<int>> atoms_of_radius;  // per probe radius, ascending atom indices
+  std::vector<int> atom_channel;               // per atom: ligand vdW channel index
+  // kernel-side tables for this ligand alone (list_off = w_off = 0)
+  LigandDesc desc{};
+  std::vector<int> chan_atoms;
+  std::vector<float> wv, wq;
+  int nch() const { return num_vdw_used + 1; }
+};
+
+std::unique_ptr<Ligand> make_ligand(const Protein& p, int natom, const float* charge, const float* eps,
+                                    const float* radius);
+
+// One docking job: a ligand, its conformers and per-conformer quaternions.
+struct Job {
+  const Ligand* lig = nullptr;
+  int nconf = 0;
+  const float* coords = nullptr;  // [nconf][natom][3]
+  int nquat = 0;
+  const float* quats = nullptr;   // [nconf][nquat][4]
+  bool shared_quats = false;      // one quaternion set [nquat][4] for every conformer
+  // Pose range [pose_begin, pose_end) in [conformer][quaternion] order.
+  size_t pose_begin = 0;
+  size_t pose_end = ~(size_t)0;
+  size_t npose() const {
+    const size_t n = (size_t)nconf * nquat;
+    const size_t e = pose_end < n ? pose_end : n;
+    return e > pose_begin ? e - pose_begin : 0;
+  }
+};
+
+// Per-pose result over a job's pose range, [conformer][quaternion] order.
+struct PoseResults {
+  std::vector<float> energy;   // +inf when the window is empty
+  std::vector<int> index;      // [n][3] best translation (i, j, k); -1 when empty
+  std::vector<float> mincoor;  // [n][3]
+};
+
+enum Engine { kEngineReference = 0, kEngineFused = 1 };
+
+struct StageTimes {
+  double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::map<int, DevBuf*> twiddles;
+  // options
+  size_t ws_budget = 8ull << 30;  // bytes of per-batch working arrays
+  int ref_batch = 50;             // poses per reference-dataflow batch (NoMini:208)
+  int fused_chunk = 0;            // 0 = from budget
+  int mid_group = 16;             // poses sharing one LDS copy of a protein tile
+  bool use_graph = true;
+  bool time_stages = false;
+  StageTimes stage_times;
+  // workspace
+  DevBuf d_slots, d_ligs, d_chan, d_wv, d_wq, d_coords, d_pmap;
+  DevBuf d_rot, d_ag, d_meta, d_keys;
+  DevBuf d_A, d_T, d_L, d_Lhat, d_S, d_E;
+  DevBuf d_stage_slots;
+  // pinned host staging
+  void* h_keys = nullptr;
+  void* h_meta = nullptr;
+  size_t h_cap = 0;
+
+  explicit Context(int dev);
+  ~Context();
+  const float2* twiddle(int N);
+  void ensure_host(size_t nslots);
+};
+
+void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const float* len, const float* radii,
+                    const float* values);
+
+// Runs every pose of every job.  Results are appended per job in order.
+void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,
+                std::vector<PoseResults>& out);
+
+// Decode helpers shared with tests.
+float key_energy(unsigned long long key);
+unsigned key_index(unsigned long long key);
+
+}  // namespace cdk

@@ -1,0 +1,39 @@
+// Host-side input producers (see host_inputs.cpp).
+#pragma once
+#include <cstdint>
+#include <random>
+#include <string>
+#include <vector>
+
+namespace cdk {
+
+struct Xoshiro {
+  uint64_t s[4];
+  explicit Xoshiro(uint64_t seed);
+  uint64_t next();
+  double uniform();
+  double normal();
+};
+
+extern const float kProbeRadii[8];
+
+void quaternion_uniform_sampling(std::mt19937_64& gen, int n, float* out);
+void synth_protein(uint64_t seed, int natom, float box, int n, float h, float* values, float* radii, float* mid3,
+                   float* len3);
+void synth_ligand(uint64_t seed, int natom, float* coords, float* charge, float* eps, float* radius);
+void synth_conformers(uint64_t seed, int natom, const float* coords, int nconf, float* out);
+
+struct GridHeader {
+  int num_grids = 0;
+  int dims[3] = {0, 0, 0};
+  float mid[3] = {0, 0, 0};
+  float len[3] = {0, 0, 0};
+  float spacing = 0, restraint_k = 0;
+  std::vector<float> radii;
+};
+
+void read_grid_header(const std::string& path, GridHeader& g);
+void read_grids(const std::string& path, GridHeader& g, std::vector<float>& values);
+void write_grids(const std::string& path, const GridHeader& g, const float* values);
+
+}  // namespace cdk

@@ -1,0 +1,882 @@
+// gfx950 kernels of the FFT translational-rotational pose search.
+//
+// Two pipelines share these kernels:
+//   * the reference-dataflow ("plain") pipeline: full-grid spread -> batched
+//     3-D R2C -> ConjMult -> SumGrids -> 3-D C2R -> windowed argmin, one
+//     materialised array per stage, exactly the stage boundaries of
+//     TranRotaConfSearchNoMini.cu:389-464;
+//   * the fused, pruned pipeline: prep -> (spread + z/y forward FFT of the
+//     occupied x-planes only) -> (x FFT + conj-multiply + channel sum +
+//     inverse x FFT, protein spectra staged in LDS and reused across poses)
+//     -> (inverse y FFT + z C2R of the window rows only + argmin).
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+#include "cdk_types.hpp"
+#include "fft_lds.hpp"
+#include "kernels.hpp"
+
+namespace cdk {
+
+// ---------------------------------------------------------------------------
+// Plans per grid edge.  The tuned radix list is selected at run time among
+// the variants compiled here (see tune_*).
+// ---------------------------------------------------------------------------
+using P32a = Plan<32, 8, 4>;
+using P48a = Plan<48, 16, 3>;
+using P48b = Plan<48, 4, 4, 3>;
+using P64a = Plan<64, 4, 4, 4>;
+using P64b = Plan<64, 8, 8>;
+using P96a = Plan<96, 8, 4, 3>;
+using P96b = Plan<96, 16, 2, 3>;
+using P128a = Plan<128, 8, 16>;
+using P128b = Plan<128, 4, 4, 8>;
+
+// ---------------------------------------------------------------------------
+// Ordered 64-bit argmin key: high word = float bits mapped to unsigned order,
+// low word = linear translation index.  min(key) = lowest energy, and among
+// equal energies the lowest (i, j, k) in x-major order, which is the first
+// strict minimum of the reference's scan (TranRotaConfSearchNoMini.cu:441-464).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long energy_key(float e, unsigned lin) {
+  unsigned u = __float_as_uint(e);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((unsigned long long)u << 32) | (unsigned long long)lin;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffu), off, 64);
+    unsigned hi = __shfl_xor((unsigned)(v >> 32), off, 64);
+    unsigned long long o = ((unsigned long long)hi << 32) | lo;
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Block-wide min of a key, then one atomicMin into `dst` (order independent,
+// so the result does not depend on workgroup scheduling).
+__device__ __forceinline__ void block_min_commit(unsigned long long v, unsigned long long* dst) {
+  __shared__ unsigned long long red[kBlock / 64];
+  v = wave_min_u64(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = red[i] < m ? red[i] : m;
+    if (m != ~0ull) atomicMin(dst, m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rotate.cpp:4-27 with QuaternionMultiply.cpp:3-11 term order, FP32, no FMA
+// contraction, so the rotated coordinates are bit-identical to the host code.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rotate_exact(const float* q, float px, float py, float pz, float* o) {
+#pragma clang fp contract(off)
+  const float zero = 0.0f;
+  const float t0 = q[0] * zero - q[1] * px - q[2] * py - q[3] * pz;
+  const float t1 = q[0] * px + q[1] * zero + q[2] * pz - q[3] * py;
+  const float t2 = q[0] * py - q[1] * pz + q[2] * zero + q[3] * px;
+  const float t3 = q[0] * pz + q[1] * py - q[2] * px + q[3] * zero;
+  const float c0 = q[0], c1 = -q[1], c2 = -q[2], c3 = -q[3];
+  o[0] = t0 * c1 + t1 * c0 + t2 * c3 - t3 * c2;
+  o[1] = t0 * c2 - t1 * c3 + t2 * c0 + t3 * c1;
+  o[2] = t0 * c3 + t1 * c2 - t2 * c1 + t3 * c0;
+}
+
+// FillLigandGrid.cpp:24-35: grid index by truncation and the double ratio,
+// where `xIndex * spacing` is a float product.
+__device__ __forceinline__ void grid_place(float x, float mn, float h, int& idx, double& f) {
+#pragma clang fp contract(off)
+  const double d = (double)x - (double)mn;
+  idx = (int)(d / (double)h);
+  const float ih = (float)idx * h;
+  f = (d - (double)ih) / (double)h;
+}
+
+// (value, index) lexicographic min/max, reproducing the first-occurrence
+// semantics of GetMinCoors.cpp / GetMaxCoors.cpp (strict comparisons).
+__device__ __forceinline__ void lex_min(float& v, int& i, float ov, int oi) {
+  if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+__device__ __forceinline__ void lex_max(float& v, int& i, float ov, int oi) {
+  if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+
+// One wavefront per pose: rotate, AABB, window, per-atom grid placement.
+__global__ __launch_bounds__(kPrepBlock) void k_prep(const PoseSlot* __restrict__ slots, int nslots,
+                                                     const LigandDesc* __restrict__ ligs,
+                                                     const float* __restrict__ coords, int N, float h, int astride,
+                                                     int boxmax, float* __restrict__ rot, AtomGrid* __restrict__ ag,
+                                                     RotMeta* __restrict__ meta,
+                                                     unsigned long long* __restrict__ keys) {
+  const int s = blockIdx.x;
+  if (s >= nslots) return;
+  const PoseSlot ps = slots[s];
+  const int na = ligs[ps.lig].natom;
+  const float* c = coords + ps.coord_off;
+  const int lane = threadIdx.x;
+  float mn[3], mx[3];
+  int mni[3], mxi[3];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = __int_as_float(0x7f800000);
+    mx[k] = -__int_as_float(0x7f800000);
+    mni[k] = INT_MAX;
+    mxi[k] = INT_MAX;
+  }
+  for (int a = lane; a < na; a += kPrepBlock) {
+    float o[3];
+    rotate_exact(ps.q, c[a * 3 + 0], c[a * 3 + 1], c[a * 3 + 2], o);
+    float* r = rot + ((size_t)s * astride + a) * 3;
+    r[0] = o[0];
+    r[1] = o[1];
+    r[2] = o[2];
+    for (int k = 0; k < 3; ++k) {
+      lex_min(mn[k], mni[k], o[k], a);
+      lex_max(mx[k], mxi[k], o[k], a);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    for (int k = 0; k < 3; ++k) {
+      float ov = __shfl_xor(mn[k], off, 64);
+      int oi = __shfl_xor(mni[k], off, 64);
+      lex_min(mn[k], mni[k], ov, oi);
+      ov = __shfl_xor(mx[k], off, 64);
+      oi = __shfl_xor(mxi[k], off, 64);
+      lex_max(mx[k], mxi[k], ov, oi);
+    }
+  }
+  // Every lane now holds the pose's min/max.  Place atoms (needs the min).
+  int bmax[3] = {0, 0, 0};
+  for (int a = lane; a < na; a += kPrepBlock) {
+    const float* r = rot + ((size_t)s * astride + a) * 3;
+    AtomGrid g;
+    grid_place(r[0], mn[0], h, g.ix, g.fx);
+    grid_place(r[1], mn[1], h, g.iy, g.fy);
+    grid_place(r[2], mn[2], h, g.iz, g.fz);
+    g.pad = 0;
+    ag[(size_t)s * astride + a] = g;
+    bmax[0] = max(bmax[0], g.ix);
+    bmax[1] = max(bmax[1], g.iy);
+    bmax[2] = max(bmax[2], g.iz);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 3; ++k) bmax[k] = max(bmax[k], __shfl_xor(bmax[k], off, 64));
+  if (lane == 0) {
+    RotMeta m;
+    int ok = na > 0;
+    for (int k = 0; k < 3; ++k) {
+      m.minc[k] = mn[k];
+      const float len = mx[k] - mn[k];
+      m.len[k] = len;
+      m.box[k] = bmax[k] + 2;
+      m.win[k] = N - (int)__fdiv_rn(len, h) - 2;
+      ok = ok && m.win[k] > 0 && m.box[k] <= N && m.box[k] <= boxmax;
+    }
+    m.valid = ok;
+    m.pad = 0;
+    meta[s] = m;
+    keys[s] = ~0ull;
+  }
+}
+
+// Per-atom weight of ligand channel c: charge for the last (elec) channel,
+// sqrt(epsilon) otherwise (FillLigandGrid.cpp:45, :97).
+__device__ __forceinline__ float channel_weight(const LigandDesc& L, int c, int a, const float* wv,
+                                                const float* wq) {
+  return (c == L.nch - 1) ? wq[L.w_off + a] : wv[L.w_off + a];
+}
+
+// One trilinear contribution, FillLigandGrid.cpp:45-66:
+//   w * X * Y * Z evaluated left to right in double, added to the float cell.
+__device__ __forceinline__ float accum_corner(float acc, double wX, double fy, double fz, int dy, int dz) {
+#pragma clang fp contract(off)
+  const double Y = dy ? fy : (1.0 - fy);
+  const double Z = dz ? fz : (1.0 - fz);
+  const double p = (wX * Y) * Z;
+  return (float)((double)acc + p);
+}
+
+__device__ __forceinline__ double x_factor(float w, double fx, int dx) {
+#pragma clang fp contract(off)
+  return (double)w * (dx ? fx : (1.0 - fx));
+}
+
+// ---------------------------------------------------------------------------
+// Reference-dataflow spread: every cell of the [slot][c][X][Y][Z] grid is
+// written (zero outside the ligand box), like memset + FillLigandGrid
+// (TranRotaConfSearchNoMini.cu:389-409).  Each cell gathers the channel's
+// atoms in ascending order, so the float accumulation order per cell is the
+// reference's.
+// ---------------------------------------------------------------------------
+struct SpreadAtom {
+  int ix, iy, iz;
+  float w;
+  double fx, fy, fz;
+};
+
+__global__ __launch_bounds__(kBlock) void k_spread_full(const PoseSlot* __restrict__ slots,
+                                                        const LigandDesc* __restrict__ ligs,
+                                                        const int* __restrict__ chan_atoms,
+                                                        const float* __restrict__ wv, const float* __restrict__ wq,
+                                                        const AtomGrid* __restrict__ ag, int astride,
+                                                        const RotMeta* __restrict__ meta, int N, int C,
+                                                        float* __restrict__ out) {
+  __shared__ SpreadAtom sa[kMaxAtoms];
+  const int s = blockIdx.z, c = blockIdx.y;
+  const size_t G = (size_t)N * N * N;
+  float* o = out + ((size_t)s * C + c) * G;
+  const size_t p0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const RotMeta m = meta[s];
+  const LigandDesc& L = ligs[slots[s].lig];
+  const bool active = m.valid && c < L.nch;
+  int nc = 0;
+  if (active) {
+    const int* ca = chan_atoms + L.list_off + L.ch_start[c];
+    nc = L.ch_start[c + 1] - L.ch_start[c];
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+      const int a = ca[i];
+      const AtomGrid g = ag[(size_t)s * astride + a];
+      SpreadAtom t;
+      t.ix = g.ix;
+      t.iy = g.iy;
+      t.iz = g.iz;
+      t.w = channel_weight(L, c, a, wv, wq);
+      t.fx = g.fx;
+      t.fy = g.fy;
+      t.fz = g.fz;
+      sa[i] = t;
+    }
+  }
+  __syncthreads();
+  if (p0 >= G) return;
+  const int x = (int)(p0 / ((size_t)N * N));
+  const int y = (int)((p0 / N) % N);
+  const int z = (int)(p0 % N);
+  float acc = 0.0f;
+  if (active && x < m.box[0] && y < m.box[1] && z < m.box[2]) {
+    for (int i = 0; i < nc; ++i) {
+      const int dx = x - sa[i].ix, dy = y - sa[i].iy, dz = z - sa[i].iz;
+      if ((unsigned)dx <= 1u && (unsigned)dy <= 1u && (unsigned)dz <= 1u)
+        acc = accum_corner(acc, x_factor(sa[i].w, sa[i].fx, dx), sa[i].fy, sa[i].fz, dy, dz);
+    }
+  }
+  o[p0] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// 3-D transform passes.  Grids are [X][Y][Z] with Z fastest; spectra are
+// [X][Y][Z/2+1] (cuFFT's R2C layout, TranRotaConfSearchNoMini.cu:158-174).
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* tw, const float2* __restrict__ twg) {
+  for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
+}
+
+// Split packed complex rows (row 2p in .x, row 2p+1 in .y, transformed) into
+// two Hermitian half spectra rows of length ZH.  Rows >= nrow are zero.
+template <int N>
+__device__ __forceinline__ void unpack_rows(const float2* __restrict__ r, float2* __restrict__ u, int nrow) {
+  constexpr int ZH = N / 2 + 1;
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) {
+    const int y = t / ZH, k = t - (t / ZH) * ZH;
+    float2 v = make_float2(0.f, 0.f);
+    if (y < nrow) {
+      const int p = y >> 1;
+      const float2 zk = r[p * N + k];
+      const float2 zm = r[p * N + (k == 0 ? 0 : N - k)];
+      if ((y & 1) == 0)
+        v = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      else
+        v = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+    u[t] = v;
+  }
+}
+
+// Pack two Hermitian half-spectrum rows (2p, 2p+1) into one full complex row
+// A + iB for a complex inverse transform (C2R two-for-one).  The imaginary
+// parts of the DC and Nyquist bins are dropped, as a C2R transform does.
+template <int N>
+__device__ __forceinline__ void pack_rows_c2r(const float2* __restrict__ r, float2* __restrict__ u, int npair) {
+  constexpr int ZH = N / 2 + 1;
+  for (int t = threadIdx.x; t < npair * N; t += blockDim.x) {
+    const int p = t / N, k = t - (t / N) * N;
+    float2 A, B;
+    if (k <= N / 2) {
+      A = r[(2 * p) * ZH + k];
+      B = r[(2 * p + 1) * ZH + k];
+      if (k == 0 || k == N / 2) {
+        A.y = 0.f;
+        B.y = 0.f;
+      }
+    } else {
+      A = r[(2 * p) * ZH + (N - k)];
+      B = r[(2 * p + 1) * ZH + (N - k)];
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    u[t] = make_float2(A.x - B.y, A.y + B.x);
+  }
+}
+
+// Forward plane pass: real plane [Y][Z] -> z R2C of all rows -> y FFT of all
+// half-spectrum columns -> [Y][ZH].
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_r2c_planes(const float* __restrict__ in, float2* __restrict__ out,
+                                                       const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  __shared__ float2 sA[N * ZH], sB[N * ZH], tw[N];
+  load_twiddles<N>(tw, twg);
+  const size_t plane = blockIdx.x;
+  const float* ip = in + plane * N * N;
+  for (int t = threadIdx.x; t < N * N / 2; t += blockDim.x) {
+    const int p = t / N, z = t - (t / N) * N;
+    sA[t] = make_float2(ip[(2 * p) * N + z], ip[(2 * p + 1) * N + z]);
+  }
+  __syncthreads();
+  float2* r = lds_fft<P, -1, false>(sA, sB, N / 2, 1, N, tw);
+  float2* u = (r == sA) ? sB : sA;
+  unpack_rows<N>(r, u, N);
+  __syncthreads();
+  float2* res = lds_fft<P, -1, true, ZH>(u, r, ZH, ZH, 1, tw);
+  float2* op = out + plane * N * ZH;
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op[t] = res[t];
+}
+
+// x-line pass over [nb][X][Y][ZH]: one workgroup per (grid, ky) transforms
+// the ZH lines of that ky in place.
+template <class P, int S>
+__global__ __launch_bounds__(kBlock) void k_xlines(float2* __restrict__ data, const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  __shared__ float2 sA[N * ZH], sB[N * ZH], tw[N];
+  load_twiddles<N>(tw, twg);
+  const size_t b = blockIdx.x / N;
+  const int ky = blockIdx.x - (int)b * N;
+  float2* base = data + b * N * N * ZH + (size_t)ky * ZH;
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) {
+    const int x = t / ZH, k = t - (t / ZH) * ZH;
+    sA[t] = base[(size_t)x * N * ZH + k];
+  }
+  __syncthreads();
+  float2* res = lds_fft<P, S, true, ZH>(sA, sB, ZH, ZH, 1, tw);
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) {
+    const int x = t / ZH, k = t - (t / ZH) * ZH;
+    base[(size_t)x * N * ZH + k] = res[t];
+  }
+}
+
+// Inverse plane pass: [Y][ZH] -> y inverse FFT -> z C2R -> real plane [Y][Z].
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_c2r_planes(const float2* __restrict__ in, float* __restrict__ out,
+                                                       const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  __shared__ float2 sA[N * ZH], sB[N * ZH], tw[N];
+  load_twiddles<N>(tw, twg);
+  const size_t plane = blockIdx.x;
+  const float2* ip = in + plane * N * ZH;
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) sA[t] = ip[t];
+  __syncthreads();
+  float2* r = lds_fft<P, 1, true, ZH>(sA, sB, ZH, ZH, 1, tw);
+  float2* u = (r == sA) ? sB : sA;
+  pack_rows_c2r<N>(r, u, N / 2);
+  __syncthreads();
+  float2* res = lds_fft<P, 1, false>(u, r, N / 2, 1, N, tw);
+  float* op = out + plane * N * N;
+  for (int t = threadIdx.x; t < N * N / 2; t += blockDim.x) {
+    const int p = t / N, z = t - (t / N) * N;
+    const float2 v = res[t];
+    op[(2 * p) * N + z] = v.x;
+    op[(2 * p + 1) * N + z] = v.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernel.cu:5-14 ConjMult: L <- conj(L) * P, protein broadcast over poses,
+// with the ligand's used-channel map and a bounds guard.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_conj_mult(float2* __restrict__ L, const float2* __restrict__ P,
+                                                      const int* __restrict__ pmap, int C, long long H,
+                                                      long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long h = i % H;
+  const int c = (int)((i / H) % C);
+  const float2 l = L[i];
+  const float2 p = P[(long long)pmap[c] * H + h];
+  L[i] = make_float2(l.x * p.x + l.y * p.y, l.x * p.y - l.y * p.x);
+}
+
+// kernel.cu:16-30 SumGrids: S[q] = (sum_c L[q][c]) / sqrtf(G), channels in
+// ascending order, float accumulation.
+__global__ __launch_bounds__(kBlock) void k_sum_grids(const float2* __restrict__ L, float2* __restrict__ S, int C,
+                                                      long long H, long long total, float G) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long q = i / H, h = i - q * H;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int c = 0; c < C; ++c) {
+    const float2 v = L[(q * C + c) * H + h];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  const float sg = __fsqrt_rn(G);
+  S[i] = make_float2(__fdiv_rn(acc.x, sg), __fdiv_rn(acc.y, sg));
+}
+
+// TranRotaConfSearchNoMini.cu:441-464 on the device: E = f / sqrt(G) in
+// double, stored as float, minimum over the window with first-index ties.
+__global__ __launch_bounds__(kBlock) void k_argmin_full(const float* __restrict__ E, const RotMeta* __restrict__ meta,
+                                                        unsigned long long* __restrict__ keys, int N) {
+  const int s = blockIdx.y, x = blockIdx.x;
+  const RotMeta m = meta[s];
+  if (!m.valid || x >= m.win[0]) return;
+  const size_t G = (size_t)N * N * N;
+  const double sg = sqrt((double)G);
+  const float* e = E + (size_t)s * G + (size_t)x * N * N;
+  const int wy = m.win[1], wz = m.win[2];
+  unsigned long long best = ~0ull;
+  for (int t = threadIdx.x; t < wy * wz; t += blockDim.x) {
+    const int y = t / wz, z = t - (t / wz) * wz;
+    const float v = (float)((double)e[y * N + z] / sg);
+    const unsigned long long k = energy_key(v, (unsigned)((x * N + y) * N + z));
+    best = k < best ? k : best;
+  }
+  block_min_commit(best, keys + s);
+}
+
+// ---------------------------------------------------------------------------
+// Fused, pruned pipeline
+// ---------------------------------------------------------------------------
+struct PlaneAtom {
+  int iy, iz;
+  double wx, fy, fz;
+};
+
+// Stage A: one workgroup per (pose, ligand channel, occupied x-plane).
+// Spreads the plane in LDS (FillLigandGrid arithmetic), z R2C of the occupied
+// rows, y FFT of all columns, writes the plane spectrum
+// A[slot][c][x][Y][ZH].
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restrict__ slots,
+                                                       const LigandDesc* __restrict__ ligs,
+                                                       const int* __restrict__ chan_atoms,
+                                                       const float* __restrict__ wv, const float* __restrict__ wq,
+                                                       const AtomGrid* __restrict__ ag, int astride,
+                                                       const RotMeta* __restrict__ meta, int cmax, int boxmax,
+                                                       float2* __restrict__ A, const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  __shared__ float2 sA[N * ZH], sB[N * ZH], tw[N];
+  __shared__ PlaneAtom pl[kMaxAtoms];
+  __shared__ int npl;
+  const int x = blockIdx.x, c = blockIdx.y, s = blockIdx.z;
+  const RotMeta m = meta[s];
+  if (!m.valid || x >= m.box[0]) return;
+  const PoseSlot ps = slots[s];
+  const LigandDesc& L = ligs[ps.lig];
+  if (c >= L.nch) return;
+  load_twiddles<N>(tw, twg);
+  // Ordered compaction of the channel's atoms that touch plane x.
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const int* ca = chan_atoms + L.list_off + L.ch_start[c];
+    const int nc = L.ch_start[c + 1] - L.ch_start[c];
+    int cnt = 0;
+    for (int b = 0; b < nc; b += 64) {
+      const int i = b + lane;
+      int a = 0, dx = -1;
+      AtomGrid g;
+      if (i < nc) {
+        a = ca[i];
+        g = ag[(size_t)s * astride + a];
+        dx = x - g.ix;
+      }
+      const bool take = (i < nc) && (dx == 0 || dx == 1);
+      const unsigned long long msk = __ballot(take);
+      if (take) {
+        const int pos = cnt + __popcll(msk & ((1ull << lane) - 1ull));
+        PlaneAtom t;
+        t.iy = g.iy;
+        t.iz = g.iz;
+        t.wx = x_factor(channel_weight(L, c, a, wv, wq), g.fx, dx);
+        t.fy = g.fy;
+        t.fz = g.fz;
+        pl[pos] = t;
+      }
+      cnt += __popcll(msk);
+    }
+    if (lane == 0) npl = cnt;
+  }
+  __syncthreads();
+  const int by = m.box[1], bz = m.box[2];
+  const int npair = (by + 1) >> 1;
+  const int n = npl;
+  for (int t = threadIdx.x; t < npair * 2 * N; t += blockDim.x) {
+    const int y = t / N, z = t - (t / N) * N;
+    float acc = 0.f;
+    if (y < by && z < bz) {
+      for (int i = 0; i < n; ++i) {
+        const int dy = y - pl[i].iy, dz = z - pl[i].iz;
+        if ((unsigned)dy <= 1u && (unsigned)dz <= 1u) acc = accum_corner(acc, pl[i].wx, pl[i].fy, pl[i].fz, dy, dz);
+      }
+    }
+    float* cell = reinterpret_cast<float*>(&sA[(y >> 1) * N + z]);
+    cell[y & 1] = acc;
+  }
+  __syncthreads();
+  float2* r = lds_fft<P, -1, false>(sA, sB, npair, 1, N, tw);
+  float2* u = (r == sA) ? sB : sA;
+  unpack_rows<N>(r, u, 2 * npair);
+  __syncthreads();
+  float2* res = lds_fft<P, -1, true, ZH>(u, r, ZH, ZH, 1, tw);
+  float2* op = A + (((size_t)s * cmax + c) * boxmax + x) * (size_t)(N * ZH);
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op[t] = res[t];
+}
+
+// Stage B: one workgroup per (ky, kz tile, group of poses).  The protein
+// spectra of the tile (all protein channels) are staged in LDS once and reused
+// for every pose of the group.  Per pose: x FFT of each used channel (input
+// non-zero for x < box), accumulate conj(L)*P over channels in registers,
+// scale by 1/G, inverse x FFT, keep x' < window.  Writes T[slot][x'][Y][ZH].
+template <class P, int TK>
+__global__ __launch_bounds__(kBlock) void k_mid(const PoseSlot* __restrict__ slots, const LigandDesc* __restrict__ ligs,
+                                                const RotMeta* __restrict__ meta, int nslots, int group,
+                                                const float2* __restrict__ A, int cmax, int boxmax,
+                                                const float2* __restrict__ Phat, int npch,
+                                                float2* __restrict__ T, float invG,
+                                                const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  constexpr int E = N * TK;
+  constexpr int PER = (E + kBlock - 1) / kBlock;
+  extern __shared__ float2 Pt[];  // [npch][N][TK]
+  __shared__ float2 sA[E], sB[E], tw[N];
+  const int ky = blockIdx.x;
+  const int kz0 = blockIdx.y * TK;
+  const int nk = min(TK, ZH - kz0);
+  load_twiddles<N>(tw, twg);
+  for (int t = threadIdx.x; t < npch * E; t += blockDim.x) {
+    const int pc = t / E, e = t - (t / E) * E;
+    const int kx = e / TK, j = e - (e / TK) * TK;
+    Pt[t] = (j < nk) ? Phat[(size_t)pc * N * N * ZH + ((size_t)kx * N + ky) * ZH + kz0 + j] : make_float2(0.f, 0.f);
+  }
+  const int s0 = blockIdx.z * group;
+  const int s1 = min(nslots, s0 + group);
+  const size_t plane = (size_t)N * ZH;
+  for (int s = s0; s < s1; ++s) {
+    const RotMeta m = meta[s];
+    if (!m.valid) continue;
+    const LigandDesc& L = ligs[slots[s].lig];
+    const int bx = min(m.box[0], boxmax);
+    float2 acc[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) acc[i] = make_float2(0.f, 0.f);
+    for (int c = 0; c < L.nch; ++c) {
+      const float2* a = A + (((size_t)s * cmax + c) * boxmax) * plane + (size_t)ky * ZH + kz0;
+      __syncthreads();
+      for (int t = threadIdx.x; t < E; t += blockDim.x) {
+        const int x = t / TK, j = t - (t / TK) * TK;
+        sA[t] = (x < bx && j < nk) ? a[(size_t)x * plane + j] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+      const float2* r = lds_fft<P, -1, true, TK>(sA, sB, TK, TK, 1, tw);
+      const float2* pt = Pt + L.pmap[c] * E;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * kBlock;
+        if (e < E) {
+          const float2 l = r[e], p = pt[e];
+          acc[i].x += l.x * p.x + l.y * p.y;
+          acc[i].y += l.x * p.y - l.y * p.x;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = threadIdx.x + i * kBlock;
+      if (e < E) sA[e] = make_float2(acc[i].x * invG, acc[i].y * invG);
+    }
+    __syncthreads();
+    const float2* r = lds_fft<P, 1, true, TK>(sA, sB, TK, TK, 1, tw);
+    float2* tp = T + (size_t)s * N * plane + (size_t)ky * ZH + kz0;
+    const int wx = m.win[0];
+    for (int t = threadIdx.x; t < E; t += blockDim.x) {
+      const int x = t / TK, j = t - (t / TK) * TK;
+      if (x < wx && j < nk) tp[(size_t)x * plane + j] = r[t];
+    }
+  }
+}
+
+// Stage C: one workgroup per (pose, window x').  Inverse y FFT of the plane,
+// z C2R of the window rows only, argmin over the (y', z') window.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_inv_argmin(const RotMeta* __restrict__ meta,
+                                                       const float2* __restrict__ T,
+                                                       unsigned long long* __restrict__ keys,
+                                                       const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  __shared__ float2 sA[N * ZH], sB[N * ZH], tw[N];
+  const int xp = blockIdx.x, s = blockIdx.y;
+  const RotMeta m = meta[s];
+  if (!m.valid || xp >= m.win[0]) return;
+  load_twiddles<N>(tw, twg);
+  const float2* ip = T + ((size_t)s * N + xp) * (size_t)(N * ZH);
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) sA[t] = ip[t];
+  __syncthreads();
+  float2* r = lds_fft<P, 1, true, ZH>(sA, sB, ZH, ZH, 1, tw);
+  float2* u = (r == sA) ? sB : sA;
+  const int wy = m.win[1], wz = m.win[2];
+  const int npair = (wy + 1) >> 1;
+  pack_rows_c2r<N>(r, u, npair);
+  __syncthreads();
+  const float2* res = lds_fft<P, 1, false>(u, r, npair, 1, N, tw);
+  unsigned long long best = ~0ull;
+  for (int t = threadIdx.x; t < wy * wz; t += blockDim.x) {
+    const int y = t / wz, z = t - (t / wz) * wz;
+    const float2 v2 = res[(y >> 1) * N + z];
+    const float v = (y & 1) ? v2.y : v2.x;
+    const unsigned long long k = energy_key(v, (unsigned)((xp * N + y) * N + z));
+    best = k < best ? k : best;
+  }
+  block_min_commit(best, keys + s);
+}
+
+// Rotate.cpp:4-27 for every (quaternion, atom): out[q][a][3].
+__global__ __launch_bounds__(kBlock) void k_rotate(int nq, int natom, const float* __restrict__ quats,
+                                                   const float* __restrict__ coords, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq * natom) return;
+  const int q = i / natom, a = i - (i / natom) * natom;
+  float qq[4] = {quats[q * 4 + 0], quats[q * 4 + 1], quats[q * 4 + 2], quats[q * 4 + 3]};
+  float o[3];
+  rotate_exact(qq, coords[a * 3 + 0], coords[a * 3 + 1], coords[a * 3 + 2], o);
+  out[(size_t)i * 3 + 0] = o[0];
+  out[(size_t)i * 3 + 1] = o[1];
+  out[(size_t)i * 3 + 2] = o[2];
+}
+
+// ---------------------------------------------------------------------------
+// FillLigandGrid.cpp:7-120 as a standalone device operation on caller-given
+// rotated coordinates and minimum corners: placement per (pose, atom), then a
+// per-cell gather that adds each channel's atoms in ascending order on top of
+// the existing cell value (the reference accumulates with +=).  Grid layout
+// [pose][channel][X][Y][Z], channel index pose*C + c (FillLigandGrid.cpp:38).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_place(int nq, int natom, const float* __restrict__ coors,
+                                                  const float* __restrict__ mincoors, float h,
+                                                  AtomGrid* __restrict__ ag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq * natom) return;
+  const int q = i / natom;
+  AtomGrid g;
+  grid_place(coors[(size_t)i * 3 + 0], mincoors[q * 3 + 0], h, g.ix, g.fx);
+  grid_place(coors[(size_t)i * 3 + 1], mincoors[q * 3 + 1], h, g.iy, g.fy);
+  grid_place(coors[(size_t)i * 3 + 2], mincoors[q * 3 + 2], h, g.iz, g.fz);
+  g.pad = 0;
+  ag[i] = g;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_generic(int natom, const AtomGrid* __restrict__ ag,
+                                                         const int* __restrict__ ch_start,
+                                                         const int* __restrict__ chan_atoms,
+                                                         const float* __restrict__ wv, const float* __restrict__ wq,
+                                                         int C, int X, int Y, int Z, float* __restrict__ grid) {
+  __shared__ SpreadAtom sa[kMaxAtoms];
+  const int q = blockIdx.z, c = blockIdx.y;
+  const size_t G = (size_t)X * Y * Z;
+  const int b0 = ch_start[c], nc = ch_start[c + 1] - ch_start[c];
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    const int a = chan_atoms[b0 + i];
+    const AtomGrid g = ag[(size_t)q * natom + a];
+    SpreadAtom t;
+    t.ix = g.ix;
+    t.iy = g.iy;
+    t.iz = g.iz;
+    t.w = (c == C - 1) ? wq[a] : wv[a];
+    t.fx = g.fx;
+    t.fy = g.fy;
+    t.fz = g.fz;
+    sa[i] = t;
+  }
+  __syncthreads();
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= G) return;
+  const int x = (int)(p / ((size_t)Y * Z));
+  const int y = (int)((p / Z) % Y);
+  const int z = (int)(p % Z);
+  float* cell = grid + ((size_t)q * C + c) * G + p;
+  float acc = *cell;
+  bool touched = false;
+  for (int i = 0; i < nc; ++i) {
+    const int dx = x - sa[i].ix, dy = y - sa[i].iy, dz = z - sa[i].iz;
+    if ((unsigned)dx <= 1u && (unsigned)dy <= 1u && (unsigned)dz <= 1u) {
+      acc = accum_corner(acc, x_factor(sa[i].w, sa[i].fx, dx), sa[i].fy, sa[i].fz, dy, dz);
+      touched = true;
+    }
+  }
+  if (touched) *cell = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+#define CDK_DISPATCH_N(N, ...)                         \
+  switch (N) {                                         \
+    case 32: { using PL = P32a; __VA_ARGS__ } break;   \
+    case 48: { using PL = P48a; __VA_ARGS__ } break;   \
+    case 64: { using PL = P64a; __VA_ARGS__ } break;   \
+    case 96: { using PL = P96a; __VA_ARGS__ } break;   \
+    case 128: { using PL = P128a; __VA_ARGS__ } break; \
+    default: return hipErrorInvalidValue;              \
+  }
+
+bool grid_edge_supported(int N) { return N == 32 || N == 48 || N == 64 || N == 96 || N == 128; }
+
+hipError_t launch_prep(hipStream_t st, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                       const float* coords, int N, float h, int astride, int boxmax, float* rot, AtomGrid* ag,
+                       RotMeta* meta, unsigned long long* keys) {
+  if (nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prep, dim3(nslots), dim3(kPrepBlock), 0, st, slots, nslots, ligs, coords, N, h, astride, boxmax,
+                     rot, ag, meta, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_spread_full(hipStream_t st, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                              const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
+                              int astride, const RotMeta* meta, int N, int C, float* out) {
+  if (nslots <= 0) return hipSuccess;
+  const size_t G = (size_t)N * N * N;
+  hipLaunchKernelGGL(k_spread_full, dim3(ceil_div(G, kBlock), C, nslots), dim3(kBlock), 0, st, slots, ligs,
+                     chan_atoms, wv, wq, ag, astride, meta, N, C, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_r2c_3d(hipStream_t st, int N, const float* in, float2* out, int nb, const float2* tw) {
+  if (nb <= 0) return hipSuccess;
+  CDK_DISPATCH_N(N, {
+    hipLaunchKernelGGL(k_r2c_planes<PL>, dim3(nb * N), dim3(kBlock), 0, st, in, out, tw);
+    hipLaunchKernelGGL((k_xlines<PL, -1>), dim3(nb * N), dim3(kBlock), 0, st, out, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_c2r_3d(hipStream_t st, int N, float2* in, float* out, int nb, const float2* tw) {
+  if (nb <= 0) return hipSuccess;
+  CDK_DISPATCH_N(N, {
+    hipLaunchKernelGGL((k_xlines<PL, 1>), dim3(nb * N), dim3(kBlock), 0, st, in, tw);
+    hipLaunchKernelGGL(k_c2r_planes<PL>, dim3(nb * N), dim3(kBlock), 0, st, in, out, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_conj_mult(hipStream_t st, float2* L, const float2* P, const int* pmap, int C, long long H,
+                            int nb) {
+  const long long total = (long long)nb * C * H;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_conj_mult, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0, st, L, P, pmap, C, H, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_grids(hipStream_t st, const float2* L, float2* S, int C, long long H, int nb, float G) {
+  const long long total = (long long)nb * H;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sum_grids, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0, st, L, S, C, H, total, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* meta, unsigned long long* keys,
+                              int N, int nb) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_argmin_full, dim3(N, nb), dim3(kBlock), 0, st, E, meta, keys, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                             const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
+                             int astride, const RotMeta* meta, int cmax, int boxmax, float2* A, const float2* tw) {
+  if (nslots <= 0) return hipSuccess;
+  CDK_DISPATCH_N(N, {
+    hipLaunchKernelGGL(k_fwd_planes<PL>, dim3(boxmax, cmax, nslots), dim3(kBlock), 0, st, slots, ligs, chan_atoms,
+                       wv, wq, ag, astride, meta, cmax, boxmax, A, tw);
+  });
+  return hipGetLastError();
+}
+
+template <int N> struct MidTile;
+template <> struct MidTile<32> { static constexpr int TK = 17; };
+template <> struct MidTile<48> { static constexpr int TK = 5; };
+template <> struct MidTile<64> { static constexpr int TK = 11; };
+template <> struct MidTile<96> { static constexpr int TK = 7; };
+template <> struct MidTile<128> { static constexpr int TK = 5; };
+
+int mid_tile(int N) {
+  switch (N) {
+    case 32: return MidTile<32>::TK;
+    case 48: return MidTile<48>::TK;
+    case 64: return MidTile<64>::TK;
+    case 96: return MidTile<96>::TK;
+    case 128: return MidTile<128>::TK;
+  }
+  return 0;
+}
+
+hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
+                      int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, int npch,
+                      float2* T, float invG, const float2* tw) {
+  if (nslots <= 0) return hipSuccess;
+  CDK_DISPATCH_N(N, {
+    constexpr int TK = MidTile<PL::N>::TK;
+    const unsigned tiles = ceil_div(PL::ZH, TK);
+    const size_t dyn = (size_t)npch * PL::N * TK * sizeof(float2);
+    auto kern = k_mid<PL, TK>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(PL::N, tiles, ceil_div(nslots, group)), dim3(kBlock), dyn, st, slots, ligs, meta,
+                       nslots, group, A, cmax, boxmax, Phat, npch, T, invG, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_inv_argmin(hipStream_t st, int N, const RotMeta* meta, int nslots, int wmax, const float2* T,
+                             unsigned long long* keys, const float2* tw) {
+  if (nslots <= 0) return hipSuccess;
+  CDK_DISPATCH_N(N, {
+    hipLaunchKernelGGL(k_inv_argmin<PL>, dim3(wmax, nslots), dim3(kBlock), 0, st, meta, T, keys, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_rotate(hipStream_t st, int nq, int natom, const float* quats, const float* coords, float* out) {
+  const size_t n = (size_t)nq * natom;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rotate, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, nq, natom, quats, coords, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_place(hipStream_t st, int nq, int natom, const float* coors, const float* mincoors, float h,
+                        AtomGrid* ag) {
+  const size_t n = (size_t)nq * natom;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_place, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, nq, natom, coors, mincoors, h, ag);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_generic(hipStream_t st, int nq, int natom, const AtomGrid* ag, const int* ch_start,
+                               const int* chan_atoms, const float* wv, const float* wq, int C, int X, int Y, int Z,
+                               float* grid) {
+  const size_t G = (size_t)X * Y * Z;
+  if (nq <= 0 || G == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_generic, dim3(ceil_div(G, kBlock), C, nq), dim3(kBlock), 0, st, natom, ag, ch_start,
+                     chan_atoms, wv, wq, C, X, Y, Z, grid);
+  return hipGetLastError();
+}
+
+}  // namespace cdk

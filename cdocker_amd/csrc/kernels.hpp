@@ -1,0 +1,47 @@
+// Host-side launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cdk_types.hpp"
+
+namespace cdk {
+
+bool grid_edge_supported(int N);
+int mid_tile(int N);
+
+hipError_t launch_prep(hipStream_t st, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                       const float* coords, int N, float h, int astride, int boxmax, float* rot, AtomGrid* ag,
+                       RotMeta* meta, unsigned long long* keys);
+
+// Reference-dataflow pipeline stages.
+hipError_t launch_spread_full(hipStream_t st, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                              const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
+                              int astride, const RotMeta* meta, int N, int C, float* out);
+hipError_t launch_r2c_3d(hipStream_t st, int N, const float* in, float2* out, int nb, const float2* tw);
+hipError_t launch_c2r_3d(hipStream_t st, int N, float2* in, float* out, int nb, const float2* tw);
+hipError_t launch_conj_mult(hipStream_t st, float2* L, const float2* P, const int* pmap, int C, long long H,
+                            int nb);
+hipError_t launch_sum_grids(hipStream_t st, const float2* L, float2* S, int C, long long H, int nb, float G);
+hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* meta, unsigned long long* keys,
+                              int N, int nb);
+
+// Fused, pruned pipeline stages.
+hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                             const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
+                             int astride, const RotMeta* meta, int cmax, int boxmax, float2* A, const float2* tw);
+hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
+                      int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, int npch,
+                      float2* T, float invG, const float2* tw);
+hipError_t launch_inv_argmin(hipStream_t st, int N, const RotMeta* meta, int nslots, int wmax, const float2* T,
+                             unsigned long long* keys, const float2* tw);
+
+hipError_t launch_rotate(hipStream_t st, int nq, int natom, const float* quats, const float* coords, float* out);
+
+// Standalone FillLigandGrid on caller-rotated coordinates (+= semantics).
+hipError_t launch_place(hipStream_t st, int nq, int natom, const float* coors, const float* mincoors, float h,
+                        AtomGrid* ag);
+hipError_t launch_fill_generic(hipStream_t st, int nq, int natom, const AtomGrid* ag, const int* ch_start,
+                               const int* chan_atoms, const float* wv, const float* wq, int C, int X, int Y, int Z,
+                               float* grid);
+
+}  // namespace cdk

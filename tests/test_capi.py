@@ -1,0 +1,92 @@
+"""C-ABI and host logic without a GPU: the library loads, exports every
+symbol include/cdk.h declares, reports errors as status codes, and its
+host-side producers (quaternion sampling, channel assignment, synthetic
+inputs, grid file I/O) match the oracle / the reference's own outputs."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cdocker_amd as cdk
+from cdocker_amd import _lib, api
+from oracle import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = np.load(os.path.join(ROOT, "tests", "golden", "ref_vectors.npz"))
+PROBE = np.array([1.2, 1.5, 1.7, 1.8, 1.9, 2.0, 2.1, 2.2], np.float32)
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "cdk.h")).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(cdk_[a-z0-9_]+)\(", text, re.M)))
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    names = declared_symbols()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(_lib.lib, n), n
+    assert names == _lib.DECLARED
+
+
+def test_status_codes_without_device():
+    h = C.c_void_p()
+    rc = _lib.lib.cdk_context_create(12345, C.byref(h))
+    assert rc in (_lib.ERR_INVALID, _lib.ERR_HIP)
+    assert len(_lib.last_error()) > 0
+    assert _lib.lib.cdk_context_create(0, None) == _lib.ERR_INVALID
+
+
+def test_quaternion_sampling_matches_reference_stream():
+    rng = cdk.Rng(int(GOLD["qus_seed"][0]))
+    got = np.stack([cdk.QuaternionUniformSampling(rng, 40) for _ in range(3)])
+    assert np.array_equal(got.view(np.uint32), GOLD["qus_out"].view(np.uint32))
+    rng.close()
+    norms = np.linalg.norm(got.reshape(-1, 4).astype(np.float64), axis=1)
+    assert np.allclose(norms, 1, atol=1e-6)
+
+
+def test_idx_of_atoms_matches_reference():
+    nu, used, ari = cdk.GetIdxOfAtomsForVdwRadius(GOLD["idx_radii"], PROBE)
+    assert nu == int(GOLD["idx_num_used"][0])
+    assert np.array_equal(used, GOLD["idx_used"]) and np.array_equal(ari, GOLD["idx_ari"])
+
+
+def test_sort_index_ties_by_position():
+    v = np.array([3.0, 1.0, 2.0, 1.0, np.inf, -1.0], np.float32)
+    assert cdk.sort_index(v).tolist() == [5, 1, 3, 2, 0, 4]
+
+
+def test_synthetic_generators_are_deterministic_and_valid():
+    x1, q1, e1, r1 = cdk.synth_ligand(5, 40)
+    x2, q2, e2, r2 = cdk.synth_ligand(5, 40)
+    assert np.array_equal(x1, x2) and np.array_equal(q1, q2)
+    d = np.linalg.norm(x1[1:] - x1[:-1], axis=1)
+    assert np.allclose(d, 1.5, atol=1e-4)
+    dm = np.linalg.norm(x1[:, None] - x1[None], axis=2)
+    assert dm[np.triu_indices(40, 2)].min() >= 1.2 - 1e-4
+    assert abs(float(q1.sum())) < 1e-4
+    assert set(np.unique(r1)).issubset(set(PROBE.tolist()))
+    confs = cdk.synth_conformers(6, x1, 5)
+    assert np.array_equal(confs[0] - confs[0].mean(0), confs[0])  or np.allclose(confs[0].mean(0), 0, atol=1e-5)
+    for c in confs:  # torsions keep bond lengths
+        assert np.allclose(np.linalg.norm(c[1:] - c[:-1], axis=1), 1.5, atol=1e-3)
+    v, radii, mid, ln = cdk.synth_protein(3, 300, 16.0, 32, 0.5)
+    assert v.shape == (9, 32, 32, 32) and np.all(v[:8] <= 3.0) and np.all(np.abs(v[8]) <= 20.0)
+    assert np.array_equal(radii, PROBE) and np.allclose(ln, 31 * 0.5)
+
+
+def test_grid_file_round_trip(tmp_path):
+    rs = np.random.default_rng(0)
+    vals = rs.standard_normal((3, 4, 5, 6)).astype(np.float32)
+    p = tmp_path / "g.txt"
+    cdk.write_grids(p, vals, np.array([1.5, 2.0], np.float32), np.array([1, 2, 3], np.float32),
+                    np.array([1.5, 2, 2.5], np.float32), 0.5, 7.0)
+    v2, r2, mid, ln, sp, rk = cdk.read_grids(p)
+    assert np.array_equal(v2, vals) and np.allclose(r2, [1.5, 2.0]) and sp == 0.5 and rk == 7.0
+    bad = tmp_path / "bad.txt"
+    bad.write_text("a\nb\nc\n2 4 4\n")
+    with pytest.raises(cdk.CdkError):
+        cdk.read_grids(bad)

@@ -1,0 +1,131 @@
+"""Batched (many ligands per launch) and served (continuous batching) forms:
+results must be identical to the per-ligand search, alone or in the mix."""
+import time
+
+import numpy as np
+import pytest
+
+import cdocker_amd as cdk
+from cdocker_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cdk.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def prots(ctx):
+    out = []
+    for name in ("p48", "p64"):
+        g = workloads.protein_grids(name)
+        out.append((g, cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])))
+    yield out
+    for _, p in out:
+        p.close()
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def test_batch_uneven_ligands_equal_per_ligand(ctx, prots):
+    say("batch uneven ligands")
+    g, prot = prots[1]
+    w = workloads.config3(nlig=6, nquat=24, conf_range=(1, 4), atom_range=(15, 60))
+    ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
+    b = cdk.Batch(ctx, prot)
+    for lig, L in zip(ligs, w["ligands"]):
+        b.add(lig, L["conformers"], L["quats"])
+    b.run(cdk.ENGINE_FUSED)
+    for j, (lig, L) in enumerate(zip(ligs, w["ligands"])):
+        E, idx, mc = b.results(j)
+        E1, idx1, mc1 = cdk.search(ctx, prot, lig, L["conformers"], L["quats"], engine=cdk.ENGINE_FUSED)
+        assert same(E, E1) and np.array_equal(idx, idx1) and same(mc, mc1)
+    b.close()
+
+
+def test_chunking_does_not_change_results(ctx, prots):
+    say("chunking invariance")
+    g, prot = prots[0]
+    w = workloads.config1()
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    q = L["quats"][:, :100]
+    E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], q)
+    ctx.set_option("fused_chunk", 7)
+    ctx.set_option("mid_group", 3)
+    E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"], q)
+    ctx.set_option("fused_chunk", 0)
+    ctx.set_option("mid_group", 16)
+    assert same(E0, E1) and np.array_equal(i0, i1)
+
+
+def test_server_alone_equals_in_mix(ctx, prots):
+    say("server alone vs mix")
+    rs = np.random.default_rng(11)
+    reqs = []
+    for k in range(6):
+        na = int(rs.integers(15, 40))
+        nc = int(rs.integers(1, 4))
+        nq = int(rs.choice([16, 40]))
+        L = workloads.ligand(500 + k, na, nc)
+        L["quats"] = workloads.quaternion_sets(600 + k, nc, nq)
+        L["protein_index"] = k % 2
+        reqs.append(L)
+    # alone: one request at a time through a server
+    srv = cdk.Server(ctx, [p for _, p in prots], 8 << 30, 64)
+    alone = []
+    for L in reqs:
+        rid = srv.submit(L["protein_index"], L["charge"], L["epsilon"], L["radius"], L["conformers"], L["quats"])
+        alone.append(srv.result(rid))
+    # mixed: all submitted at once, small launch sets so they interleave
+    srv.close()
+    srv = cdk.Server(ctx, [p for _, p in prots], 8 << 30, 24)
+    ids = [srv.submit(L["protein_index"], L["charge"], L["epsilon"], L["radius"], L["conformers"], L["quats"])
+           for L in reqs]
+    mixed = [srv.result(i) for i in ids]
+    st = srv.stats()
+    say("server stats", st)
+    assert st["requests_done"] == len(reqs)
+    for a, m, L in zip(alone, mixed, reqs):
+        assert same(a[0], m[0]) and np.array_equal(a[1], m[1]) and same(a[2], m[2])
+        # and equal to a direct search
+        lig = cdk.Ligand(prots[L["protein_index"]][1], L["charge"], L["epsilon"], L["radius"])
+        E, idx, mc = cdk.search(ctx, prots[L["protein_index"]][1], lig, L["conformers"], L["quats"])
+        assert same(E, m[0]) and np.array_equal(idx, m[1])
+        assert m[3] > 0
+    srv.close()
+
+
+def test_server_budget_rejects_oversized(ctx, prots):
+    say("server budget")
+    srv = cdk.Server(ctx, [prots[1][1]], 1 << 20, 64)
+    L = workloads.ligand(900, 30, 2)
+    q = workloads.quaternion_sets(901, 2, 100)
+    with pytest.raises(cdk.CdkError) as ei:
+        srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], q)
+    assert ei.value.code == 6
+    srv.close()
+
+
+def test_server_poll_pending_then_done(ctx, prots):
+    say("server poll")
+    srv = cdk.Server(ctx, [prots[1][1]], 8 << 30, 256)
+    L = workloads.ligand(910, 40, 4)
+    q = workloads.quaternion_sets(911, 4, 500)
+    rid = srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], q)
+    r = srv.result(rid, wait_ms=0)
+    t0 = time.time()
+    while r is None and time.time() - t0 < 60:
+        r = srv.result(rid, wait_ms=50)
+    assert r is not None and np.isfinite(r[0]).any()
+    srv.close()

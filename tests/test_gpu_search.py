@@ -1,0 +1,128 @@
+"""End-to-end parity of the search (both engines) against the oracle."""
+import numpy as np
+import pytest
+
+import cdocker_amd as cdk
+from cdocker_amd import workloads
+from oracle import oracle as orc
+from tests.parity import check_search
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cdk.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def p48(ctx):
+    g = workloads.protein_grids("p48")
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    yield g, prot
+    prot.close()
+
+
+@pytest.fixture(scope="module")
+def p64(ctx):
+    g = workloads.protein_grids("p64")
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    yield g, prot
+    prot.close()
+
+
+def test_protein_spectrum_vs_numpy(p48):
+    say("protein spectrum 48")
+    g, prot = p48
+    for c in (0, 8):
+        S = prot.spectrum(c)
+        ref = np.fft.rfftn(g["values"][c].astype(np.float64))
+        assert np.abs(S - ref).max() / np.abs(ref).max() < 2e-6
+
+
+@pytest.mark.parametrize("engine", [cdk.ENGINE_REFERENCE, cdk.ENGINE_FUSED])
+def test_config1_subset_vs_oracle(ctx, p48, engine):
+    """48^3 x 9, 24-atom ligand, 60 of the 500 quaternions."""
+    say("cfg1 subset engine", engine)
+    g, prot = p48
+    w = workloads.config1()
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    quats = L["quats"][:, :60]
+    E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats, engine=engine)
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    stats = check_search(prob, L["conformers"], quats, E, idx, mc)
+    say("cfg1 engine", engine, stats)
+    assert stats.get("exact", 0) >= 55
+
+
+def test_engines_agree_and_are_deterministic(ctx, p48):
+    say("engines agree")
+    g, prot = p48
+    w = workloads.config1()
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    quats = L["quats"][:, :200]
+    Er, ir, mr = cdk.search(ctx, prot, lig, L["conformers"], quats, engine=cdk.ENGINE_REFERENCE)
+    Ef, jf, mf = cdk.search(ctx, prot, lig, L["conformers"], quats, engine=cdk.ENGINE_FUSED)
+    Ef2, jf2, mf2 = cdk.search(ctx, prot, lig, L["conformers"], quats, engine=cdk.ENGINE_FUSED)
+    assert np.array_equal(Ef.view(np.uint32), Ef2.view(np.uint32)) and np.array_equal(jf, jf2)
+    assert np.array_equal(mr, mf)
+    fin = np.isfinite(Er)
+    assert np.array_equal(fin, np.isfinite(Ef))
+    assert np.allclose(Er[fin], Ef[fin], rtol=1e-3, atol=2e-3)
+    assert (ir == jf).all(axis=-1).mean() > 0.95
+
+
+def test_config2_subset_vs_oracle(ctx, p64):
+    """64^3 x 9, 40-atom ligand: 2 conformers x 12 quaternions per engine."""
+    g, prot = p64
+    w = workloads.config2(nconf=2, nquat=12)
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    for engine in (cdk.ENGINE_REFERENCE, cdk.ENGINE_FUSED):
+        say("cfg2 subset engine", engine)
+        E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], L["quats"], engine=engine)
+        stats = check_search(prob, L["conformers"], L["quats"], E, idx, mc)
+        say("cfg2 engine", engine, stats)
+        assert stats.get("exact", 0) >= 20
+
+
+def test_shared_quaternions_and_empty_windows(ctx, p48):
+    say("shared quats / empty windows")
+    g, prot = p48
+    x, q, e, r = cdk.synth_ligand(55, 20)
+    big = x * np.float32(4.0)  # too large for most rotations -> empty windows
+    lig = cdk.Ligand(prot, q, e, r)
+    rng = cdk.Rng(4)
+    quats = cdk.QuaternionUniformSampling(rng, 16)
+    confs = np.stack([x, big])
+    for engine in (cdk.ENGINE_REFERENCE, cdk.ENGINE_FUSED):
+        E, idx, mc = cdk.search(ctx, prot, lig, confs, quats, engine=engine)
+        prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], q, e, r)
+        stats = check_search(prob, confs, quats, E, idx, mc)
+        assert stats.get("empty", 0) >= 1
+
+
+def test_rank_poses_rebuild(ctx, p48):
+    say("rank poses")
+    g, prot = p48
+    w = workloads.config1()
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    quats = L["quats"][0, :40]
+    E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats, engine=cdk.ENGINE_FUSED)
+    rot = ctx.Rotate(quats, L["conformers"][0])
+    rank, pose = cdk.rank_poses(prot, rot, E[0], idx[0], mc[0], 5)
+    assert rank.tolist() == cdk.sort_index(E[0])[:5].tolist()
+    gmin = prot.grid_min()
+    for i, rq in enumerate(rank):
+        shift = gmin - mc[0, rq].astype(np.float64) + (idx[0, rq].astype(np.float32) * np.float32(prot.spacing))
+        assert np.allclose(pose[i], rot[rq].astype(np.float64) + shift, atol=1e-9)

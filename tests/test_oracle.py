@@ -1,0 +1,283 @@
+"""Pins the oracle (CPU restatement) before anything is checked against it.
+
+* golden vectors written by the reference's own compiled sources
+  (tests/golden/ref_vectors.npz): bit-exact;
+* the compiled reference subset live, on fresh inputs, when it is built;
+* closed forms and invariants (rotation matrix, spreading mass and first
+  moment, constant-grid and single-atom correlations);
+* an independent FP64 formulation (per-atom trilinear gather) and an FFT
+  restatement of the reference's correlation in numpy;
+* planted faults that those checks must catch.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"))
+PROBE = np.array([1.2, 1.5, 1.7, 1.8, 1.9, 2.0, 2.1, 2.2], np.float32)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+# ---------------------------------------------------------------- golden
+def test_rotate_golden_bit_exact():
+    out = orc.rotate(GOLD["rot_quats"], GOLD["rot_coords"])
+    assert np.array_equal(bits(out), bits(GOLD["rot_out"]))
+
+
+def test_min_max_golden_bit_exact():
+    mn, mx = orc.min_max_coors(GOLD["rot_out"])
+    assert np.array_equal(bits(mn), bits(GOLD["min_out"]))
+    assert np.array_equal(bits(mx), bits(GOLD["max_out"]))
+    tmn, tmx = orc.min_max_coors(GOLD["tie_coords"])
+    assert np.array_equal(bits(tmn), bits(GOLD["tie_min"]))  # signed zeros: first occurrence wins
+    assert np.array_equal(bits(tmx), bits(GOLD["tie_max"]))
+
+
+def test_idx_of_atoms_golden():
+    nu, used, ari = orc.idx_of_atoms_for_vdw_radius(GOLD["idx_radii"], PROBE)
+    assert nu == int(GOLD["idx_num_used"][0])
+    assert np.array_equal(used, GOLD["idx_used"])
+    assert np.array_equal(ari, GOLD["idx_ari"])
+
+
+def test_fill_ligand_grid_golden_bit_exact():
+    g = orc.fill_ligand_grid(GOLD["fill_rot"], GOLD["fill_min"], GOLD["fill_charge"], GOLD["fill_eps"],
+                             int(GOLD["fill_num_used"][0]), GOLD["fill_used"], GOLD["fill_ari"], 8,
+                             tuple(GOLD["fill_dims"]), float(GOLD["fill_spacing"][0]))
+    assert tuple(g.shape) == tuple(GOLD["fill_shape"])
+    flat = g.reshape(-1)
+    nz = np.flatnonzero(flat)
+    assert np.array_equal(nz, GOLD["fill_nz_idx"])
+    assert np.array_equal(bits(flat[nz]), bits(GOLD["fill_nz_val"]))
+
+
+def test_quaternion_sampling_golden_bit_exact():
+    out = orc.quaternion_uniform_sampling(int(GOLD["qus_seed"][0]), 3, 40)
+    assert np.array_equal(bits(out), bits(GOLD["qus_out"]))
+
+
+def test_filter_golden():
+    out = orc.filter_quaternions(GOLD["filt_coords"], GOLD["filt_quats"], GOLD["filt_lens"])
+    assert np.array_equal(bits(out), bits(GOLD["filt_out"]))
+    assert 0 < out.shape[0] < GOLD["filt_quats"].shape[0]
+
+
+# ------------------------------------------------- live compiled reference
+needs_ref = pytest.mark.skipif(not orc.have_ref(), reason="oracle/_ref not built (reference tree absent)")
+
+
+@needs_ref
+def test_live_reference_fresh_inputs():
+    rs = np.random.default_rng(77)
+    for trial in range(5):
+        n = int(rs.integers(5, 60))
+        x = (rs.standard_normal((n, 3)) * 4).astype(np.float32)
+        q = rs.standard_normal((16, 4)).astype(np.float32)
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
+        a = orc.rotate(q, x)
+        b = orc.rotate(q, x, lib="ref")
+        assert np.array_equal(bits(a), bits(b))
+        mna, mxa = orc.min_max_coors(a)
+        mnb, mxb = orc.min_max_coors(b, lib="ref")
+        assert np.array_equal(bits(mna), bits(mnb)) and np.array_equal(bits(mxa), bits(mxb))
+        r = rs.choice(PROBE, n).astype(np.float32)
+        assert all(np.array_equal(u, v) for u, v in zip(orc.idx_of_atoms_for_vdw_radius(r, PROBE)[1:],
+                                                        orc.idx_of_atoms_for_vdw_radius(r, PROBE, lib="ref")[1:]))
+        nu, used, ari = orc.idx_of_atoms_for_vdw_radius(r, PROBE)
+        ch = (0.3 * rs.standard_normal(n)).astype(np.float32)
+        ep = rs.uniform(0.05, 0.2, n).astype(np.float32)
+        dims = (40, 40, 40)
+        g1 = orc.fill_ligand_grid(a[:4], mna[:4], ch, ep, nu, used, ari, 8, dims, 0.5)
+        g2 = orc.fill_ligand_grid(a[:4], mna[:4], ch, ep, nu, used, ari, 8, dims, 0.5, lib="ref")
+        assert np.array_equal(bits(g1), bits(g2))
+    s1 = orc.quaternion_uniform_sampling(5, 4, 33)
+    s2 = orc.quaternion_uniform_sampling(5, 4, 33, lib="ref")
+    assert np.array_equal(bits(s1), bits(s2))
+
+
+# ------------------------------------------------------ closed forms
+def rotmat(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_rotate_closed_form():
+    rs = np.random.default_rng(1)
+    x = (rs.standard_normal((30, 3)) * 5).astype(np.float32)
+    q = rs.standard_normal((50, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    out = orc.rotate(q.astype(np.float32), x)
+    for i in range(50):
+        ref = x.astype(np.float64) @ rotmat(q[i]).T
+        assert np.allclose(out[i], ref, atol=5e-5)
+    # identity and q == -q
+    assert np.array_equal(orc.rotate(np.array([1, 0, 0, 0], np.float32), x)[0], x)
+    assert np.array_equal(bits(orc.rotate(q[:5].astype(np.float32), x)),
+                          bits(orc.rotate(-q[:5].astype(np.float32), x)))
+
+
+def test_spread_mass_and_first_moment():
+    rs = np.random.default_rng(2)
+    n = 25
+    x = (rs.standard_normal((n, 3)) * 3).astype(np.float32)
+    r = rs.choice(PROBE, n).astype(np.float32)
+    ch = rs.standard_normal(n).astype(np.float32)
+    ep = rs.uniform(0.05, 0.2, n).astype(np.float32)
+    nu, used, ari = orc.idx_of_atoms_for_vdw_radius(r, PROBE)
+    q = np.array([[0.8, 0.2, -0.4, 0.4]], np.float32)
+    q /= np.linalg.norm(q)
+    rot = orc.rotate(q, x)
+    mn, _ = orc.min_max_coors(rot)
+    h = 0.5
+    g = orc.fill_ligand_grid(rot, mn, ch, ep, nu, used, ari, 8, (40, 40, 40), h)[0]
+    idx = np.indices((40, 40, 40)).reshape(3, -1).T
+    for c in range(nu + 1):
+        atoms = np.nonzero(ari == used[c])[0] if c < nu else np.arange(n)
+        w = np.sqrt(ep[atoms]).astype(np.float64) if c < nu else ch[atoms].astype(np.float64)
+        assert abs(g[c].sum(dtype=np.float64) - w.sum()) < 1e-5 * (1 + np.abs(w).sum())
+        mom = (g[c].reshape(-1, 1).astype(np.float64) * idx).sum(0)
+        expect = (w[:, None] * (rot[0][atoms].astype(np.float64) - mn[0]) / h).sum(0)
+        assert np.allclose(mom, expect, atol=1e-4 * (1 + np.abs(expect).max()))
+
+
+def test_single_atom_on_node():
+    x = np.array([[0.0, 0.0, 0.0]], np.float32)
+    g = orc.fill_ligand_grid(x[None], np.zeros((1, 3), np.float32), np.array([0.7], np.float32),
+                             np.array([0.09], np.float32), 1, np.array([3]), np.array([3]), 8, (6, 6, 6), 0.5)[0]
+    assert g[1, 0, 0, 0] == np.float32(0.7) and np.count_nonzero(g[1]) == 1
+    assert g[0, 0, 0, 0] == np.float32(0.3) and np.count_nonzero(g[0]) == 1
+
+
+# --------------------------------------------- correlation (cpuNaive)
+def small_problem(seed=3, N=32, natom=12, h=0.5):
+    rs = np.random.default_rng(seed)
+    vals = rs.standard_normal((9, N, N, N)).astype(np.float32)
+    x = (rs.standard_normal((natom, 3)) * 1.6).astype(np.float32)
+    ch = (0.3 * rs.standard_normal(natom)).astype(np.float32)
+    ep = rs.uniform(0.05, 0.2, natom).astype(np.float32)
+    r = rs.choice(PROBE, natom).astype(np.float32)
+    return orc.PoseProblem(vals, PROBE, h, ch, ep, r), x, rs
+
+
+def test_dense_matches_sparse_fp64():
+    prob, x, rs = small_problem()
+    q = rs.standard_normal(4).astype(np.float32)
+    q /= np.linalg.norm(q)
+    res = prob.search_pose(x, q, double=True, want_E=True)
+    assert res["valid"]
+    rot = orc.rotate(q, x)[0]
+    E = res["E"]
+    # The dense form correlates the FP32-rounded ligand cells (as the
+    # reference does); the sparse form uses unrounded weights, so they agree
+    # to float rounding of the cells.  A dropped atom or corner is >= 1e-2.
+    for t in [(0, 0, 0), tuple(np.array(E.shape) - 1), tuple(res["index"]), (3, 1, 2)]:
+        s = prob.sparse_energy(rot, res["mincoor"], t)
+        assert abs(E[t] - s) <= 2e-6 * (1 + abs(s))
+    # planted fault: dropping one atom's contribution must be detected
+    prob.charge = prob.charge.copy()
+    prob.charge[0] += np.float32(0.05)
+    s_bad = prob.sparse_energy(rot, res["mincoor"], tuple(res["index"]))
+    assert abs(E[tuple(res["index"])] - s_bad) > 1e-3
+
+
+def test_dense_matches_fft_restatement():
+    """The reference computes IFFT(sum_c conj(FFT L_c) FFT P_c)/G; in FP64 it
+    must equal the direct correlation on the window."""
+    prob, x, rs = small_problem(seed=4)
+    q = rs.standard_normal(4).astype(np.float32)
+    q /= np.linalg.norm(q)
+    L, box, win, minc = prob.pose_box(x, q)
+    N = prob.N
+    Lf = np.zeros((L.shape[0], N, N, N))
+    Lf[:, : box[0], : box[1], : box[2]] = L
+    S = (np.conj(np.fft.rfftn(Lf, axes=(1, 2, 3))) * np.fft.rfftn(prob.Pused.astype(np.float64), axes=(1, 2, 3)))
+    Efft = np.fft.irfftn(S.sum(0), s=(N, N, N), axes=(0, 1, 2))
+    Ed = orc.correlate_window(prob.Pused, L, win, double=True)
+    assert np.allclose(Efft[: win[0], : win[1], : win[2]], Ed, rtol=1e-9, atol=1e-9)
+    # planted fault: transposing the ligand box (x<->z) must be detected
+    Lt = np.ascontiguousarray(np.transpose(L, (0, 3, 2, 1)))
+    bad = orc.correlate_window(prob.Pused, Lt, (win[2], win[1], win[0]), double=True)
+    assert not np.allclose(bad[: min(win), : min(win), : min(win)], Ed[: min(win), : min(win), : min(win)],
+                           rtol=1e-3)
+
+
+def test_dense_fp32_is_cpunaive_order():
+    """FP32 accumulation in c, m, n, o order: a plain-loop restatement on a
+    tiny box must agree bit for bit (zero cells included)."""
+    rs = np.random.default_rng(5)
+    N = 10
+    P = rs.standard_normal((2, N, N, N)).astype(np.float32)
+    L = rs.standard_normal((2, 3, 3, 4)).astype(np.float32)
+    L[L < 0.3] = 0
+    win = (N - 3, N - 3, N - 4)
+    E = orc.correlate_window(P, L, win)
+    ref = np.zeros(win, np.float32)
+    for i in range(win[0]):
+        for j in range(win[1]):
+            for k in range(win[2]):
+                e = np.float32(0)
+                for c in range(2):
+                    for m in range(3):
+                        for n in range(3):
+                            for o in range(4):
+                                e = np.float32(e + np.float32(L[c, m, n, o] * P[c, m + i, n + j, o + k]))
+                ref[i, j, k] = e
+    assert np.array_equal(bits(E), bits(ref))
+
+
+def test_constant_grid_known_answer():
+    prob, x, rs = small_problem(seed=6)
+    k = np.arange(1, prob.Pused.shape[0] + 1, dtype=np.float32) * np.float32(0.25)
+    prob.Pused = np.ascontiguousarray(np.broadcast_to(k[:, None, None, None], prob.Pused.shape).astype(np.float32))
+    q = np.array([1, 0, 0, 0], np.float32)
+    res = prob.search_pose(x, q, double=True, want_E=True)
+    expect = 0.0
+    for c in range(prob.num_used):
+        expect += k[c] * np.sqrt(prob.eps[prob.ari == prob.idx_used[c]]).astype(np.float64).sum()
+    expect += k[-1] * prob.charge.astype(np.float64).sum()
+    assert np.allclose(res["E"], expect, rtol=1e-6, atol=1e-6)
+
+
+def test_single_atom_correlation_is_window_of_grid():
+    rs = np.random.default_rng(8)
+    N = 32
+    vals = rs.standard_normal((9, N, N, N)).astype(np.float32)
+    prob = orc.PoseProblem(vals, PROBE, 0.5, np.array([0.5], np.float32), np.array([0.16], np.float32),
+                           np.array([1.8], np.float32))
+    res = prob.search_pose(np.zeros((1, 3), np.float32), np.array([1, 0, 0, 0], np.float32), double=True,
+                           want_E=True)
+    w = res["window"]
+    expect = 0.4 * vals[3][: w[0], : w[1], : w[2]].astype(np.float64) + 0.5 * vals[8][: w[0], : w[1], : w[2]]
+    assert np.allclose(res["E"], expect, rtol=1e-6, atol=1e-6)
+    # and the argmin is the textbook argmin of that window
+    assert tuple(res["index"]) == np.unravel_index(np.argmin(expect), expect.shape)
+
+
+def test_argmin_first_minimum_wins():
+    E = np.ones((4, 5, 6), np.float32)
+    E[2, 3, 1] = -1
+    E[1, 4, 5] = -1
+    E[3, 0, 0] = -1
+    e, idx = orc.argmin_first(E)
+    assert e == -1 and tuple(idx) == (1, 4, 5)
+    E[0, 0, 0] = np.nan
+    e, idx = orc.argmin_first(E)
+    assert tuple(idx) == (1, 4, 5)
+
+
+def test_window_and_empty_window():
+    prob, x, rs = small_problem(seed=9, N=32)
+    big = x * 8  # extent beyond the grid -> empty window, sentinel
+    res = prob.search_pose(big, np.array([1, 0, 0, 0], np.float32))
+    assert not res["valid"] and not np.isfinite(res["energy"]) and list(res["index"]) == [-1, -1, -1]
+    res = prob.search_pose(x, np.array([1, 0, 0, 0], np.float32))
+    ln = np.ptp(x, axis=0).astype(np.float32)
+    assert list(res["window"]) == [32 - int(np.float32(l) / np.float32(0.5)) - 2 for l in ln]
