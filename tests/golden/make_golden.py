@@ -89,7 +89,7 @@ def main():
     xf = walk(rs, 30)
     qf = unit_quats(rs, 200)
     ext = np.ptp(orc.rotate(qf, xf, lib="ref"), axis=1)
-    lens = np.percentile(ext, 60, axis=0).astype(np.float32)
+    lens = np.percentile(ext, 85, axis=0).astype(np.float32)
     d.update(filt_coords=xf, filt_quats=qf, filt_lens=lens, filt_out=orc.filter_quaternions(xf, qf, lens, lib="ref"))
     np.savez_compressed(OUT, **d)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")

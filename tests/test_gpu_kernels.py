@@ -72,7 +72,7 @@ def test_fill_accumulates_into_existing_values(ctx):
     dims = tuple(int(v) for v in GOLD["fill_dims"])
     base = np.full(tuple(GOLD["fill_shape"]), 0.25, np.float32)
     g = ctx.FillLigandGrid(GOLD["fill_rot"], GOLD["fill_min"], GOLD["fill_charge"], GOLD["fill_eps"], nu,
-                           GOLD["fill_used"], lists, *dims, float(GOLD["fill_spacing"][0]), grid=base)
+                           GOLD["fill_used"], lists, *dims, float(GOLD["fill_spacing"][0]), grid=base.copy())
     o = orc.fill_ligand_grid(GOLD["fill_rot"], GOLD["fill_min"], GOLD["fill_charge"], GOLD["fill_eps"], nu,
                              GOLD["fill_used"], ari, 8, dims, float(GOLD["fill_spacing"][0]), grid=base.copy())
     assert np.array_equal(bits(g), bits(o))
