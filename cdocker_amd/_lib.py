@@ -79,6 +79,7 @@ _SIGS = {
     "cdk_batch_run": [P, I],
     "cdk_batch_poses": [P, I64P],
     "cdk_batch_results": [P, I, FP, IP, FP],
+    "cdk_batch_geometry": [P, I, IP, IP],
     "cdk_batch_destroy": [P],
     "cdk_server_create": [P, I, PP, I64, I, PP],
     "cdk_server_submit": [P, I, I, FP, FP, FP, I, FP, I, FP, I, I64P],

@@ -316,6 +316,13 @@ class Batch:
         check(lib.cdk_batch_results(self.h, int(job), _fp(e), _ip(idx), _fp(mc)), "cdk_batch_results")
         return e, idx, mc
 
+    def geometry(self, job):
+        nconf, nquat = self.jobs[job]
+        box = np.empty((nconf, nquat, 3), np.int32)
+        win = np.empty((nconf, nquat, 3), np.int32)
+        check(lib.cdk_batch_geometry(self.h, int(job), _ip(box), _ip(win)), "cdk_batch_geometry")
+        return box, win
+
     def close(self):
         if self.h:
             check(lib.cdk_batch_destroy(self.h))

@@ -331,6 +331,7 @@ int cdk_batch_add(cdk_batch* b, const cdk_ligand* l, int nconf, const float* con
     it.coords.assign(conf_coords, conf_coords + (size_t)nconf * l->l->natom * 3);
     it.quats.assign(quats, quats + (size_t)(it.shared ? 1 : nconf) * nquat * 4);
     b->b.items.push_back(std::move(it));
+    b->b.dirty = true;
     if (job_id) *job_id = (int)b->b.items.size() - 1;
   });
 }
@@ -355,11 +356,21 @@ int cdk_batch_poses(cdk_batch* b, int64_t* total) {
 int cdk_batch_results(cdk_batch* b, int job_id, float* energy, int* index, float* mincoor) {
   return guard([&] {
     require(b != nullptr, "NULL batch");
-    require(job_id >= 0 && job_id < (int)b->b.results.size(), "job id out of range or batch not run");
-    const cdk::PoseResults& r = b->b.results[job_id];
+    require(job_id >= 0 && job_id < (int)b->b.results().size(), "job id out of range or batch not run");
+    const cdk::PoseResults& r = b->b.results()[job_id];
     if (energy) std::memcpy(energy, r.energy.data(), r.energy.size() * sizeof(float));
     if (index) std::memcpy(index, r.index.data(), r.index.size() * sizeof(int));
     if (mincoor) std::memcpy(mincoor, r.mincoor.data(), r.mincoor.size() * sizeof(float));
+  });
+}
+
+int cdk_batch_geometry(cdk_batch* b, int job_id, int* box, int* windows) {
+  return guard([&] {
+    require(b != nullptr, "NULL batch");
+    require(job_id >= 0 && job_id < (int)b->b.results().size(), "job id out of range or batch not run");
+    const cdk::PoseResults& r = b->b.results()[job_id];
+    if (box) std::memcpy(box, r.box.data(), r.box.size() * sizeof(int));
+    if (windows) std::memcpy(windows, r.win.data(), r.win.size() * sizeof(int));
   });
 }
 

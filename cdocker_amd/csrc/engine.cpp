@@ -205,17 +205,7 @@ double quat_scale(const float* q, size_t n) {
   return m * (1.0 + 1e-5);
 }
 
-struct Tables {
-  std::vector<LigandDesc> ligs;
-  std::vector<int> chan;
-  std::vector<float> wv, wq, coords;
-  std::vector<PoseSlot> slots;
-  std::vector<size_t> job_first;  // first slot of each job
-  int astride = 0, cmax = 0, boxmax = 2;
-  int dummy_lig = 0;
-};
-
-void build_tables(const Protein& prot, const std::vector<Job>& jobs, Tables& t) {
+void build_tables(const Protein& prot, const std::vector<Job>& jobs, Prepared& t) {
   std::map<const Ligand*, int> lig_index;
   for (const Job& j : jobs) {
     if (!j.lig) throw Error(kErrInvalid, "job without ligand");
@@ -330,146 +320,235 @@ struct EventTimer {
 
 }  // namespace
 
-void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,
-                std::vector<PoseResults>& out) {
-  std::lock_guard<std::mutex> lock(ctx.mu);
-  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
-  Tables t;
-  build_tables(prot, jobs, t);
+namespace {
+
+void ensure_work(Context& ctx, size_t bytes_needed, DevBuf& b) {
+  if (b.cap >= bytes_needed) return;
+  b.ensure(bytes_needed);
+  ++ctx.ws_generation;
+}
+
+// Reference dataflow: one launch sequence per batch of <= B rotations of one
+// conformer (TranRotaConfSearchNoMini.cu:385-464).
+void enqueue_reference(Context& ctx, const Protein& prot, Prepared& t, EventTimer& timer) {
   const int N = prot.N;
   const size_t G = prot.G(), H = prot.H();
-  const int ZH = N / 2 + 1;
   const float2* tw = ctx.twiddle(N);
+  const int B = std::max(1, ctx.ref_batch);
+  const int C = t.cmax;
+  ensure_work(ctx, (size_t)B * t.astride * 3 * sizeof(float), ctx.d_rot);
+  ensure_work(ctx, (size_t)B * t.astride * sizeof(AtomGrid), ctx.d_ag);
+  ensure_work(ctx, (size_t)B * C * G * sizeof(float), ctx.d_L);
+  ensure_work(ctx, (size_t)B * C * H * sizeof(float2), ctx.d_Lhat);
+  ensure_work(ctx, (size_t)B * H * sizeof(float2), ctx.d_S);
+  ensure_work(ctx, (size_t)B * G * sizeof(float), ctx.d_E);
+  const PoseSlot* d_slots = t.d_slots.as<PoseSlot>();
+  const LigandDesc* d_ligs = t.d_ligs.as<LigandDesc>();
+  unsigned long long* d_keys = t.d_keys.as<unsigned long long>();
+  RotMeta* d_meta = t.d_meta.as<RotMeta>();
   const size_t total = t.slots.size();
-  out.assign(jobs.size(), PoseResults());
+  size_t s = 0;
+  while (s < total) {
+    const int lig_idx = t.slots[s].lig;
+    const int coff = t.slots[s].coord_off;
+    size_t e2 = s + 1;
+    while (e2 < total && (int)(e2 - s) < B && t.slots[e2].coord_off == coff && t.slots[e2].lig == lig_idx) ++e2;
+    const int n = (int)(e2 - s);
+    const int nch = t.ligs[lig_idx].nch;
+    const int* pmap = reinterpret_cast<const int*>(reinterpret_cast<const char*>(d_ligs + lig_idx) +
+                                                   offsetof(LigandDesc, pmap));
+    const PoseSlot* sl = d_slots + s;
+    hipEvent_t e;
+    e = timer.begin(0);
+    hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, N,
+                          ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
+              "prep");
+    timer.end(e);
+    e = timer.begin(1);
+    hip_check(launch_spread_full(ctx.stream, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(), t.d_wq.as<float>(),
+                                 ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, N, nch, ctx.d_L.as<float>()),
+              "spread");
+    timer.end(e);
+    e = timer.begin(2);
+    hip_check(launch_r2c_3d(ctx.stream, N, ctx.d_L.as<float>(), ctx.d_Lhat.as<float2>(), n * nch, tw), "ligand R2C");
+    timer.end(e);
+    e = timer.begin(3);
+    hip_check(launch_conj_mult(ctx.stream, ctx.d_Lhat.as<float2>(), prot.spec.as<float2>(), pmap, nch, (long long)H,
+                               n),
+              "ConjMult");
+    timer.end(e);
+    e = timer.begin(4);
+    hip_check(launch_sum_grids(ctx.stream, ctx.d_Lhat.as<float2>(), ctx.d_S.as<float2>(), nch, (long long)H, n,
+                               (float)G),
+              "SumGrids");
+    timer.end(e);
+    e = timer.begin(5);
+    hip_check(launch_c2r_3d(ctx.stream, N, ctx.d_S.as<float2>(), ctx.d_E.as<float>(), n, tw), "C2R");
+    timer.end(e);
+    e = timer.begin(6);
+    hip_check(launch_argmin_full(ctx.stream, ctx.d_E.as<float>(), d_meta + s, d_keys + s, N, n), "argmin");
+    timer.end(e);
+    s = e2;
+  }
+}
+
+size_t fused_chunk(Context& ctx, const Protein& prot, const Prepared& t) {
+  const int N = prot.N, ZH = N / 2 + 1;
+  const size_t plane = (size_t)N * ZH;
+  const size_t per_slot = (size_t)t.cmax * t.boxmax * plane * sizeof(float2) + (size_t)N * plane * sizeof(float2) +
+                          (size_t)t.astride * (3 * sizeof(float) + sizeof(AtomGrid));
+  size_t chunk = ctx.fused_chunk > 0 ? (size_t)ctx.fused_chunk : std::max<size_t>(1, ctx.ws_budget / per_slot);
+  return std::min(chunk, t.slots.size());
+}
+
+void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, size_t chunk) {
+  const int N = prot.N, ZH = N / 2 + 1;
+  const size_t plane = (size_t)N * ZH;
+  ensure_work(ctx, chunk * t.astride * 3 * sizeof(float), ctx.d_rot);
+  ensure_work(ctx, chunk * t.astride * sizeof(AtomGrid), ctx.d_ag);
+  ensure_work(ctx, chunk * t.cmax * t.boxmax * plane * sizeof(float2), ctx.d_A);
+  ensure_work(ctx, chunk * N * plane * sizeof(float2), ctx.d_T);
+}
+
+// Fused pipeline: four launches per chunk of poses, any mix of ligands.
+void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk, EventTimer& timer) {
+  const int N = prot.N;
+  const size_t G = prot.G();
+  const float2* tw = ctx.twiddle(N);
+  const PoseSlot* d_slots = t.d_slots.as<PoseSlot>();
+  const LigandDesc* d_ligs = t.d_ligs.as<LigandDesc>();
+  unsigned long long* d_keys = t.d_keys.as<unsigned long long>();
+  RotMeta* d_meta = t.d_meta.as<RotMeta>();
+  const size_t total = t.slots.size();
+  const int group = std::max(1, ctx.mid_group);
+  for (size_t s = 0; s < total; s += chunk) {
+    const int n = (int)std::min(chunk, total - s);
+    const PoseSlot* sl = d_slots + s;
+    hipEvent_t e;
+    e = timer.begin(0);
+    hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, t.boxmax,
+                          ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
+              "prep");
+    timer.end(e);
+    e = timer.begin(1);
+    hip_check(launch_fwd_planes(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+                                t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, t.cmax, t.boxmax,
+                                ctx.d_A.as<float2>(), tw),
+              "fwd planes");
+    timer.end(e);
+    e = timer.begin(2);
+    hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
+                         prot.spec.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+              "mid");
+    timer.end(e);
+    e = timer.begin(3);
+    hip_check(launch_inv_argmin(ctx.stream, N, d_meta + s, n, N - 2, ctx.d_T.as<float2>(), d_keys + s, tw),
+              "inverse + argmin");
+    timer.end(e);
+  }
+}
+
+}  // namespace
+
+Prepared::~Prepared() { drop_graph(); }
+
+void Prepared::drop_graph() {
+  if (graph) (void)hipGraphExecDestroy(graph);
+  graph = nullptr;
+}
+
+void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Prepared& t) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  t.drop_graph();
+  t.ligs.clear();
+  t.chan.clear();
+  t.wv.clear();
+  t.wq.clear();
+  t.coords.clear();
+  t.slots.clear();
+  t.job_first.clear();
+  t.job_n.clear();
+  t.astride = 0;
+  t.cmax = 0;
+  t.boxmax = 2;
+  build_tables(prot, jobs, t);
+  for (const Job& j : jobs) t.job_n.push_back(j.npose());
+  t.prot = &prot;
+  upload(ctx, t.d_ligs, t.ligs);
+  upload(ctx, t.d_chan, t.chan);
+  upload(ctx, t.d_wv, t.wv);
+  upload(ctx, t.d_wq, t.wq);
+  upload(ctx, t.d_coords, t.coords);
+  upload(ctx, t.d_slots, t.slots);
+  const size_t total = std::max<size_t>(t.slots.size(), 1);
+  t.d_keys.ensure(total * sizeof(unsigned long long));
+  t.d_meta.ensure(total * sizeof(RotMeta));
+  hip_check(hipStreamSynchronize(ctx.stream), "prepare sync");
+}
+
+void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  if (!t.prot) throw Error(kErrInvalid, "search not prepared");
+  const Protein& prot = *t.prot;
+  const size_t total = t.slots.size();
   if (total == 0) return;
-
-  upload(ctx, ctx.d_ligs, t.ligs);
-  upload(ctx, ctx.d_chan, t.chan);
-  upload(ctx, ctx.d_wv, t.wv);
-  upload(ctx, ctx.d_wq, t.wq);
-  upload(ctx, ctx.d_coords, t.coords);
-  upload(ctx, ctx.d_slots, t.slots);
-  ctx.d_keys.ensure(total * sizeof(unsigned long long));
-  ctx.d_meta.ensure(total * sizeof(RotMeta));
-  const PoseSlot* d_slots = ctx.d_slots.as<PoseSlot>();
-  const LigandDesc* d_ligs = ctx.d_ligs.as<LigandDesc>();
-  unsigned long long* d_keys = ctx.d_keys.as<unsigned long long>();
-  RotMeta* d_meta = ctx.d_meta.as<RotMeta>();
   EventTimer timer(ctx);
-
   if (engine == kEngineReference) {
-    // One launch sequence per batch of B rotations of one conformer
-    // (TranRotaConfSearchNoMini.cu:297-464).
-    const int B = std::max(1, ctx.ref_batch);
-    const int C = t.cmax;
-    ctx.d_rot.ensure((size_t)B * t.astride * 3 * sizeof(float));
-    ctx.d_ag.ensure((size_t)B * t.astride * sizeof(AtomGrid));
-    ctx.d_L.ensure((size_t)B * C * G * sizeof(float));
-    ctx.d_Lhat.ensure((size_t)B * C * H * sizeof(float2));
-    ctx.d_S.ensure((size_t)B * H * sizeof(float2));
-    ctx.d_E.ensure((size_t)B * G * sizeof(float));
-    size_t s = 0;
-    while (s < total) {
-      // Batch: consecutive poses of one conformer, at most B of them.
-      const int lig_idx = t.slots[s].lig;
-      const int coff = t.slots[s].coord_off;
-      size_t e2 = s + 1;
-      while (e2 < total && (int)(e2 - s) < B && t.slots[e2].coord_off == coff && t.slots[e2].lig == lig_idx) ++e2;
-      const int n = (int)(e2 - s);
-      const int nch = t.ligs[lig_idx].nch;
-      const int* pmap = reinterpret_cast<const int*>(reinterpret_cast<const char*>(d_ligs + lig_idx) +
-                                                     offsetof(LigandDesc, pmap));
-      const PoseSlot* sl = d_slots + s;
-      hipEvent_t e;
-      e = timer.begin(0);
-      hip_check(launch_prep(ctx.stream, sl, n, d_ligs, ctx.d_coords.as<float>(), N, prot.h, t.astride, N,
-                            ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
-                "prep");
-      timer.end(e);
-      e = timer.begin(1);
-      hip_check(launch_spread_full(ctx.stream, sl, n, d_ligs, ctx.d_chan.as<int>(), ctx.d_wv.as<float>(),
-                                   ctx.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, N, nch,
-                                   ctx.d_L.as<float>()),
-                "spread");
-      timer.end(e);
-      e = timer.begin(2);
-      hip_check(launch_r2c_3d(ctx.stream, N, ctx.d_L.as<float>(), ctx.d_Lhat.as<float2>(), n * nch, tw),
-                "ligand R2C");
-      timer.end(e);
-      e = timer.begin(3);
-      hip_check(launch_conj_mult(ctx.stream, ctx.d_Lhat.as<float2>(), prot.spec.as<float2>(), pmap, nch,
-                                 (long long)H, n),
-                "ConjMult");
-      timer.end(e);
-      e = timer.begin(4);
-      hip_check(launch_sum_grids(ctx.stream, ctx.d_Lhat.as<float2>(), ctx.d_S.as<float2>(), nch, (long long)H, n,
-                                 (float)G),
-                "SumGrids");
-      timer.end(e);
-      e = timer.begin(5);
-      hip_check(launch_c2r_3d(ctx.stream, N, ctx.d_S.as<float2>(), ctx.d_E.as<float>(), n, tw), "C2R");
-      timer.end(e);
-      e = timer.begin(6);
-      hip_check(launch_argmin_full(ctx.stream, ctx.d_E.as<float>(), d_meta + s, d_keys + s, N, n), "argmin");
-      timer.end(e);
-      s = e2;
-    }
+    enqueue_reference(ctx, prot, t, timer);
   } else {
-    const int cmax = t.cmax, boxmax = t.boxmax;
-    const size_t plane = (size_t)N * ZH;
-    const size_t per_slot = (size_t)cmax * boxmax * plane * sizeof(float2) + (size_t)N * plane * sizeof(float2) +
-                            (size_t)t.astride * (3 * sizeof(float) + sizeof(AtomGrid));
-    size_t chunk = ctx.fused_chunk > 0 ? (size_t)ctx.fused_chunk : std::max<size_t>(1, ctx.ws_budget / per_slot);
-    chunk = std::min(chunk, total);
-    ctx.d_rot.ensure(chunk * t.astride * 3 * sizeof(float));
-    ctx.d_ag.ensure(chunk * t.astride * sizeof(AtomGrid));
-    ctx.d_A.ensure(chunk * cmax * boxmax * plane * sizeof(float2));
-    ctx.d_T.ensure(chunk * N * plane * sizeof(float2));
-    const int group = std::max(1, ctx.mid_group);
-    for (size_t s = 0; s < total; s += chunk) {
-      const int n = (int)std::min(chunk, total - s);
-      const PoseSlot* sl = d_slots + s;
-      hipEvent_t e;
-      e = timer.begin(0);
-      hip_check(launch_prep(ctx.stream, sl, n, d_ligs, ctx.d_coords.as<float>(), N, prot.h, t.astride, boxmax,
-                            ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
-                "prep");
-      timer.end(e);
-      e = timer.begin(1);
-      hip_check(launch_fwd_planes(ctx.stream, N, sl, n, d_ligs, ctx.d_chan.as<int>(), ctx.d_wv.as<float>(),
-                                  ctx.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, cmax, boxmax,
-                                  ctx.d_A.as<float2>(), tw),
-                "fwd planes");
-      timer.end(e);
-      e = timer.begin(2);
-      hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), cmax, boxmax,
-                           prot.spec.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
-                "mid");
-      timer.end(e);
-      e = timer.begin(3);
-      hip_check(launch_inv_argmin(ctx.stream, N, d_meta + s, n, N - 2, ctx.d_T.as<float2>(), d_keys + s, tw),
-                "inverse + argmin");
-      timer.end(e);
+    const size_t chunk = fused_chunk(ctx, prot, t);
+    ensure_fused_work(ctx, prot, t, chunk);
+    (void)ctx.twiddle(prot.N);
+    if (use_graph && !ctx.time_stages) {
+      if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk ||
+          t.graph_group != ctx.mid_group) {
+        t.drop_graph();
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        try {
+          enqueue_fused(ctx, prot, t, chunk, timer);
+        } catch (...) {
+          (void)hipStreamEndCapture(ctx.stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        hip_check(hipStreamEndCapture(ctx.stream, &g), "hipStreamEndCapture");
+        hipError_t ie = hipGraphInstantiate(&t.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(ie, "hipGraphInstantiate");
+        t.graph_gen = ctx.ws_generation;
+        t.graph_chunk = chunk;
+        t.graph_group = ctx.mid_group;
+      }
+      hip_check(hipGraphLaunch(t.graph, ctx.stream), "hipGraphLaunch");
+    } else {
+      enqueue_fused(ctx, prot, t, chunk, timer);
     }
   }
   ctx.ensure_host(total);
-  hip_check(hipMemcpyAsync(ctx.h_keys, d_keys, total * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+  hip_check(hipMemcpyAsync(ctx.h_keys, t.d_keys.p, total * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            ctx.stream),
             "keys D2H");
-  hip_check(hipMemcpyAsync(ctx.h_meta, d_meta, total * sizeof(RotMeta), hipMemcpyDeviceToHost, ctx.stream),
+  hip_check(hipMemcpyAsync(ctx.h_meta, t.d_meta.p, total * sizeof(RotMeta), hipMemcpyDeviceToHost, ctx.stream),
             "meta D2H");
   hip_check(hipStreamSynchronize(ctx.stream), "search sync");
   timer.collect();
+  // Decode while the pinned staging is ours (context lock held).
   const unsigned long long* keys = static_cast<const unsigned long long*>(ctx.h_keys);
   const RotMeta* meta = static_cast<const RotMeta*>(ctx.h_meta);
-  for (size_t j = 0; j < jobs.size(); ++j) {
+  const int N = prot.N;
+  t.results.assign(t.job_first.size(), PoseResults());
+  for (size_t j = 0; j < t.job_first.size(); ++j) {
     const size_t first = t.job_first[j];
-    const size_t n = jobs[j].npose();
-    PoseResults& r = out[j];
+    const size_t n = t.job_n[j];
+    PoseResults& r = t.results[j];
     r.energy.resize(n);
     r.index.resize(n * 3);
     r.mincoor.resize(n * 3);
+    r.box.resize(n * 3);
+    r.win.resize(n * 3);
     for (size_t i = 0; i < n; ++i) {
       const unsigned long long k = keys[first + i];
       const RotMeta& m = meta[first + i];
@@ -483,9 +562,21 @@ void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs,
         r.index[i * 3 + 1] = (int)((lin / (unsigned)N) % (unsigned)N);
         r.index[i * 3 + 2] = (int)(lin % (unsigned)N);
       }
-      for (int k3 = 0; k3 < 3; ++k3) r.mincoor[i * 3 + k3] = m.minc[k3];
+      for (int k3 = 0; k3 < 3; ++k3) {
+        r.mincoor[i * 3 + k3] = m.minc[k3];
+        r.box[i * 3 + k3] = m.box[k3];
+        r.win[i * 3 + k3] = m.win[k3];
+      }
     }
   }
+}
+
+void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,
+                std::vector<PoseResults>& out) {
+  Prepared t;
+  prepare_search(ctx, prot, jobs, t);
+  execute_search(ctx, t, engine, false);
+  out = std::move(t.results);
 }
 
 }  // namespace cdk

@@ -109,6 +109,8 @@ struct PoseResults {
   std::vector<float> energy;   // +inf when the window is empty
   std::vector<int> index;      // [n][3] best translation (i, j, k); -1 when empty
   std::vector<float> mincoor;  // [n][3]
+  std::vector<int> box;        // [n][3] occupied ligand box (max grid index + 2)
+  std::vector<int> win;        // [n][3] translation window
 };
 
 enum Engine { kEngineReference = 0, kEngineFused = 1 };
@@ -116,6 +118,31 @@ enum Engine { kEngineReference = 0, kEngineFused = 1 };
 struct StageTimes {
   double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Search tables resident on the device (ligand table, channel lists,
+// weights, conformer coordinates, pose slots, results) plus an optional
+// hipGraph of the whole fused launch sequence.
+struct Prepared {
+  const Protein* prot = nullptr;
+  std::vector<LigandDesc> ligs;
+  std::vector<int> chan;
+  std::vector<float> wv, wq, coords;
+  std::vector<PoseSlot> slots;
+  std::vector<size_t> job_first, job_n;
+  int astride = 0, cmax = 0, boxmax = 2;
+  int dummy_lig = 0;
+  DevBuf d_ligs, d_chan, d_wv, d_wq, d_coords, d_slots, d_keys, d_meta;
+  hipGraphExec_t graph = nullptr;
+  uint64_t graph_gen = 0;
+  size_t graph_chunk = 0;
+  int graph_group = 0;
+  std::vector<PoseResults> results;
+  Prepared() = default;
+  Prepared(const Prepared&) = delete;
+  Prepared& operator=(const Prepared&) = delete;
+  ~Prepared();
+  void drop_graph();
 };
 
 struct Context {
@@ -131,11 +158,11 @@ struct Context {
   bool use_graph = true;
   bool time_stages = false;
   StageTimes stage_times;
-  // workspace
-  DevBuf d_slots, d_ligs, d_chan, d_wv, d_wq, d_coords, d_pmap;
-  DevBuf d_rot, d_ag, d_meta, d_keys;
+  // per-launch-set working arrays; ws_generation counts reallocations so a
+  // captured graph that baked in the old addresses is rebuilt
+  DevBuf d_rot, d_ag;
   DevBuf d_A, d_T, d_L, d_Lhat, d_S, d_E;
-  DevBuf d_stage_slots;
+  uint64_t ws_generation = 1;
   // pinned host staging
   void* h_keys = nullptr;
   void* h_meta = nullptr;
@@ -150,7 +177,12 @@ struct Context {
 void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const float* len, const float* radii,
                     const float* values);
 
-// Runs every pose of every job.  Results are appended per job in order.
+// Builds and uploads the tables of a set of jobs (inputs resident in HBM).
+void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Prepared& t);
+// Runs every prepared pose; results land in t.results (one entry per job).
+// With use_graph the fused launch sequence is captured once and replayed.
+void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph);
+// prepare + execute without a graph.
 void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,
                 std::vector<PoseResults>& out);
 

@@ -28,6 +28,13 @@ size_t fused_bytes_per_pose(int N, int nch, int boxmax, int natom) {
 }
 
 void Batch::run(Engine engine) {
+  if (!prepared || dirty) {
+    prepare();
+  }
+  execute_search(*ctx, prep, engine, ctx->use_graph);
+}
+
+void Batch::prepare() {
   std::vector<Job> jobs;
   jobs.reserve(items.size());
   for (const Item& it : items) {
@@ -40,7 +47,9 @@ void Batch::run(Engine engine) {
     j.shared_quats = it.shared;
     jobs.push_back(j);
   }
-  run_search(*ctx, *prot, jobs, engine, results);
+  prepare_search(*ctx, *prot, jobs, prep);
+  prepared = true;
+  dirty = false;
 }
 
 Server::Server(Context& ctx, std::vector<Protein*> prots, size_t budget, int max_batch)

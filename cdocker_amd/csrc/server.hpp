@@ -29,8 +29,11 @@ struct Batch {
     std::vector<float> coords, quats;
   };
   std::vector<Item> items;
-  std::vector<PoseResults> results;
+  Prepared prep;
+  bool prepared = false, dirty = true;
+  void prepare();
   void run(Engine engine);
+  const std::vector<PoseResults>& results() const { return prep.results; }
 };
 
 struct Request {

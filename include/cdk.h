@@ -118,6 +118,9 @@ int cdk_batch_add(cdk_batch* b, const cdk_ligand* l, int nconf, const float* con
 int cdk_batch_run(cdk_batch* b, int engine);
 int cdk_batch_poses(cdk_batch* b, int64_t* total_poses);
 int cdk_batch_results(cdk_batch* b, int job_id, float* energy, int* index, float* mincoor);
+/* Per pose of a job: occupied ligand box [.][3] (max grid index + 2) and
+ * translation window [.][3]; the sizes the pruned passes work on. */
+int cdk_batch_geometry(cdk_batch* b, int job_id, int* box, int* windows);
 int cdk_batch_destroy(cdk_batch* b);
 
 /* ---- serving: continuous batching over resident proteins ------------ */

@@ -190,6 +190,14 @@ extern "C" void orc_argmin_first(const float* E, int wx, int wy, int wz, float* 
   *emin = m;
 }
 
+static int g_threads = 0;
+// Worker threads of the correlation loops (0 = hardware concurrency).
+extern "C" void orc_set_threads(int n) { g_threads = n; }
+static int n_threads(int cap) {
+  const int hw = g_threads > 0 ? g_threads : (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(cap, hw));
+}
+
 // ---- A15: dense direct correlation over the translation window.
 //   E[i][j][k] = sum_c sum_m sum_n sum_o L[c][m][n][o] * P[c][m+i][n+j][o+k]
 // accumulated in T in the order c, m, n, o.  Cells with L == 0 add an exact
@@ -224,7 +232,34 @@ static void correlate_window(const float* P, int N, const float* L, int C, int b
           E[((size_t)i * wy + j) * wz + k] = e;
         }
   };
-  const int nt = std::max(1, std::min(wx, (int)std::thread::hardware_concurrency()));
+  const int nt = n_threads(wx);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) th.emplace_back(work, wx * t / nt, wx * (t + 1) / nt);
+  for (auto& t : th) t.join();
+}
+
+// The literal cpuNaive.cpp:194-216 loop nest: every cell of the ligand box,
+// zeros included, FP32 accumulation in c, m, n, o order.  Used as the timed
+// CPU baseline (bench.py); results equal correlate_window<float>.
+extern "C" void orc_correlate_window_full_f32(const float* P, int N, const float* L, int C, int bx, int by, int bz,
+                                              int wx, int wy, int wz, float* E) {
+  const size_t G = (size_t)N * N * N;
+  auto work = [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i)
+      for (int j = 0; j < wy; ++j)
+        for (int k = 0; k < wz; ++k) {
+          float e = 0.0f;
+          for (int c = 0; c < C; ++c)
+            for (int m = 0; m < bx; ++m)
+              for (int n = 0; n < by; ++n) {
+                const float* lp = L + (((size_t)c * bx + m) * by + n) * bz;
+                const float* pp = P + (size_t)c * G + ((size_t)(m + i) * N + (n + j)) * N + k;
+                for (int o = 0; o < bz; ++o) e = e + lp[o] * pp[o];
+              }
+          E[((size_t)i * wy + j) * wz + k] = e;
+        }
+  };
+  const int nt = n_threads(wx);
   std::vector<std::thread> th;
   for (int t = 0; t < nt; ++t) th.emplace_back(work, wx * t / nt, wx * (t + 1) / nt);
   for (auto& t : th) t.join();

@@ -157,6 +157,22 @@ def argmin_first(E):
     return e.value, idx
 
 
+def set_threads(n):
+    _orc.orc_set_threads(int(n))
+
+
+def correlate_window_full(Pused, L, win):
+    """The literal cpuNaive loop (every box cell), FP32."""
+    P = _f32(Pused)
+    L = _f32(L)
+    Cn, bx, by, bz = L.shape
+    N = P.shape[-1]
+    wx, wy, wz = [int(w) for w in win]
+    E = np.empty((wx, wy, wz), np.float32)
+    _orc.orc_correlate_window_full_f32(_fp(P), N, _fp(L), Cn, bx, by, bz, wx, wy, wz, _fp(E))
+    return E
+
+
 def correlate_window(Pused, L, win, double=False):
     """Dense direct correlation (cpuNaive.cpp:184-216) of a ligand box L
     [C][bx][by][bz] with protein channels Pused [C][N][N][N] over win."""

@@ -231,6 +231,8 @@ def test_dense_fp32_is_cpunaive_order():
                                 e = np.float32(e + np.float32(L[c, m, n, o] * P[c, m + i, n + j, o + k]))
                 ref[i, j, k] = e
     assert np.array_equal(bits(E), bits(ref))
+    # the literal loop (zeros included, the timed CPU baseline) is bit-identical
+    assert np.array_equal(bits(orc.correlate_window_full(P, L, win)), bits(ref))
 
 
 def test_constant_grid_known_answer():
