@@ -128,11 +128,12 @@ template <int S> struct SmallDFT<16, S> {
 // ---------------------------------------------------------------------------
 template <int N, int R, int NS, int S, bool LINE_FAST, int NLINES_CT>
 __device__ __forceinline__ void stockham_pass(const float2* __restrict__ in, float2* __restrict__ out,
-                                              int nlines_rt, int es, int ls, const float2* __restrict__ tw) {
+                                              int nlines_rt, int es, int ls, const float2* __restrict__ tw,
+                                              int tid, int nthreads) {
   constexpr int NB = N / R;
   const int nlines = NLINES_CT > 0 ? NLINES_CT : nlines_rt;
   const int total = nlines * NB;
-  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+  for (int t = tid; t < total; t += nthreads) {
     int line, j;
     if constexpr (LINE_FAST) {
       if constexpr (NLINES_CT > 0) {
@@ -169,20 +170,37 @@ __device__ __forceinline__ void stockham_pass(const float2* __restrict__ in, flo
 
 template <int... Rs> struct RadixList {};
 
-template <int N, int S, bool LINE_FAST, int NLINES_CT, int NS>
-__device__ __forceinline__ float2* fft_passes(float2* a, float2*, int, int, int, const float2*, RadixList<>) {
+// Synchronisation between passes: the whole workgroup, or one wavefront
+// when each wavefront owns its lines and its LDS slice.
+struct BlockSync {
+  static __device__ __forceinline__ void sync() { __syncthreads(); }
+  static __device__ __forceinline__ int tid() { return threadIdx.x; }
+  static __device__ __forceinline__ int count() { return blockDim.x; }
+};
+struct WaveSync {
+  static __device__ __forceinline__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  static __device__ __forceinline__ int tid() { return threadIdx.x & 63; }
+  static __device__ __forceinline__ int count() { return 64; }
+};
+
+template <int N, int S, bool LINE_FAST, int NLINES_CT, int NS, class SY>
+__device__ __forceinline__ float2* fft_passes(float2* a, float2*, int, int, int, const float2*, RadixList<>, SY) {
   return a;
 }
 
 // Runs all passes; returns the buffer holding the natural-order result.
-// Caller must __syncthreads() before (data in `a` visible) — each pass ends
-// with a barrier so the result is visible on return.
-template <int N, int S, bool LINE_FAST, int NLINES_CT, int NS, int R, int... Rest>
+// Caller must synchronise before (data in `a` visible); each pass ends with
+// a sync so the result is visible on return.
+template <int N, int S, bool LINE_FAST, int NLINES_CT, int NS, class SY, int R, int... Rest>
 __device__ __forceinline__ float2* fft_passes(float2* a, float2* b, int nlines, int es, int ls, const float2* tw,
-                                              RadixList<R, Rest...>) {
-  stockham_pass<N, R, NS, S, LINE_FAST, NLINES_CT>(a, b, nlines, es, ls, tw);
-  __syncthreads();
-  return fft_passes<N, S, LINE_FAST, NLINES_CT, NS * R>(b, a, nlines, es, ls, tw, RadixList<Rest...>{});
+                                              RadixList<R, Rest...>, SY sy) {
+  stockham_pass<N, R, NS, S, LINE_FAST, NLINES_CT>(a, b, nlines, es, ls, tw, SY::tid(), SY::count());
+  SY::sync();
+  return fft_passes<N, S, LINE_FAST, NLINES_CT, NS * R>(b, a, nlines, es, ls, tw, RadixList<Rest...>{}, sy);
 }
 
 template <int... Rs> struct RadixProduct;
@@ -204,7 +222,13 @@ template <int N_, int... Rs> struct Plan {
 
 template <class P, int S, bool LINE_FAST, int NLINES_CT = 0>
 __device__ __forceinline__ float2* lds_fft(float2* a, float2* b, int nlines, int es, int ls, const float2* tw) {
-  return fft_passes<P::N, S, LINE_FAST, NLINES_CT, 1>(a, b, nlines, es, ls, tw, typename P::radices{});
+  return fft_passes<P::N, S, LINE_FAST, NLINES_CT, 1>(a, b, nlines, es, ls, tw, typename P::radices{}, BlockSync{});
+}
+
+// Same transform run by one wavefront on lines it owns.
+template <class P, int S, bool LINE_FAST, int NLINES_CT = 0>
+__device__ __forceinline__ float2* wave_fft(float2* a, float2* b, int nlines, int es, int ls, const float2* tw) {
+  return fft_passes<P::N, S, LINE_FAST, NLINES_CT, 1>(a, b, nlines, es, ls, tw, typename P::radices{}, WaveSync{});
 }
 
 }  // namespace cdk

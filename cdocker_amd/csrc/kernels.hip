@@ -26,8 +26,8 @@ namespace cdk {
 using P32a = Plan<32, 8, 4>;
 using P48a = Plan<48, 16, 3>;
 using P48b = Plan<48, 4, 4, 3>;
-using P64a = Plan<64, 4, 4, 4>;
-using P64b = Plan<64, 8, 8>;
+using P64a = Plan<64, 8, 8>;
+using P64b = Plan<64, 4, 4, 4>;
 using P96a = Plan<96, 8, 4, 3>;
 using P96b = Plan<96, 16, 2, 3>;
 using P128a = Plan<128, 8, 16>;
@@ -539,35 +539,43 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
   for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op[t] = res[t];
 }
 
-// Stage B: one workgroup per (ky, kz tile, group of poses).  The protein
-// spectra of the tile (all protein channels) are staged in LDS once and reused
-// for every pose of the group.  Per pose: x FFT of each used channel (input
-// non-zero for x < box), accumulate conj(L)*P over channels in registers,
-// scale by 1/G, inverse x FFT, keep x' < window.  Writes T[slot][x'][Y][ZH].
-template <class P, int TK>
+// Stage B: x FFT of every used channel, conj-multiply with the protein
+// spectra, channel sum, 1/G, inverse x FFT, for the (ky, kz) lines of the
+// spectrum.  Each wavefront owns 8 consecutive lines (flattened ky*ZH + kz)
+// and runs its transforms in its own LDS slice with wavefront-level
+// synchronisation only; a workgroup is 4 wavefronts = 32 lines and walks a
+// group of poses.  Workgroup w runs on XCD w % 8; line groups are split so
+// each XCD touches one eighth of the protein spectra, which then stays in
+// that XCD's L2 across the group of poses.  Reads A[slot][c][x][line] for
+// x < box, writes T[slot][x'][line] for x' < window.
+template <class P>
 __global__ __launch_bounds__(kBlock) void k_mid(const PoseSlot* __restrict__ slots, const LigandDesc* __restrict__ ligs,
                                                 const RotMeta* __restrict__ meta, int nslots, int group,
                                                 const float2* __restrict__ A, int cmax, int boxmax,
-                                                const float2* __restrict__ Phat, int npch,
-                                                float2* __restrict__ T, float invG,
+                                                const float2* __restrict__ Phat, float2* __restrict__ T, float invG,
                                                 const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
-  constexpr int E = N * TK;
-  constexpr int PER = (E + kBlock - 1) / kBlock;
-  extern __shared__ float2 Pt[];  // [npch][N][TK]
-  __shared__ float2 sA[E], sB[E], tw[N];
-  const int ky = blockIdx.x;
-  const int kz0 = blockIdx.y * TK;
-  const int nk = min(TK, ZH - kz0);
+  constexpr int LINES = 8;               // lines per wavefront
+  constexpr int E = LINES * N;           // elements per wavefront slice
+  constexpr int PER = E / 64;            // elements per lane
+  constexpr int NL = N * ZH;             // lines in a spectrum
+  constexpr int NLG = (NL + 4 * LINES - 1) / (4 * LINES);
+  constexpr int PERX = (NLG + 7) / 8;    // line groups per XCD
+  __shared__ float2 sbuf[4][2][E];
+  __shared__ float2 tw[N];
+  const int w = blockIdx.x;
+  const int lg = (w & 7) * PERX + ((w >> 3) % PERX);
+  const int sg = (w >> 3) / PERX;
+  if (lg >= NLG) return;
   load_twiddles<N>(tw, twg);
-  for (int t = threadIdx.x; t < npch * E; t += blockDim.x) {
-    const int pc = t / E, e = t - (t / E) * E;
-    const int kx = e / TK, j = e - (e / TK) * TK;
-    Pt[t] = (j < nk) ? Phat[(size_t)pc * N * N * ZH + ((size_t)kx * N + ky) * ZH + kz0 + j] : make_float2(0.f, 0.f);
-  }
-  const int s0 = blockIdx.z * group;
-  const int s1 = min(nslots, s0 + group);
-  const size_t plane = (size_t)N * ZH;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int l0 = lg * 4 * LINES + wave * LINES;
+  if (l0 >= NL) return;
+  float2* bufA = sbuf[wave][0];
+  float2* bufB = sbuf[wave][1];
+  const size_t plane = (size_t)NL;
+  const int s0 = sg * group, s1 = min(nslots, s0 + group);
   for (int s = s0; s < s1; ++s) {
     const RotMeta m = meta[s];
     if (!m.valid) continue;
@@ -577,38 +585,39 @@ __global__ __launch_bounds__(kBlock) void k_mid(const PoseSlot* __restrict__ slo
 #pragma unroll
     for (int i = 0; i < PER; ++i) acc[i] = make_float2(0.f, 0.f);
     for (int c = 0; c < L.nch; ++c) {
-      const float2* a = A + (((size_t)s * cmax + c) * boxmax) * plane + (size_t)ky * ZH + kz0;
-      __syncthreads();
-      for (int t = threadIdx.x; t < E; t += blockDim.x) {
-        const int x = t / TK, j = t - (t / TK) * TK;
-        sA[t] = (x < bx && j < nk) ? a[(size_t)x * plane + j] : make_float2(0.f, 0.f);
-      }
-      __syncthreads();
-      const float2* r = lds_fft<P, -1, true, TK>(sA, sB, TK, TK, 1, tw);
-      const float2* pt = Pt + L.pmap[c] * E;
+      const float2* a = A + (((size_t)s * cmax + c) * boxmax) * plane + l0;
+      WaveSync::sync();
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * kBlock;
-        if (e < E) {
-          const float2 l = r[e], p = pt[e];
-          acc[i].x += l.x * p.x + l.y * p.y;
-          acc[i].y += l.x * p.y - l.y * p.x;
-        }
+        const int e = lane + 64 * i;
+        const int x = e >> 3, ln = e & 7;
+        bufA[e] = (x < bx && l0 + ln < NL) ? a[(size_t)x * plane + ln] : make_float2(0.f, 0.f);
+      }
+      WaveSync::sync();
+      const float2* r = wave_fft<P, -1, true, LINES>(bufA, bufB, LINES, LINES, 1, tw);
+      const float2* pp = Phat + (size_t)L.pmap[c] * N * plane + l0;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = lane + 64 * i;
+        const int kx = e >> 3, ln = e & 7;
+        const float2 l = r[e];
+        const float2 p = (l0 + ln < NL) ? pp[(size_t)kx * plane + ln] : make_float2(0.f, 0.f);
+        acc[i].x += l.x * p.x + l.y * p.y;
+        acc[i].y += l.x * p.y - l.y * p.x;
       }
     }
-    __syncthreads();
+    WaveSync::sync();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) bufA[lane + 64 * i] = make_float2(acc[i].x * invG, acc[i].y * invG);
+    WaveSync::sync();
+    const float2* r = wave_fft<P, 1, true, LINES>(bufA, bufB, LINES, LINES, 1, tw);
+    float2* tp = T + (size_t)s * N * plane + l0;
+    const int wx = m.win[0];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + i * kBlock;
-      if (e < E) sA[e] = make_float2(acc[i].x * invG, acc[i].y * invG);
-    }
-    __syncthreads();
-    const float2* r = lds_fft<P, 1, true, TK>(sA, sB, TK, TK, 1, tw);
-    float2* tp = T + (size_t)s * N * plane + (size_t)ky * ZH + kz0;
-    const int wx = m.win[0];
-    for (int t = threadIdx.x; t < E; t += blockDim.x) {
-      const int x = t / TK, j = t - (t / TK) * TK;
-      if (x < wx && j < nk) tp[(size_t)x * plane + j] = r[t];
+      const int e = lane + 64 * i;
+      const int x = e >> 3, ln = e & 7;
+      if (x < wx && l0 + ln < NL) tp[(size_t)x * plane + ln] = r[e];
     }
   }
 }
@@ -810,37 +819,22 @@ hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int n
   return hipGetLastError();
 }
 
-template <int N> struct MidTile;
-template <> struct MidTile<32> { static constexpr int TK = 17; };
-template <> struct MidTile<48> { static constexpr int TK = 5; };
-template <> struct MidTile<64> { static constexpr int TK = 11; };
-template <> struct MidTile<96> { static constexpr int TK = 7; };
-template <> struct MidTile<128> { static constexpr int TK = 5; };
-
-int mid_tile(int N) {
-  switch (N) {
-    case 32: return MidTile<32>::TK;
-    case 48: return MidTile<48>::TK;
-    case 64: return MidTile<64>::TK;
-    case 96: return MidTile<96>::TK;
-    case 128: return MidTile<128>::TK;
-  }
-  return 0;
+int mid_line_groups(int N) {
+  const int NL = N * (N / 2 + 1);
+  const int NLG = (NL + 31) / 32;
+  return ((NLG + 7) / 8) * 8;
 }
 
 hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
                       int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, int npch,
                       float2* T, float invG, const float2* tw) {
+  (void)npch;
   if (nslots <= 0) return hipSuccess;
+  const unsigned nsg = ceil_div(nslots, group);
+  const unsigned grid = (unsigned)mid_line_groups(N) * nsg;
   CDK_DISPATCH_N(N, {
-    constexpr int TK = MidTile<PL::N>::TK;
-    const unsigned tiles = ceil_div(PL::ZH, TK);
-    const size_t dyn = (size_t)npch * PL::N * TK * sizeof(float2);
-    auto kern = k_mid<PL, TK>;
-    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(PL::N, tiles, ceil_div(nslots, group)), dim3(kBlock), dyn, st, slots, ligs, meta,
-                       nslots, group, A, cmax, boxmax, Phat, npch, T, invG, tw);
+    hipLaunchKernelGGL(k_mid<PL>, dim3(grid), dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, A, cmax, boxmax,
+                       Phat, T, invG, tw);
   });
   return hipGetLastError();
 }

@@ -7,7 +7,7 @@
 namespace cdk {
 
 bool grid_edge_supported(int N);
-int mid_tile(int N);
+int mid_line_groups(int N);
 
 hipError_t launch_prep(hipStream_t st, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
                        const float* coords, int N, float h, int astride, int boxmax, float* rot, AtomGrid* ag,
