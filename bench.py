@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=12, help="rotations timed on the CPU (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
+    ap.add_argument("--group", type=int, default=16, help="poses per mid-kernel workgroup")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the search shards "
@@ -90,7 +92,7 @@ def main():
     w = workloads.config2(nconf=args.conformers, nquat=args.quaternions)
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
-    ctx = cdk.Context(0, use_graph=int(not args.no_graph))
+    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     nu, used, _ = lig.channels()
@@ -122,7 +124,7 @@ def main():
     ms, launches = ctx.stage_times(reset=True)
     ctx.set_option("time_stages", 0)
     names = ["prep", "fwd_planes", "mid_lines", "inv_argmin"]
-    kb = fused_kernel_bytes(box, win, C, prot.num_grids, N, 16)
+    kb = fused_kernel_bytes(box, win, C, prot.num_grids, N, args.group)
     kernels = {}
     for i, nm in enumerate(names):
         if launches[i]:

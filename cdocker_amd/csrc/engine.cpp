@@ -224,6 +224,7 @@ void build_tables(const Protein& prot, const std::vector<Job>& jobs, Prepared& t
     t.ligs.push_back(d);
     t.astride = std::max(t.astride, L.natom);
     t.cmax = std::max(t.cmax, d.nch);
+    for (int c = 0; c < d.nch; ++c) t.max_chan_atoms = std::max(t.max_chan_atoms, d.ch_start[c + 1] - d.ch_start[c]);
   }
   // Padding ligand: zero atoms, so its poses are never valid.
   LigandDesc dummy;
@@ -433,7 +434,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
     e = timer.begin(1);
     hip_check(launch_fwd_planes(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
                                 t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, t.cmax, t.boxmax,
-                                ctx.d_A.as<float2>(), tw),
+                                t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
               "fwd planes");
     timer.end(e);
     e = timer.begin(2);
@@ -472,6 +473,7 @@ void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& j
   t.astride = 0;
   t.cmax = 0;
   t.boxmax = 2;
+  t.max_chan_atoms = 1;
   build_tables(prot, jobs, t);
   for (const Job& j : jobs) t.job_n.push_back(j.npose());
   t.prot = &prot;

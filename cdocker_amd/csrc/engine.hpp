@@ -130,7 +130,7 @@ struct Prepared {
   std::vector<float> wv, wq, coords;
   std::vector<PoseSlot> slots;
   std::vector<size_t> job_first, job_n;
-  int astride = 0, cmax = 0, boxmax = 2;
+  int astride = 0, cmax = 0, boxmax = 2, max_chan_atoms = 1;
   int dummy_lig = 0;
   DevBuf d_ligs, d_chan, d_wv, d_wq, d_coords, d_slots, d_keys, d_meta;
   hipGraphExec_t graph = nullptr;

@@ -10,6 +10,7 @@
 //     inverse x FFT, protein spectra staged in LDS and reused across poses)
 //     -> (inverse y FFT + z C2R of the window rows only + argmin).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 
@@ -459,10 +460,11 @@ struct PlaneAtom {
   double wx, fy, fz;
 };
 
-// Stage A: one workgroup per (pose, ligand channel, occupied x-plane).
-// Spreads the plane in LDS (FillLigandGrid arithmetic), z R2C of the occupied
-// rows, y FFT of all columns, writes the plane spectrum
-// A[slot][c][x][Y][ZH].
+// Stage A: one workgroup per (pose, ligand channel).  The channel's atoms are
+// staged in LDS once; then for every occupied x-plane: spread the plane in
+// LDS (FillLigandGrid arithmetic), z R2C of the occupied rows (two real rows
+// per complex line), y FFT of the half-spectrum columns, and write the plane
+// spectrum A[slot][c][x][Y][ZH].  No global loads inside the plane loop.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restrict__ slots,
                                                        const LigandDesc* __restrict__ ligs,
@@ -473,70 +475,87 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
                                                        float2* __restrict__ A, const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
   __shared__ float2 sA[N * ZH], sB[N * ZH], tw[N];
-  __shared__ PlaneAtom pl[kMaxAtoms];
+  extern __shared__ double dyn[];  // [nmax] SpreadAtom then [nmax] PlaneAtom
   __shared__ int npl;
-  const int x = blockIdx.x, c = blockIdx.y, s = blockIdx.z;
+  const int c = blockIdx.x, s = blockIdx.y;
   const RotMeta m = meta[s];
-  if (!m.valid || x >= m.box[0]) return;
+  if (!m.valid) return;
   const PoseSlot ps = slots[s];
   const LigandDesc& L = ligs[ps.lig];
   if (c >= L.nch) return;
+  const int nc = L.ch_start[c + 1] - L.ch_start[c];
+  SpreadAtom* sa = reinterpret_cast<SpreadAtom*>(dyn);
+  PlaneAtom* pl = reinterpret_cast<PlaneAtom*>(sa + nc);
   load_twiddles<N>(tw, twg);
-  // Ordered compaction of the channel's atoms that touch plane x.
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
+  {
     const int* ca = chan_atoms + L.list_off + L.ch_start[c];
-    const int nc = L.ch_start[c + 1] - L.ch_start[c];
-    int cnt = 0;
-    for (int b = 0; b < nc; b += 64) {
-      const int i = b + lane;
-      int a = 0, dx = -1;
-      AtomGrid g;
-      if (i < nc) {
-        a = ca[i];
-        g = ag[(size_t)s * astride + a];
-        dx = x - g.ix;
-      }
-      const bool take = (i < nc) && (dx == 0 || dx == 1);
-      const unsigned long long msk = __ballot(take);
-      if (take) {
-        const int pos = cnt + __popcll(msk & ((1ull << lane) - 1ull));
-        PlaneAtom t;
-        t.iy = g.iy;
-        t.iz = g.iz;
-        t.wx = x_factor(channel_weight(L, c, a, wv, wq), g.fx, dx);
-        t.fy = g.fy;
-        t.fz = g.fz;
-        pl[pos] = t;
-      }
-      cnt += __popcll(msk);
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+      const int a = ca[i];
+      const AtomGrid g = ag[(size_t)s * astride + a];
+      SpreadAtom t;
+      t.ix = g.ix;
+      t.iy = g.iy;
+      t.iz = g.iz;
+      t.w = channel_weight(L, c, a, wv, wq);
+      t.fx = g.fx;
+      t.fy = g.fy;
+      t.fz = g.fz;
+      sa[i] = t;
     }
-    if (lane == 0) npl = cnt;
   }
-  __syncthreads();
-  const int by = m.box[1], bz = m.box[2];
+  const int bxm = min(m.box[0], boxmax), by = m.box[1], bz = m.box[2];
   const int npair = (by + 1) >> 1;
-  const int n = npl;
-  for (int t = threadIdx.x; t < npair * 2 * N; t += blockDim.x) {
-    const int y = t / N, z = t - (t / N) * N;
-    float acc = 0.f;
-    if (y < by && z < bz) {
-      for (int i = 0; i < n; ++i) {
-        const int dy = y - pl[i].iy, dz = z - pl[i].iz;
-        if ((unsigned)dy <= 1u && (unsigned)dz <= 1u) acc = accum_corner(acc, pl[i].wx, pl[i].fy, pl[i].fz, dy, dz);
+  float2* op0 = A + (((size_t)s * cmax + c) * boxmax) * (size_t)(N * ZH);
+  for (int x = 0; x < bxm; ++x) {
+    __syncthreads();  // atoms staged; previous plane written out
+    // Ordered compaction of the atoms touching plane x.
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      int cnt = 0;
+      for (int b = 0; b < nc; b += 64) {
+        const int i = b + lane;
+        const int dx = (i < nc) ? x - sa[i].ix : -1;
+        const bool take = (i < nc) && (dx == 0 || dx == 1);
+        const unsigned long long msk = __ballot(take);
+        if (take) {
+          const int pos = cnt + __popcll(msk & ((1ull << lane) - 1ull));
+          PlaneAtom t;
+          t.iy = sa[i].iy;
+          t.iz = sa[i].iz;
+          t.wx = x_factor(sa[i].w, sa[i].fx, dx);
+          t.fy = sa[i].fy;
+          t.fz = sa[i].fz;
+          pl[pos] = t;
+        }
+        cnt += __popcll(msk);
       }
+      if (lane == 0) npl = cnt;
     }
-    float* cell = reinterpret_cast<float*>(&sA[(y >> 1) * N + z]);
-    cell[y & 1] = acc;
+    __syncthreads();
+    const int n = npl;
+    // Spread into packed rows: row pair p holds rows 2p (.x) and 2p+1 (.y).
+    for (int t = threadIdx.x; t < npair * 2 * N; t += blockDim.x) {
+      const int y = t / N, z = t - (t / N) * N;
+      float acc = 0.f;
+      if (y < by && z < bz) {
+        for (int i = 0; i < n; ++i) {
+          const int dy = y - pl[i].iy, dz = z - pl[i].iz;
+          if ((unsigned)dy <= 1u && (unsigned)dz <= 1u)
+            acc = accum_corner(acc, pl[i].wx, pl[i].fy, pl[i].fz, dy, dz);
+        }
+      }
+      float* cell = reinterpret_cast<float*>(&sA[(y >> 1) * N + z]);
+      cell[y & 1] = acc;
+    }
+    __syncthreads();
+    float2* r = lds_fft<P, -1, false>(sA, sB, npair, 1, N, tw);
+    float2* u = (r == sA) ? sB : sA;
+    unpack_rows<N>(r, u, 2 * npair);
+    __syncthreads();
+    const float2* res = lds_fft<P, -1, true, ZH>(u, r, ZH, ZH, 1, tw);
+    float2* op = op0 + (size_t)x * (N * ZH);
+    for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op[t] = res[t];
   }
-  __syncthreads();
-  float2* r = lds_fft<P, -1, false>(sA, sB, npair, 1, N, tw);
-  float2* u = (r == sA) ? sB : sA;
-  unpack_rows<N>(r, u, 2 * npair);
-  __syncthreads();
-  float2* res = lds_fft<P, -1, true, ZH>(u, r, ZH, ZH, 1, tw);
-  float2* op = A + (((size_t)s * cmax + c) * boxmax + x) * (size_t)(N * ZH);
-  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op[t] = res[t];
 }
 
 // Stage B: x FFT of every used channel, conj-multiply with the protein
@@ -544,10 +563,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
 // spectrum.  Each wavefront owns 8 consecutive lines (flattened ky*ZH + kz)
 // and runs its transforms in its own LDS slice with wavefront-level
 // synchronisation only; a workgroup is 4 wavefronts = 32 lines and walks a
-// group of poses.  Workgroup w runs on XCD w % 8; line groups are split so
-// each XCD touches one eighth of the protein spectra, which then stays in
-// that XCD's L2 across the group of poses.  Reads A[slot][c][x][line] for
-// x < box, writes T[slot][x'][line] for x' < window.
+// group of poses.  The loads of the next (pose, channel) pair are issued
+// before the current transform so their latency overlaps it.  Workgroup w
+// runs on XCD w % 8; line groups are split so each XCD touches one eighth of
+// the protein spectra, which then stays in that XCD's L2.  Reads
+// A[slot][c][x][line] for x < box, writes T[slot][x'][line] for x' < window.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_mid(const PoseSlot* __restrict__ slots, const LigandDesc* __restrict__ ligs,
                                                 const RotMeta* __restrict__ meta, int nslots, int group,
@@ -575,33 +595,67 @@ __global__ __launch_bounds__(kBlock) void k_mid(const PoseSlot* __restrict__ slo
   float2* bufA = sbuf[wave][0];
   float2* bufB = sbuf[wave][1];
   const size_t plane = (size_t)NL;
-  const int s0 = sg * group, s1 = min(nslots, s0 + group);
-  for (int s = s0; s < s1; ++s) {
-    const RotMeta m = meta[s];
-    if (!m.valid) continue;
+  const int s1 = min(nslots, sg * group + group);
+  const bool full = l0 + LINES <= NL;
+  auto load_a = [&](int s, int c, int bx, float2* ra) {
+    const float2* a = A + (((size_t)s * cmax + c) * boxmax) * plane + l0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = lane + 64 * i;
+      const int x = e >> 3, ln = e & 7;
+      ra[i] = (x < bx && (full || l0 + ln < NL)) ? a[(size_t)x * plane + ln] : make_float2(0.f, 0.f);
+    }
+  };
+  auto load_p = [&](int pc, float2* rp) {
+    const float2* pp = Phat + (size_t)pc * N * plane + l0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = lane + 64 * i;
+      const int kx = e >> 3, ln = e & 7;
+      rp[i] = (full || l0 + ln < NL) ? pp[(size_t)kx * plane + ln] : make_float2(0.f, 0.f);
+    }
+  };
+  auto next_valid = [&](int s) {
+    while (s < s1 && !meta[s].valid) ++s;
+    return s;
+  };
+  int s = next_valid(sg * group);
+  if (s >= s1) return;
+  float2 ra[PER], rp[PER];
+  {
     const LigandDesc& L = ligs[slots[s].lig];
-    const int bx = min(m.box[0], boxmax);
+    load_a(s, 0, min(meta[s].box[0], boxmax), ra);
+    load_p(L.pmap[0], rp);
+  }
+  for (;;) {
+    const LigandDesc& L = ligs[slots[s].lig];
+    const int nch = L.nch;
+    const int bx = min(meta[s].box[0], boxmax);
+    const int wx = meta[s].win[0];
+    const int sn = next_valid(s + 1);
     float2 acc[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) acc[i] = make_float2(0.f, 0.f);
-    for (int c = 0; c < L.nch; ++c) {
-      const float2* a = A + (((size_t)s * cmax + c) * boxmax) * plane + l0;
+    for (int c = 0; c < nch; ++c) {
       WaveSync::sync();
+      float2 pc_[PER];
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
-        const int e = lane + 64 * i;
-        const int x = e >> 3, ln = e & 7;
-        bufA[e] = (x < bx && l0 + ln < NL) ? a[(size_t)x * plane + ln] : make_float2(0.f, 0.f);
+        bufA[lane + 64 * i] = ra[i];
+        pc_[i] = rp[i];
+      }
+      if (c + 1 < nch) {
+        load_a(s, c + 1, bx, ra);
+        load_p(L.pmap[c + 1], rp);
+      } else if (sn < s1) {
+        load_a(sn, 0, min(meta[sn].box[0], boxmax), ra);
+        load_p(ligs[slots[sn].lig].pmap[0], rp);
       }
       WaveSync::sync();
       const float2* r = wave_fft<P, -1, true, LINES>(bufA, bufB, LINES, LINES, 1, tw);
-      const float2* pp = Phat + (size_t)L.pmap[c] * N * plane + l0;
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
-        const int e = lane + 64 * i;
-        const int kx = e >> 3, ln = e & 7;
-        const float2 l = r[e];
-        const float2 p = (l0 + ln < NL) ? pp[(size_t)kx * plane + ln] : make_float2(0.f, 0.f);
+        const float2 l = r[lane + 64 * i], p = pc_[i];
         acc[i].x += l.x * p.x + l.y * p.y;
         acc[i].y += l.x * p.y - l.y * p.x;
       }
@@ -612,13 +666,14 @@ __global__ __launch_bounds__(kBlock) void k_mid(const PoseSlot* __restrict__ slo
     WaveSync::sync();
     const float2* r = wave_fft<P, 1, true, LINES>(bufA, bufB, LINES, LINES, 1, tw);
     float2* tp = T + (size_t)s * N * plane + l0;
-    const int wx = m.win[0];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int e = lane + 64 * i;
       const int x = e >> 3, ln = e & 7;
-      if (x < wx && l0 + ln < NL) tp[(size_t)x * plane + ln] = r[e];
+      if (x < wx && (full || l0 + ln < NL)) tp[(size_t)x * plane + ln] = r[e];
     }
+    if (sn >= s1) break;
+    s = sn;
   }
 }
 
@@ -810,11 +865,13 @@ hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* met
 
 hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
                              const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
-                             int astride, const RotMeta* meta, int cmax, int boxmax, float2* A, const float2* tw) {
+                             int astride, const RotMeta* meta, int cmax, int boxmax, int max_chan_atoms, float2* A,
+                             const float2* tw) {
   if (nslots <= 0) return hipSuccess;
+  const size_t dyn = (size_t)std::max(1, max_chan_atoms) * (sizeof(SpreadAtom) + sizeof(PlaneAtom));
   CDK_DISPATCH_N(N, {
-    hipLaunchKernelGGL(k_fwd_planes<PL>, dim3(boxmax, cmax, nslots), dim3(kBlock), 0, st, slots, ligs, chan_atoms,
-                       wv, wq, ag, astride, meta, cmax, boxmax, A, tw);
+    hipLaunchKernelGGL(k_fwd_planes<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms, wv, wq,
+                       ag, astride, meta, cmax, boxmax, A, tw);
   });
   return hipGetLastError();
 }

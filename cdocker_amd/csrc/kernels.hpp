@@ -28,7 +28,8 @@ hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* met
 // Fused, pruned pipeline stages.
 hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
                              const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
-                             int astride, const RotMeta* meta, int cmax, int boxmax, float2* A, const float2* tw);
+                             int astride, const RotMeta* meta, int cmax, int boxmax, int max_chan_atoms, float2* A,
+                             const float2* tw);
 hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
                       int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, int npch,
                       float2* T, float invG, const float2* tw);
