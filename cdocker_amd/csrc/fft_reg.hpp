@@ -1,0 +1,134 @@
+// Register-resident four-step FFT of N = V * L points run by one wavefront.
+//
+// A line is held by L lanes, V values each: lane (line, n2) holds
+// x[n2 + L*n1] in register n1.  The transform is
+//   1. DFT_V over the lane's registers,
+//   2. twiddle W_N^(n2*k1),
+//   3. one transpose through LDS (XOR-swizzled so that both the write and
+//      the read touch every 8-byte bank slot exactly twice per instruction),
+//   4. V/L DFT_L per lane over the values gathered from the L lanes,
+// and leaves lane (line, j) holding X[j + L*m] in register m -- the same
+// layout as the input, so a forward transform, a pointwise product and an
+// inverse transform chain with no reordering.  64/L lines per wavefront.
+#pragma once
+#include "fft_lds.hpp"
+
+namespace cdk {
+
+template <int M> struct RootTable;
+
+template <> struct RootTable<12> {
+  static __device__ __forceinline__ float cosv(int m) {
+    constexpr float t[12] = {1.000000000e+00f, 8.660254038e-01f, 5.000000000e-01f, 0.0f, -5.000000000e-01f, -8.660254038e-01f, -1.000000000e+00f, -8.660254038e-01f, -5.000000000e-01f, 0.0f, 5.000000000e-01f, 8.660254038e-01f};
+    return t[m];
+  }
+  static __device__ __forceinline__ float sinv(int m) {
+    constexpr float t[12] = {0.000000000e+00f, 5.000000000e-01f, 8.660254038e-01f, 1.000000000e+00f, 8.660254038e-01f, 5.000000000e-01f, 0.0f, -5.000000000e-01f, -8.660254038e-01f, -1.000000000e+00f, -8.660254038e-01f, -5.000000000e-01f};
+    return t[m];
+  }
+};
+
+template <> struct RootTable<24> {
+  static __device__ __forceinline__ float cosv(int m) {
+    constexpr float t[24] = {1.000000000e+00f, 9.659258263e-01f, 8.660254038e-01f, 7.071067812e-01f, 5.000000000e-01f, 2.588190451e-01f, 0.0f, -2.588190451e-01f, -5.000000000e-01f, -7.071067812e-01f, -8.660254038e-01f, -9.659258263e-01f, -1.000000000e+00f, -9.659258263e-01f, -8.660254038e-01f, -7.071067812e-01f, -5.000000000e-01f, -2.588190451e-01f, 0.0f, 2.588190451e-01f, 5.000000000e-01f, 7.071067812e-01f, 8.660254038e-01f, 9.659258263e-01f};
+    return t[m];
+  }
+  static __device__ __forceinline__ float sinv(int m) {
+    constexpr float t[24] = {0.000000000e+00f, 2.588190451e-01f, 5.000000000e-01f, 7.071067812e-01f, 8.660254038e-01f, 9.659258263e-01f, 1.000000000e+00f, 9.659258263e-01f, 8.660254038e-01f, 7.071067812e-01f, 5.000000000e-01f, 2.588190451e-01f, 0.0f, -2.588190451e-01f, -5.000000000e-01f, -7.071067812e-01f, -8.660254038e-01f, -9.659258263e-01f, -1.000000000e+00f, -9.659258263e-01f, -8.660254038e-01f, -7.071067812e-01f, -5.000000000e-01f, -2.588190451e-01f};
+    return t[m];
+  }
+};
+
+// Composite small DFT R1*R2 in registers (Cooley-Tukey, input index
+// R2*n1 + n2, output index k1 + R1*k2).
+template <int R1, int R2, int S>
+__device__ __forceinline__ void compose_dft(float2* v) {
+  constexpr int M = R1 * R2;
+  float2 z[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) z[n2][n1] = v[R2 * n1 + n2];
+    SmallDFT<R1, S>::run(z[n2]);
+  }
+#pragma unroll
+  for (int n2 = 1; n2 < R2; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) {
+      const int m = (n2 * k1) % M;
+      const float2 w = make_float2(RootTable<M>::cosv(m), S > 0 ? RootTable<M>::sinv(m) : -RootTable<M>::sinv(m));
+      z[n2][k1] = c_mul(z[n2][k1], w);
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    float2 t[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) t[n2] = z[n2][k1];
+    SmallDFT<R2, S>::run(t);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = t[k2];
+  }
+}
+
+template <int R, int S> struct RegDFT {
+  static __device__ __forceinline__ void run(float2* v) { SmallDFT<R, S>::run(v); }
+};
+template <int S> struct RegDFT<12, S> {
+  static __device__ __forceinline__ void run(float2* v) { compose_dft<3, 4, S>(v); }
+};
+template <int S> struct RegDFT<24, S> {
+  static __device__ __forceinline__ void run(float2* v) { compose_dft<3, 8, S>(v); }
+};
+
+// Lanes per line for each transform length.
+template <int N> struct RegPlan;
+template <> struct RegPlan<32> { static constexpr int L = 4; };
+template <> struct RegPlan<48> { static constexpr int L = 4; };
+template <> struct RegPlan<64> { static constexpr int L = 8; };
+template <> struct RegPlan<96> { static constexpr int L = 4; };
+template <> struct RegPlan<128> { static constexpr int L = 8; };
+
+template <int N> struct RegShape {
+  static constexpr int L = RegPlan<N>::L;
+  static constexpr int V = N / L;
+  static constexpr int LPW = 64 / L;  // lines per wavefront
+  static_assert(V % L == 0, "four-step split needs V divisible by L");
+};
+
+// Per-lane twiddles W_N^(n2*k1), k1 < V, from the forward table
+// tw[m] = exp(-2 pi i m / N).
+template <int N>
+__device__ __forceinline__ void reg_twiddles(const float2* __restrict__ tw, int n2, float2* twl) {
+#pragma unroll
+  for (int k1 = 0; k1 < RegShape<N>::V; ++k1) twl[k1] = tw[n2 * k1];
+}
+
+// One transform of the wavefront's lines.  `lds` is the wavefront's slice of
+// V*64 float2.  ln = line within the wavefront, n2 = lane within the line.
+template <int N, int S>
+__device__ __forceinline__ void reg_fft(float2* v, const float2* twl, float2* lds, int ln, int n2) {
+  constexpr int L = RegShape<N>::L, V = RegShape<N>::V, LPW = RegShape<N>::LPW;
+  RegDFT<V, S>::run(v);
+#pragma unroll
+  for (int k1 = 1; k1 < V; ++k1) {
+    float2 w = twl[k1];
+    if (S > 0) w.y = -w.y;
+    v[k1] = c_mul(v[k1], w);
+  }
+  WaveSync::sync();  // previous reads of the slice are done
+#pragma unroll
+  for (int k1 = 0; k1 < V; ++k1) lds[(k1 * L + (n2 ^ (k1 % L))) * LPW + ln] = v[k1];
+  WaveSync::sync();
+  const int j = n2;
+#pragma unroll
+  for (int q = 0; q < V / L; ++q) {
+    float2 u[L];
+#pragma unroll
+    for (int m2 = 0; m2 < L; ++m2) u[m2] = lds[((j + L * q) * L + (m2 ^ j)) * LPW + ln];
+    SmallDFT<L, S>::run(u);
+#pragma unroll
+    for (int k2 = 0; k2 < L; ++k2) v[q + (V / L) * k2] = u[k2];
+  }
+}
+
+}  // namespace cdk

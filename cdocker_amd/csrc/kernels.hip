@@ -16,6 +16,7 @@
 
 #include "cdk_types.hpp"
 #include "fft_lds.hpp"
+#include "fft_reg.hpp"
 #include "kernels.hpp"
 
 namespace cdk {
@@ -71,6 +72,25 @@ __device__ __forceinline__ void block_min_commit(unsigned long long v, unsigned 
     if (m != ~0ull) atomicMin(dst, m);
   }
 }
+
+// ---------------------------------------------------------------------------
+// Raw buffer access.  A descriptor covers [base, base + bytes); loads past the
+// end return zero and stores past the end are dropped, which is how the
+// kernels zero-pad the ligand box and clip the translation window without
+// per-element compares.  The descriptor is wave-uniform (scalar registers).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buffer_load_f2(__amdgpu_buffer_rsrc_t r, int voff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+__device__ __forceinline__ void buffer_store_f2(float2 v, __amdgpu_buffer_rsrc_t r, int voff) {
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, 0, 0);
+}
+// Offset that is past every descriptor (lanes with no line to serve).
+constexpr int kVoffNone = 0x40000000;
 
 // ---------------------------------------------------------------------------
 // Rotate.cpp:4-27 with QuaternionMultiply.cpp:3-11 term order, FP32, no FMA
@@ -475,8 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
                                                        float2* __restrict__ A, const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
   __shared__ float2 sA[N * ZH], sB[N * ZH], tw[N];
-  extern __shared__ double dyn[];  // [nmax] SpreadAtom then [nmax] PlaneAtom
-  __shared__ int npl;
+  extern __shared__ double dyn[];  // [nmax] SpreadAtom
   const int c = blockIdx.x, s = blockIdx.y;
   const RotMeta m = meta[s];
   if (!m.valid) return;
@@ -485,7 +504,6 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
   if (c >= L.nch) return;
   const int nc = L.ch_start[c + 1] - L.ch_start[c];
   SpreadAtom* sa = reinterpret_cast<SpreadAtom*>(dyn);
-  PlaneAtom* pl = reinterpret_cast<PlaneAtom*>(sa + nc);
   load_twiddles<N>(tw, twg);
   {
     const int* ca = chan_atoms + L.list_off + L.ch_start[c];
@@ -503,49 +521,32 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
       sa[i] = t;
     }
   }
-  const int bxm = min(m.box[0], boxmax), by = m.box[1], bz = m.box[2];
+  const int bxm = min(m.box[0], boxmax), by = m.box[1];
   const int npair = (by + 1) >> 1;
   float2* op0 = A + (((size_t)s * cmax + c) * boxmax) * (size_t)(N * ZH);
   for (int x = 0; x < bxm; ++x) {
     __syncthreads();  // atoms staged; previous plane written out
-    // Ordered compaction of the atoms touching plane x.
+    // Zero the packed rows (row pair p holds rows 2p in .x, 2p+1 in .y).
+    for (int t = threadIdx.x; t < npair * N; t += blockDim.x) sA[t] = make_float2(0.f, 0.f);
+    __syncthreads();
+    // Ordered scatter by one wavefront: atoms touching plane x in ascending
+    // order, the 4 corners of one atom in 4 lanes (distinct cells), so every
+    // cell accumulates its contributions in the reference's atom order
+    // (FillLigandGrid.cpp:45-66, 97-117).
     if (threadIdx.x < 64) {
       const int lane = threadIdx.x;
-      int cnt = 0;
-      for (int b = 0; b < nc; b += 64) {
-        const int i = b + lane;
-        const int dx = (i < nc) ? x - sa[i].ix : -1;
-        const bool take = (i < nc) && (dx == 0 || dx == 1);
-        const unsigned long long msk = __ballot(take);
-        if (take) {
-          const int pos = cnt + __popcll(msk & ((1ull << lane) - 1ull));
-          PlaneAtom t;
-          t.iy = sa[i].iy;
-          t.iz = sa[i].iz;
-          t.wx = x_factor(sa[i].w, sa[i].fx, dx);
-          t.fy = sa[i].fy;
-          t.fz = sa[i].fz;
-          pl[pos] = t;
+      const int dy = (lane >> 1) & 1, dz = lane & 1;
+      float* cells = reinterpret_cast<float*>(sA);
+      for (int i = 0; i < nc; ++i) {
+        const int dx = x - sa[i].ix;
+        if (dx != 0 && dx != 1) continue;  // wave-uniform
+        if (lane < 4) {
+          const int y = sa[i].iy + dy, z = sa[i].iz + dz;
+          float* cell = cells + (((y >> 1) * N + z) << 1) + (y & 1);
+          *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, dx), sa[i].fy, sa[i].fz, dy, dz);
         }
-        cnt += __popcll(msk);
+        WaveSync::sync();
       }
-      if (lane == 0) npl = cnt;
-    }
-    __syncthreads();
-    const int n = npl;
-    // Spread into packed rows: row pair p holds rows 2p (.x) and 2p+1 (.y).
-    for (int t = threadIdx.x; t < npair * 2 * N; t += blockDim.x) {
-      const int y = t / N, z = t - (t / N) * N;
-      float acc = 0.f;
-      if (y < by && z < bz) {
-        for (int i = 0; i < n; ++i) {
-          const int dy = y - pl[i].iy, dz = z - pl[i].iz;
-          if ((unsigned)dy <= 1u && (unsigned)dz <= 1u)
-            acc = accum_corner(acc, pl[i].wx, pl[i].fy, pl[i].fz, dy, dz);
-        }
-      }
-      float* cell = reinterpret_cast<float*>(&sA[(y >> 1) * N + z]);
-      cell[y & 1] = acc;
     }
     __syncthreads();
     float2* r = lds_fft<P, -1, false>(sA, sB, npair, 1, N, tw);
@@ -560,118 +561,126 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
 
 // Stage B: x FFT of every used channel, conj-multiply with the protein
 // spectra, channel sum, 1/G, inverse x FFT, for the (ky, kz) lines of the
-// spectrum.  Each wavefront owns 8 consecutive lines (flattened ky*ZH + kz)
-// and runs its transforms in its own LDS slice with wavefront-level
-// synchronisation only; a workgroup is 4 wavefronts = 32 lines and walks a
-// group of poses.  The loads of the next (pose, channel) pair are issued
-// before the current transform so their latency overlaps it.  Workgroup w
-// runs on XCD w % 8; line groups are split so each XCD touches one eighth of
-// the protein spectra, which then stays in that XCD's L2.  Reads
+// spectrum.  Each wavefront owns 64/L consecutive lines (flattened
+// ky*ZH + kz) held in registers, L lanes per line (register four-step FFT,
+// one swizzled LDS transpose per transform).  A line's data arrives from
+// global memory already in the register layout the transform wants, and the
+// forward output layout is the inverse's input layout.  A workgroup is 4
+// wavefronts and walks a group of poses; the loads of the next (pose,
+// channel) pair are issued before the current transform.  Workgroup w runs
+// on XCD w % 8 and the line groups are split so each XCD touches one eighth
+// of the protein spectra, which then stays in that XCD's L2.  Reads
 // A[slot][c][x][line] for x < box, writes T[slot][x'][line] for x' < window.
+#ifdef CDK_MID_WAVES
+#define CDK_MID_ATTR __attribute__((amdgpu_waves_per_eu(CDK_MID_WAVES)))
+#else
+#define CDK_MID_ATTR
+#endif
 template <class P>
-__global__ __launch_bounds__(kBlock) void k_mid(const PoseSlot* __restrict__ slots, const LigandDesc* __restrict__ ligs,
+__global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __restrict__ slots, const LigandDesc* __restrict__ ligs,
                                                 const RotMeta* __restrict__ meta, int nslots, int group,
                                                 const float2* __restrict__ A, int cmax, int boxmax,
                                                 const float2* __restrict__ Phat, float2* __restrict__ T, float invG,
                                                 const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
-  constexpr int LINES = 8;               // lines per wavefront
-  constexpr int E = LINES * N;           // elements per wavefront slice
-  constexpr int PER = E / 64;            // elements per lane
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
   constexpr int NL = N * ZH;             // lines in a spectrum
-  constexpr int NLG = (NL + 4 * LINES - 1) / (4 * LINES);
+  constexpr int LPB = 4 * LPW;           // lines per workgroup
+  constexpr int NLG = (NL + LPB - 1) / LPB;
   constexpr int PERX = (NLG + 7) / 8;    // line groups per XCD
-  __shared__ float2 sbuf[4][2][E];
-  __shared__ float2 tw[N];
+  constexpr bool PREFETCH = V <= 16;
+  __shared__ float2 sbuf[4][V * 64];
   const int w = blockIdx.x;
   const int lg = (w & 7) * PERX + ((w >> 3) % PERX);
   const int sg = (w >> 3) / PERX;
   if (lg >= NLG) return;
-  load_twiddles<N>(tw, twg);
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int l0 = lg * 4 * LINES + wave * LINES;
+  // Wave index and line offset are wave-uniform; say so, so the buffer
+  // descriptors built from them live in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int ln = lane % LPW, n2 = lane / LPW;
+  const int l0 = __builtin_amdgcn_readfirstlane(lg * LPB + wave * LPW);
   if (l0 >= NL) return;
-  float2* bufA = sbuf[wave][0];
-  float2* bufB = sbuf[wave][1];
+  const bool lok = l0 + ln < NL;
   const size_t plane = (size_t)NL;
+  float2 twl[V];
+  reg_twiddles<N>(twg, n2, twl);
+  float2* lds = sbuf[wave];
   const int s1 = min(nslots, sg * group + group);
-  const bool full = l0 + LINES <= NL;
+  // Lane offset of row x = n2 within a plane-major block; row n1 adds L*n1
+  // planes.  Rows past the descriptor's end (x >= box, x' >= window) read as
+  // zero / are dropped.
+  const int voff0 = lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone;
+  constexpr int kRow = L * NL * (int)sizeof(float2);
   auto load_a = [&](int s, int c, int bx, float2* ra) {
-    const float2* a = A + (((size_t)s * cmax + c) * boxmax) * plane + l0;
+    const __amdgpu_buffer_rsrc_t r =
+        buffer_rsrc(A + (((size_t)s * cmax + c) * boxmax) * plane + l0, (unsigned)(bx * NL * sizeof(float2)));
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = lane + 64 * i;
-      const int x = e >> 3, ln = e & 7;
-      ra[i] = (x < bx && (full || l0 + ln < NL)) ? a[(size_t)x * plane + ln] : make_float2(0.f, 0.f);
-    }
+    for (int n1 = 0; n1 < V; ++n1) ra[n1] = buffer_load_f2(r, voff0 + n1 * kRow);
   };
   auto load_p = [&](int pc, float2* rp) {
-    const float2* pp = Phat + (size_t)pc * N * plane + l0;
+    const __amdgpu_buffer_rsrc_t r = buffer_rsrc(Phat + (size_t)pc * N * plane + l0, (unsigned)(N * NL * sizeof(float2)));
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = lane + 64 * i;
-      const int kx = e >> 3, ln = e & 7;
-      rp[i] = (full || l0 + ln < NL) ? pp[(size_t)kx * plane + ln] : make_float2(0.f, 0.f);
-    }
+    for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(r, voff0 + m * kRow);
   };
+  // Slot-derived values are wave-uniform; readfirstlane keeps them (and the
+  // descriptors built from them) in scalar registers.
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
   auto next_valid = [&](int s) {
-    while (s < s1 && !meta[s].valid) ++s;
+    while (s < s1 && !uni(meta[s].valid)) ++s;
     return s;
   };
   int s = next_valid(sg * group);
   if (s >= s1) return;
-  float2 ra[PER], rp[PER];
-  {
-    const LigandDesc& L = ligs[slots[s].lig];
-    load_a(s, 0, min(meta[s].box[0], boxmax), ra);
-    load_p(L.pmap[0], rp);
+  float2 ra[V], rp[V];
+  if (PREFETCH) {
+    load_a(s, 0, uni(min(meta[s].box[0], boxmax)), ra);
+    load_p(uni(ligs[slots[s].lig].pmap[0]), rp);
   }
   for (;;) {
-    const LigandDesc& L = ligs[slots[s].lig];
-    const int nch = L.nch;
-    const int bx = min(meta[s].box[0], boxmax);
-    const int wx = meta[s].win[0];
+    const LigandDesc& Ld = ligs[uni(slots[s].lig)];
+    const int nch = uni(Ld.nch);
+    const int bx = uni(min(meta[s].box[0], boxmax));
+    const int wx = uni(meta[s].win[0]);
     const int sn = next_valid(s + 1);
-    float2 acc[PER];
+    float2 acc[V];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) acc[i] = make_float2(0.f, 0.f);
+    for (int i = 0; i < V; ++i) acc[i] = make_float2(0.f, 0.f);
     for (int c = 0; c < nch; ++c) {
-      WaveSync::sync();
-      float2 pc_[PER];
+      float2 v[V], pc_[V];
+      if (PREFETCH) {
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        bufA[lane + 64 * i] = ra[i];
-        pc_[i] = rp[i];
+        for (int i = 0; i < V; ++i) {
+          v[i] = ra[i];
+          pc_[i] = rp[i];
+        }
+        if (c + 1 < nch) {
+          load_a(s, c + 1, bx, ra);
+          load_p(uni(Ld.pmap[c + 1]), rp);
+        } else if (sn < s1) {
+          load_a(sn, 0, uni(min(meta[sn].box[0], boxmax)), ra);
+          load_p(uni(ligs[uni(slots[sn].lig)].pmap[0]), rp);
+        }
+      } else {
+        load_a(s, c, bx, v);
+        load_p(uni(Ld.pmap[c]), pc_);
       }
-      if (c + 1 < nch) {
-        load_a(s, c + 1, bx, ra);
-        load_p(L.pmap[c + 1], rp);
-      } else if (sn < s1) {
-        load_a(sn, 0, min(meta[sn].box[0], boxmax), ra);
-        load_p(ligs[slots[sn].lig].pmap[0], rp);
-      }
-      WaveSync::sync();
-      const float2* r = wave_fft<P, -1, true, LINES>(bufA, bufB, LINES, LINES, 1, tw);
+      reg_fft<N, -1>(v, twl, lds, ln, n2);
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const float2 l = r[lane + 64 * i], p = pc_[i];
+      for (int i = 0; i < V; ++i) {
+        const float2 l = v[i], p = pc_[i];
         acc[i].x += l.x * p.x + l.y * p.y;
         acc[i].y += l.x * p.y - l.y * p.x;
       }
     }
-    WaveSync::sync();
 #pragma unroll
-    for (int i = 0; i < PER; ++i) bufA[lane + 64 * i] = make_float2(acc[i].x * invG, acc[i].y * invG);
-    WaveSync::sync();
-    const float2* r = wave_fft<P, 1, true, LINES>(bufA, bufB, LINES, LINES, 1, tw);
-    float2* tp = T + (size_t)s * N * plane + l0;
+    for (int i = 0; i < V; ++i) acc[i] = make_float2(acc[i].x * invG, acc[i].y * invG);
+    reg_fft<N, 1>(acc, twl, lds, ln, n2);
+    const __amdgpu_buffer_rsrc_t rt =
+        buffer_rsrc(T + (size_t)s * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = lane + 64 * i;
-      const int x = e >> 3, ln = e & 7;
-      if (x < wx && (full || l0 + ln < NL)) tp[(size_t)x * plane + ln] = r[e];
-    }
+    for (int m = 0; m < V; ++m) buffer_store_f2(acc[m], rt, voff0 + m * kRow);
     if (sn >= s1) break;
     s = sn;
   }
@@ -868,7 +877,7 @@ hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int n
                              int astride, const RotMeta* meta, int cmax, int boxmax, int max_chan_atoms, float2* A,
                              const float2* tw) {
   if (nslots <= 0) return hipSuccess;
-  const size_t dyn = (size_t)std::max(1, max_chan_atoms) * (sizeof(SpreadAtom) + sizeof(PlaneAtom));
+  const size_t dyn = (size_t)std::max(1, max_chan_atoms) * sizeof(SpreadAtom);
   CDK_DISPATCH_N(N, {
     hipLaunchKernelGGL(k_fwd_planes<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms, wv, wq,
                        ag, astride, meta, cmax, boxmax, A, tw);
@@ -877,8 +886,10 @@ hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int n
 }
 
 int mid_line_groups(int N) {
+  const int L = (N == 64 || N == 128) ? 8 : 4;
+  const int lpb = 4 * (64 / L);
   const int NL = N * (N / 2 + 1);
-  const int NLG = (NL + 31) / 32;
+  const int NLG = (NL + lpb - 1) / lpb;
   return ((NLG + 7) / 8) * 8;
 }
 
