@@ -15,6 +15,18 @@ launch sequence per batch of 50 rotations of one conformer, full-grid stages)
 on a conformer subset, per-kernel device time from HIP events on the launch
 stream (for the roofline), and the literal cpuNaive dense correlation timed on
 the host cores over a bounded sample of the same poses.
+
+The serving leg streams docking requests (15-80 atoms, 1-50 conformers,
+500/2000/5000 quaternions each, against resident 64^3 and 96^3 x 9 grids) into
+the continuous-batching server: first all at once to find its capacity, then
+as a Poisson stream at a fixed fraction of that capacity; it reports p50/p99
+submit-to-result latency and the served poses/s.
+
+--workload batched runs the uneven-batching workload instead (64 ligands of
+15-60 atoms, 10-100 conformers, 1000 quaternions, one hipGraph-captured launch
+set); --workload large the 128^3 x 9 grids at 0.375 A (50-atom ligand, 50
+conformers x 3000 quaternions, top-10 poses per conformer, 1% of the rotations
+re-scored in FP64 by the oracle).
 """
 import argparse
 import json
@@ -35,10 +47,12 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def fused_kernel_bytes(box, win, nch, npch, N, group):
+def fused_kernel_bytes(geoms, npch, N, group):
     """Algorithmic HBM bytes of each fused kernel over a set of poses.
 
-    plane = N*(N/2+1)*8 bytes (one complex half-spectrum plane).
+    geoms: [(box, win, nch)] per job, box/win [..., 3] per pose, nch the
+    job's channel count.  plane = N*(N/2+1)*8 bytes (one complex
+    half-spectrum plane).
       forward planes: writes nch*box_x planes per pose (the pruned y/z spectra)
       mid lines:      reads those planes, writes win_x planes (x-inverse done),
                       reads the protein spectra (npch*H*8) once per group of poses
@@ -46,14 +60,15 @@ def fused_kernel_bytes(box, win, nch, npch, N, group):
     """
     plane = N * (N // 2 + 1) * 8
     H8 = N * N * (N // 2 + 1) * 8
-    bx = box[..., 0].astype(np.float64).ravel()
-    wx = win[..., 0].astype(np.float64).ravel()
-    valid = (win.reshape(-1, 3) > 0).all(axis=1)
-    bx, wx = bx[valid], wx[valid]
-    n = bx.size
-    fwd = float((nch * bx * plane).sum())
-    mid = float((nch * bx * plane + wx * plane).sum()) + npch * H8 * n / group
-    inv = float((wx * plane + 8).sum())
+    fwd = mid = inv = 0.0
+    for box, win, nch in geoms:
+        bx = box[..., 0].astype(np.float64).ravel()
+        wx = win[..., 0].astype(np.float64).ravel()
+        valid = (win.reshape(-1, 3) > 0).all(axis=1)
+        bx, wx = bx[valid], wx[valid]
+        fwd += float((nch * bx * plane).sum())
+        mid += float((nch * bx * plane + wx * plane).sum()) + npch * H8 * bx.size / group
+        inv += float((wx * plane + 8).sum())
     return {"prep": 0.0, "fwd_planes": fwd, "mid_lines": mid, "inv_argmin": inv}
 
 
@@ -65,66 +80,100 @@ def reference_bytes(N, C):
     return 8 * C * G + 24 * C * H + 16 * H + 8 * G
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--conformers", type=int, default=100)
-    ap.add_argument("--quaternions", type=int, default=2000)
-    ap.add_argument("--plain-conformers", type=int, default=10)
-    ap.add_argument("--cpu-sample", type=int, default=12, help="rotations timed on the CPU (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
-    ap.add_argument("--group", type=int, default=16, help="poses per mid-kernel workgroup")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X and the search shards "
-              "by independent poses (see DESIGN.md); run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def percentile(a, q):
+    return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
-    import cdocker_amd as cdk
-    from cdocker_amd import workloads
 
-    t0 = time.time()
-    g = workloads.protein_grids("p64")
-    w = workloads.config2(nconf=args.conformers, nquat=args.quaternions)
-    L = w["ligands"][0]
-    log("inputs ready in %.1fs" % (time.time() - t0))
-    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group)
-    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
-    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
-    nu, used, _ = lig.channels()
-    C = nu + 1
-    N = g["n"]
-    npose = args.conformers * args.quaternions
+def serve_leg(ctx, cdk, workloads, nreq, load, max_batch, seed):
+    """Continuous-batching server over resident 64^3 and 96^3 grids.
 
-    batch = cdk.Batch(ctx, prot)
-    batch.add(lig, L["conformers"], L["quats"])
-    for i in range(args.warmup):
+    Pass 1 submits every request at once; the makespan gives the server's
+    capacity in requests/s.  Pass 2 submits the same requests as a Poisson
+    stream at `load` x that capacity; latency is submit-to-result as the
+    server measures it."""
+    prots = []
+    for name in ("p64", "p96"):
+        g = workloads.protein_grids(name)
+        prots.append(cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"]))
+    rs = np.random.default_rng(seed)
+    reqs = [workloads.serving_request(rs, seed + 2 * k) for k in range(nreq)]
+    poses = np.array([r["quats"].shape[0] * r["quats"].shape[1] for r in reqs], np.float64)
+    budget = (8 << 30) + (8 << 30)  # context workspace + resident request state
+
+    def run(arrivals):
+        srv = cdk.Server(ctx, prots, budget, max_batch)
+        t0 = time.perf_counter()
+        sub = []
+        ids = []
+        for r, ta in zip(reqs, arrivals):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            sub.append(time.perf_counter() - t0)
+            ids.append(srv.submit(r["protein_index"], r["charge"], r["epsilon"], r["radius"], r["conformers"],
+                                  r["quats"]))
+        lat = np.array([srv.result(i)[3] for i in ids])
+        st = srv.stats()
+        srv.close()
+        span = float(np.max(np.asarray(sub) + lat))
+        return lat, span, st
+
+    lat0, span0, st0 = run(np.zeros(nreq))
+    cap_req = nreq / span0
+    rate = load * cap_req
+    arrivals = np.cumsum(rs.exponential(1.0 / rate, nreq))
+    arrivals -= arrivals[0]
+    lat, span, st = run(arrivals)
+    for p in prots:
+        p.close()
+    out = {
+        "requests": nreq,
+        "poses": int(poses.sum()),
+        "capacity_poses_per_s": round(poses.sum() / span0, 1),
+        "capacity_req_s": round(cap_req, 3),
+        "offered_rate_req_s": round(rate, 3),
+        "load": load,
+        "p50_s": round(percentile(lat, 50), 4),
+        "p99_s": round(percentile(lat, 99), 4),
+        "mean_s": round(float(lat.mean()), 4),
+        "poses_per_s": round(poses.sum() / span, 1),
+        "saturated_p50_s": round(percentile(lat0, 50), 4),
+        "launch_sets": int(st["launch_sets"]),
+        "mean_launch_set_poses": round(st["mean_batch_poses"], 1),
+        "peak_admitted_bytes": int(st["peak_admitted_bytes"]),
+        "max_batch_poses": max_batch,
+        "proteins": "64^3 x 9 and 96^3 x 9 FP32 at 0.5 A, spectra resident",
+    }
+    return out
+
+
+def fused_steps(cdk, batch, steps, warmup, npose):
+    """W untimed and K timed passes of the whole fused search (hipGraph
+    replay + result copy-back), each bracketed by device synchronisation
+    inside Batch.run."""
+    for i in range(warmup):
         batch.run(cdk.ENGINE_FUSED)
         log("warmup %d done" % i)
     times = []
-    for i in range(args.steps):
+    for i in range(steps):
         ts = time.perf_counter()
         batch.run(cdk.ENGINE_FUSED)
         times.append(time.perf_counter() - ts)
         log("step %d: %.3f s (%.0f poses/s)" % (i, times[-1], npose / times[-1]))
-    total = float(sum(times))
-    value = npose * args.steps / total
-    E, idx, mc = batch.results(0)
-    box, win = batch.geometry(0)
-    n_valid = int(np.isfinite(E).sum())
+    return float(sum(times))
 
-    # Per-kernel device time (HIP events on the launch stream), eager launches.
+
+def kernel_leg(ctx, cdk, batch, geoms, npch, N):
+    """Per-kernel device time from HIP events on the launch stream (one eager
+    pass), algorithmic GB/s per kernel, and the roofline of the dominant one."""
     ctx.set_option("time_stages", 1)
     ctx.stage_times(reset=True)
     batch.run(cdk.ENGINE_FUSED)
     ms, launches = ctx.stage_times(reset=True)
     ctx.set_option("time_stages", 0)
+    group = ctx.get_option("last_mid_group")
     names = ["prep", "fwd_planes", "mid_lines", "inv_argmin"]
-    kb = fused_kernel_bytes(box, win, C, prot.num_grids, N, args.group)
+    kb = fused_kernel_bytes(geoms, npch, N, group)
     kernels = {}
     for i, nm in enumerate(names):
         if launches[i]:
@@ -133,14 +182,22 @@ def main():
                            "GBps": round(kb[nm] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None}
     dom = max(names, key=lambda k: kernels.get(k, {}).get("ms_total", 0))
     dom_avg_s = kernels[dom]["ms_total"] * 1e-3 / kernels[dom]["launches"]
-    dom_bytes_per_launch = kb[dom] / kernels[dom]["launches"]
-    achieved = dom_bytes_per_launch / dom_avg_s / 1e9
+    dom_bytes = kb[dom] / kernels[dom]["launches"]
+    achieved = dom_bytes / dom_avg_s / 1e9
     log("kernels", json.dumps(kernels))
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
+            "algorithmic_bytes_per_launch": round(dom_bytes)}
+    kernels["tuned"] = {"fwd_variant": ["block", "wavefront"][ctx.get_option("last_fwd_variant")],
+                        "mid_group": group}
+    return kernels, roof
 
-    # Reference dataflow, per conformer in batches of 50 (the >=5x denominator).
-    pc = max(1, min(args.plain_conformers, args.conformers))
+
+def reference_leg(ctx, cdk, prot, lig, confs, quats, fused_idx):
+    """The reference dataflow, one launch sequence per batch of 50 rotations
+    of one conformer (the plain per-ligand GPU loop)."""
     pb = cdk.Batch(ctx, prot)
-    pb.add(lig, L["conformers"][:pc], L["quats"][:pc])
+    pb.add(lig, confs, quats)
     pb.run(cdk.ENGINE_REFERENCE)
     ctx.set_option("time_stages", 1)
     ctx.stage_times(reset=True)
@@ -152,32 +209,45 @@ def main():
     ts = time.perf_counter()
     pb.run(cdk.ENGINE_REFERENCE)
     plain_s = min(plain_s, time.perf_counter() - ts)
-    plain_value = pc * args.quaternions / plain_s
+    n = confs.shape[0] * quats.shape[-2]
     Er, ir, _ = pb.results(0)
-    agree = float((ir == idx[:pc]).all(axis=-1).mean())
+    pb.close()
+    agree = float((ir == fused_idx).all(axis=-1).mean())
     ref_names = ["prep", "spread", "r2c", "conj_mult", "sum_grids", "c2r", "argmin"]
-    plain_kernels = {nm: round(float(rms[i]), 2) for i, nm in enumerate(ref_names) if rl[i]}
-    log("reference engine %.0f poses/s, stage ms %s" % (plain_value, plain_kernels))
+    stage = {nm: round(float(rms[i]), 2) for i, nm in enumerate(ref_names) if rl[i]}
+    value = n / plain_s
+    log("reference engine %.0f poses/s, stage ms %s" % (value, stage))
+    return {"value": round(value, 1), "unit": "poses/s", "poses": n, "stage_ms": stage,
+            "argmin_agreement": round(agree, 4)}
 
-    cpu = None
-    if args.cpu_sample > 0:
-        from oracle import oracle as orc
-        orc.set_threads(args.cpu_threads)
-        prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
-        ns = min(args.cpu_sample, args.quaternions)
-        ts = time.perf_counter()
-        for r in range(ns):
-            Lb, bx, wn, _ = prob.pose_box(L["conformers"][0], L["quats"][0, r])
-            if (wn > 0).all() and (bx <= N).all():
-                orc.correlate_window_full(prob.Pused, Lb, wn)
-        cpu_s = time.perf_counter() - ts
-        cpu = {"value": round(ns / cpu_s, 4), "unit": "poses/s", "cores": int(args.cpu_threads), "kind": "port",
-               "sample": "first %d rotations of conformer 0, literal cpuNaive dense FP32 loop (cpuNaive.cpp:194-216) "
-                         "over the ligand box and window, %d threads over x, %.1f s" % (ns, args.cpu_threads, cpu_s)}
-        log("cpu baseline", cpu)
 
+def cpu_leg(g, L, nsample, threads, N, budget_s=25.0):
+    """The literal cpuNaive dense FP32 correlation on the host cores over the
+    first rotations of conformer 0 (a bounded sample of the same poses: at
+    most nsample rotations, stopping once budget_s has been spent)."""
+    from oracle import oracle as orc
+    orc.set_threads(threads)
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    ts = time.perf_counter()
+    ns = 0
+    for r in range(min(nsample, L["quats"].shape[1])):
+        Lb, bx, wn, _ = prob.pose_box(L["conformers"][0], L["quats"][0, r])
+        if (wn > 0).all() and (bx <= N).all():
+            orc.correlate_window_full(prob.Pused, Lb, wn)
+        ns += 1
+        if time.perf_counter() - ts > budget_s:
+            break
+    cpu_s = time.perf_counter() - ts
+    cpu = {"value": round(ns / cpu_s, 4), "unit": "poses/s", "cores": int(threads), "kind": "port",
+           "sample": "first %d rotations of conformer 0, literal cpuNaive dense FP32 loop (cpuNaive.cpp:194-216) "
+                     "over the ligand box and window, %d threads over x, %.1f s" % (ns, threads, cpu_s)}
+    log("cpu baseline", cpu)
+    return cpu
+
+
+def line(args, metric, value, total, workload, extra_config, roof, kernels, cpu, **more):
     out = {
-        "metric": METRIC,
+        "metric": metric,
         "value": round(value, 1),
         "unit": "poses/s",
         "n_gpus": 1,
@@ -188,31 +258,215 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded pseudo-protein grids, random-walk ligand, torsion conformers, Shoemake quaternions)",
-        "config": {"workload": "single-ligand throughput: 64^3 x 9ch FP32 grids at 0.5 A, 40-atom ligand, "
-                               "%d conformers x %d quaternions = %d poses" % (args.conformers, args.quaternions, npose),
-                   "grid_edge": N, "channels_used": C, "poses": npose, "valid_poses": n_valid,
-                   "engine": "fused" + ("" if args.no_graph else "+hipGraph"),
-                   "translations_per_s": round(value * N ** 3, 1)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
-                     "avg_launch_ms": round(dom_avg_s * 1e3, 4),
-                     "algorithmic_bytes_per_launch": round(dom_bytes_per_launch)},
+        "data": "synthetic (seeded pseudo-protein grids, random-walk ligands, torsion conformers, Shoemake quaternions)",
+        "config": dict({"workload": workload}, **extra_config),
+        "roofline": roof,
         "kernels": kernels,
-        "pipeline_equiv": {"bytes_per_pose": reference_bytes(N, C),
-                           "GBps": round(value * reference_bytes(N, C) / 1e9, 1),
-                           "note": "poses/s x B(N,C) of the unfused reference dataflow"},
-        "reference_engine": {"value": round(plain_value, 1), "unit": "poses/s", "poses": pc * args.quaternions,
-                             "stage_ms": plain_kernels, "speedup_fused": round(value / plain_value, 2),
-                             "argmin_agreement": round(agree, 4)},
         "cpu_baseline": cpu,
     }
+    out.update(more)
     print(json.dumps(out), flush=True)
-    pb.close()
+
+
+def run_single(args, cdk, workloads):
+    t0 = time.time()
+    g = workloads.protein_grids("p64")
+    w = workloads.config2(nconf=args.conformers, nquat=args.quaternions)
+    L = w["ligands"][0]
+    log("inputs ready in %.1fs" % (time.time() - t0))
+    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd)
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    nu, used, _ = lig.channels()
+    C = nu + 1
+    N = g["n"]
+    npose = args.conformers * args.quaternions
+
+    batch = cdk.Batch(ctx, prot)
+    batch.add(lig, L["conformers"], L["quats"])
+    total = fused_steps(cdk, batch, args.steps, args.warmup, npose)
+    value = npose * args.steps / total
+    E, idx, mc = batch.results(0)
+    box, win = batch.geometry(0)
+    n_valid = int(np.isfinite(E).sum())
+    kernels, roof = kernel_leg(ctx, cdk, batch, [(box, win, C)], prot.num_grids, N)
+
+    pc = max(1, min(args.plain_conformers, args.conformers))
+    ref = reference_leg(ctx, cdk, prot, lig, L["conformers"][:pc], L["quats"][:pc], idx[:pc])
+    ref["speedup_fused"] = round(value / ref["value"], 2)
+    cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
+
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_leg(ctx, cdk, workloads, args.serve_requests, args.serve_load, args.serve_batch, 7100)
+        serving["speedup_vs_reference_engine"] = round(serving["poses_per_s"] / ref["value"], 2)
+        log("serving", json.dumps(serving))
+
+    line(args, METRIC, value, total,
+         "single-ligand throughput: 64^3 x 9ch FP32 grids at 0.5 A, 40-atom ligand, "
+         "%d conformers x %d quaternions = %d poses" % (args.conformers, args.quaternions, npose),
+         {"grid_edge": N, "channels_used": C, "poses": npose, "valid_poses": n_valid,
+          "engine": "fused" + ("" if args.no_graph else "+hipGraph"), "translations_per_s": round(value * N ** 3, 1)},
+         roof, kernels, cpu,
+         pipeline_equiv={"bytes_per_pose": reference_bytes(N, C), "GBps": round(value * reference_bytes(N, C) / 1e9, 1),
+                         "note": "poses/s x B(N,C) of the unfused reference dataflow"},
+         reference_engine=ref, serving=serving)
     batch.close()
     lig.close()
     prot.close()
     ctx.close()
+
+
+def run_batched(args, cdk, workloads):
+    """64 uneven ligands in one hipGraph-captured launch set sequence."""
+    t0 = time.time()
+    g = workloads.protein_grids("p64")
+    w = workloads.config3()
+    log("inputs ready in %.1fs" % (time.time() - t0))
+    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd)
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    N = g["n"]
+    ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
+    batch = cdk.Batch(ctx, prot)
+    for lig, L in zip(ligs, w["ligands"]):
+        batch.add(lig, L["conformers"], L["quats"])
+    npose = batch.total_poses()
+    total = fused_steps(cdk, batch, args.steps, args.warmup, npose)
+    value = npose * args.steps / total
+    geoms = []
+    n_valid = 0
+    for j, lig in enumerate(ligs):
+        box, win = batch.geometry(j)
+        geoms.append((box, win, lig.channels()[0] + 1))
+        n_valid += int(np.isfinite(batch.results(j)[0]).sum())
+    kernels, roof = kernel_leg(ctx, cdk, batch, geoms, prot.num_grids, N)
+
+    # The same ligands searched one at a time must give identical results.
+    same = True
+    for j in range(min(4, len(ligs))):
+        L = w["ligands"][j]
+        E1, i1, m1 = cdk.search(ctx, prot, ligs[j], L["conformers"], L["quats"])
+        E, i, m = batch.results(j)
+        same &= bool(np.array_equal(E.view(np.uint32), E1.view(np.uint32)) and np.array_equal(i, i1))
+    L0 = w["ligands"][0]
+    pc = max(1, min(args.plain_conformers, L0["conformers"].shape[0]))
+    ref = reference_leg(ctx, cdk, prot, ligs[0], L0["conformers"][:pc], L0["quats"][:pc], batch.results(0)[1][:pc])
+    ref["speedup_fused"] = round(value / ref["value"], 2)
+    cpu = cpu_leg(g, L0, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
+    atoms = [L["charge"].shape[0] for L in w["ligands"]]
+    confs = [L["conformers"].shape[0] for L in w["ligands"]]
+    line(args, "poses scored/sec at 64³×9ch FP32, 64 uneven ligands batched", value, total,
+         "uneven batching: 64^3 x 9ch FP32 at 0.5 A, 64 ligands (atoms %d-%d, conformers %d-%d) x 1000 "
+         "quaternions = %d poses in one hipGraph" % (min(atoms), max(atoms), min(confs), max(confs), npose),
+         {"grid_edge": N, "ligands": len(ligs), "poses": npose, "valid_poses": n_valid,
+          "engine": "fused" + ("" if args.no_graph else "+hipGraph")},
+         roof, kernels, cpu, reference_engine=ref, per_ligand_identical_first4=same)
+    batch.close()
+    for lig in ligs:
+        lig.close()
+    prot.close()
+    ctx.close()
+
+
+def run_large(args, cdk, workloads):
+    """128^3 x 9 at 0.375 A; top-10 poses per conformer; 1% FP64 re-score."""
+    from oracle import oracle as orc
+    t0 = time.time()
+    g = workloads.protein_grids("p128")
+    w = workloads.config5()
+    L = w["ligands"][0]
+    log("inputs ready in %.1fs" % (time.time() - t0))
+    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd)
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    C = lig.channels()[0] + 1
+    N = g["n"]
+    nconf, nquat = L["quats"].shape[:2]
+    npose = nconf * nquat
+    batch = cdk.Batch(ctx, prot)
+    batch.add(lig, L["conformers"], L["quats"])
+    total = fused_steps(cdk, batch, args.steps, args.warmup, npose)
+    value = npose * args.steps / total
+    E, idx, mc = batch.results(0)
+    box, win = batch.geometry(0)
+    kernels, roof = kernel_leg(ctx, cdk, batch, [(box, win, C)], prot.num_grids, N)
+
+    # Top-10 poses per conformer (sort_index + pose rebuild, NoMini:466-487).
+    ts = time.perf_counter()
+    top = []
+    for c in range(nconf):
+        rot = ctx.Rotate(L["quats"][c], L["conformers"][c])
+        rank, pose = cdk.rank_poses(prot, rot, E[c], idx[c], mc[c], 10)
+        top.append(rank)
+    top_s = time.perf_counter() - ts
+
+    # 1% of the rotations re-scored by the FP64 oracle at the GPU's translation,
+    # and a few searched in full (FP64 dense) for the argmin.
+    orc.set_threads(args.cpu_threads)
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    rs = np.random.default_rng(55)
+    flat = rs.choice(npose, max(1, npose // 100), replace=False)
+    max_rel = 0.0
+    checked = 0
+    for f in flat:
+        c, i = divmod(int(f), nquat)
+        if not np.isfinite(E[c, i]):
+            continue
+        rot = orc.rotate(L["quats"][c, i:i + 1], L["conformers"][c])[0]
+        e64 = prob.sparse_energy(rot, mc[c, i], idx[c, i])
+        max_rel = max(max_rel, abs(float(E[c, i]) - e64) / max(abs(e64), 1e-30))
+        checked += 1
+    full = {"poses": 0, "exact": 0}
+    for f in flat[:args.large_full_checks]:
+        c, i = divmod(int(f), nquat)
+        o = prob.search_pose(L["conformers"][c], L["quats"][c, i], double=True)
+        full["poses"] += 1
+        full["exact"] += int(o["valid"] and np.array_equal(o["index"], idx[c, i]))
+    log("fp64 check: %d poses, max rel err %.3g; full searches %s" % (checked, max_rel, full))
+    cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
+    line(args, "poses scored/sec at 128³×9ch FP32", value, total,
+         "large grid: 128^3 x 9ch FP32 at 0.375 A, 50-atom ligand, %d conformers x %d quaternions = %d poses, "
+         "top-10 per conformer" % (nconf, nquat, npose),
+         {"grid_edge": N, "channels_used": C, "poses": npose, "valid_poses": int(np.isfinite(E).sum()),
+          "engine": "fused" + ("" if args.no_graph else "+hipGraph")},
+         roof, kernels, cpu,
+         fp64_check={"sampled_rotations": checked, "max_rel_energy_err": float("%.3g" % max_rel),
+                     "full_search_argmin": full},
+         top10={"conformers": nconf, "host_s": round(top_s, 3)})
+    batch.close()
+    lig.close()
+    prot.close()
+    ctx.close()
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["single", "batched", "large"], default="single")
+    ap.add_argument("--conformers", type=int, default=100)
+    ap.add_argument("--quaternions", type=int, default=2000)
+    ap.add_argument("--plain-conformers", type=int, default=10)
+    ap.add_argument("--cpu-sample", type=int, default=12, help="rotations timed on the CPU (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
+    ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
+    ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 wavefront, -1 autotuned")
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-load", type=float, default=0.7, help="Poisson rate as a fraction of capacity")
+    ap.add_argument("--serve-batch", type=int, default=4096, help="poses per server launch set")
+    ap.add_argument("--large-full-checks", type=int, default=6, help="full FP64 searches in the large workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X and the search shards "
+              "by independent poses (see DESIGN.md); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import cdocker_amd as cdk
+    from cdocker_amd import workloads
+    {"single": run_single, "batched": run_batched, "large": run_large}[args.workload](args, cdk, workloads)
 
 
 if __name__ == "__main__":

@@ -64,6 +64,7 @@ _SIGS = {
     "cdk_context_create": [I, PP],
     "cdk_context_destroy": [P],
     "cdk_context_set_option": [P, CP, I64],
+    "cdk_context_get_option": [P, CP, I64P],
     "cdk_context_stage_times": [P, DP, IP, I],
     "cdk_protein_create": [P, I, I, I, I, FP, FP, F, FP, FP, PP],
     "cdk_protein_destroy": [P],

@@ -49,6 +49,11 @@ class Context:
     def set_option(self, key, value):
         check(lib.cdk_context_set_option(self.h, key.encode(), int(value)), "set_option " + key)
 
+    def get_option(self, key):
+        v = C.c_int64()
+        check(lib.cdk_context_get_option(self.h, key.encode(), C.byref(v)), "get_option " + key)
+        return v.value
+
     def stage_times(self, reset=False):
         ms = np.zeros(8, np.float64)
         n = np.zeros(8, np.int32)

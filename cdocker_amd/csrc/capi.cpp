@@ -153,8 +153,11 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
       require(v >= 0 && v <= (1 << 24), "fused_chunk out of range");
       c.fused_chunk = (int)v;
     } else if (k == "mid_group") {
-      require(v > 0 && v <= 4096, "mid_group out of range");
+      require(v >= 0 && v <= 4096, "mid_group out of range");
       c.mid_group = (int)v;
+    } else if (k == "fwd_variant") {
+      require(v >= -1 && v <= 1, "fwd_variant must be -1 (tuned), 0 (block) or 1 (wavefront)");
+      c.fwd_variant = (int)v;
     } else if (k == "use_graph") {
       c.use_graph = v != 0;
     } else if (k == "time_stages") {
@@ -162,6 +165,26 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
     } else {
       throw cdk::Error(CDK_ERR_INVALID, "unknown option " + k);
     }
+  });
+}
+
+int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value) {
+  return guard([&] {
+    require(ctx && key && value, "NULL argument");
+    cdk::Context& c = *ctx->c;
+    std::lock_guard<std::mutex> lk(c.mu);
+    const std::string k(key);
+    if (k == "ws_budget_bytes") *value = (int64_t)c.ws_budget;
+    else if (k == "ref_batch") *value = c.ref_batch;
+    else if (k == "fused_chunk") *value = c.fused_chunk;
+    else if (k == "mid_group") *value = c.mid_group;
+    else if (k == "fwd_variant") *value = c.fwd_variant;
+    else if (k == "use_graph") *value = c.use_graph;
+    else if (k == "time_stages") *value = c.time_stages;
+    else if (k == "last_fwd_variant") *value = c.last_fwd;
+    else if (k == "last_mid_group") *value = c.last_group;
+    else if (k == "max_dyn_lds") *value = (int64_t)c.max_dyn_lds;
+    else throw cdk::Error(CDK_ERR_INVALID, "unknown option " + k);
   });
 }
 

@@ -40,6 +40,10 @@ Context::Context(int dev) : device(dev) {
   if (dev < 0 || dev >= n) throw Error(kErrInvalid, "device index out of range");
   hip_check(hipSetDevice(dev), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  int lds = 0;
+  hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+  max_dyn_lds = (size_t)lds;
+  hip_check(configure_fwd_wave(max_dyn_lds), "configure forward kernels");
 }
 
 Context::~Context() {
@@ -411,8 +415,108 @@ void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, siz
   ensure_work(ctx, chunk * N * plane * sizeof(float2), ctx.d_T);
 }
 
+struct Tuned {
+  int fwd = kFwdBlock, group = 16;
+};
+
+// Autotuner: picks the forward plane form (once per grid size, among the
+// forms that fit every batch of that size) and the mid-stage group size (per
+// launch shape) by timing them on the first launch set of this search.
+Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
+  const int N = prot.N;
+  Tuned tu;
+  const bool wave_ok = fwd_wave_supported(N);
+  const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)chunk};
+  bool need_fwd = false, need_group = false;
+  if (ctx.fwd_variant >= 0) {
+    tu.fwd = (ctx.fwd_variant == kFwdWave && wave_ok) ? kFwdWave : kFwdBlock;
+  } else {
+    auto it = ctx.tuned_fwd.find(N);
+    if (it != ctx.tuned_fwd.end())
+      tu.fwd = it->second;
+    else if (wave_ok)
+      need_fwd = true;
+    else
+      ctx.tuned_fwd[N] = tu.fwd = kFwdBlock;
+  }
+  if (ctx.mid_group > 0) {
+    tu.group = ctx.mid_group;
+  } else {
+    auto it = ctx.tuned_group.find(gkey);
+    if (it != ctx.tuned_group.end())
+      tu.group = it->second;
+    else
+      need_group = true;
+  }
+  if (!need_fwd && !need_group) return tu;
+
+  const int n = (int)std::min(chunk, t.slots.size());
+  const float2* tw = ctx.twiddle(N);
+  const PoseSlot* sl = t.d_slots.as<PoseSlot>();
+  const LigandDesc* d_ligs = t.d_ligs.as<LigandDesc>();
+  RotMeta* d_meta = t.d_meta.as<RotMeta>();
+  hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, t.boxmax,
+                        ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta, t.d_keys.as<unsigned long long>()),
+            "tune prep");
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  auto timed = [&](auto&& launch) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      hip_check(hipEventRecord(e0, ctx.stream), "hipEventRecord");
+      launch();
+      hip_check(hipEventRecord(e1, ctx.stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  auto fwd = [&](int variant) {
+    hip_check(launch_fwd_planes(ctx.stream, N, variant, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+                                t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, t.cmax, t.boxmax,
+                                t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
+              "tune fwd planes");
+  };
+  try {
+    if (need_fwd) {
+      const float tb = timed([&] { fwd(kFwdBlock); });
+      const float twv = timed([&] { fwd(kFwdWave); });
+      tu.fwd = twv < tb ? kFwdWave : kFwdBlock;
+      ctx.tuned_fwd[N] = tu.fwd;
+    } else {
+      fwd(tu.fwd);
+    }
+    if (need_group) {
+      float best = 1e30f;
+      for (int g : {4, 8, 16, 32}) {
+        const float ms = timed([&] {
+          hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
+                               prot.spec.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)prot.G(), tw),
+                    "tune mid");
+        });
+        if (ms < best) {
+          best = ms;
+          tu.group = g;
+        }
+      }
+      ctx.tuned_group[gkey] = tu.group;
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return tu;
+}
+
 // Fused pipeline: four launches per chunk of poses, any mix of ligands.
-void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk, EventTimer& timer) {
+void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk, const Tuned& tu,
+                   EventTimer& timer) {
   const int N = prot.N;
   const size_t G = prot.G();
   const float2* tw = ctx.twiddle(N);
@@ -421,7 +525,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
   unsigned long long* d_keys = t.d_keys.as<unsigned long long>();
   RotMeta* d_meta = t.d_meta.as<RotMeta>();
   const size_t total = t.slots.size();
-  const int group = std::max(1, ctx.mid_group);
+  const int group = std::max(1, tu.group);
   for (size_t s = 0; s < total; s += chunk) {
     const int n = (int)std::min(chunk, total - s);
     const PoseSlot* sl = d_slots + s;
@@ -432,7 +536,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
               "prep");
     timer.end(e);
     e = timer.begin(1);
-    hip_check(launch_fwd_planes(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+    hip_check(launch_fwd_planes(ctx.stream, N, tu.fwd, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
                                 t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, t.cmax, t.boxmax,
                                 t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
               "fwd planes");
@@ -503,14 +607,17 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
     const size_t chunk = fused_chunk(ctx, prot, t);
     ensure_fused_work(ctx, prot, t, chunk);
     (void)ctx.twiddle(prot.N);
+    const Tuned tu = tune_fused(ctx, prot, t, chunk);
+    ctx.last_fwd = tu.fwd;
+    ctx.last_group = tu.group;
     if (use_graph && !ctx.time_stages) {
-      if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk ||
-          t.graph_group != ctx.mid_group) {
+      if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.group ||
+          t.graph_fwd != tu.fwd) {
         t.drop_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
         try {
-          enqueue_fused(ctx, prot, t, chunk, timer);
+          enqueue_fused(ctx, prot, t, chunk, tu, timer);
         } catch (...) {
           (void)hipStreamEndCapture(ctx.stream, &g);
           if (g) (void)hipGraphDestroy(g);
@@ -522,11 +629,12 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
         hip_check(ie, "hipGraphInstantiate");
         t.graph_gen = ctx.ws_generation;
         t.graph_chunk = chunk;
-        t.graph_group = ctx.mid_group;
+        t.graph_group = tu.group;
+        t.graph_fwd = tu.fwd;
       }
       hip_check(hipGraphLaunch(t.graph, ctx.stream), "hipGraphLaunch");
     } else {
-      enqueue_fused(ctx, prot, t, chunk, timer);
+      enqueue_fused(ctx, prot, t, chunk, tu, timer);
     }
   }
   ctx.ensure_host(total);

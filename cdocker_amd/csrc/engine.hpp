@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -136,7 +137,7 @@ struct Prepared {
   hipGraphExec_t graph = nullptr;
   uint64_t graph_gen = 0;
   size_t graph_chunk = 0;
-  int graph_group = 0;
+  int graph_group = 0, graph_fwd = -1;
   std::vector<PoseResults> results;
   Prepared() = default;
   Prepared(const Prepared&) = delete;
@@ -154,9 +155,18 @@ struct Context {
   size_t ws_budget = 8ull << 30;  // bytes of per-batch working arrays
   int ref_batch = 50;             // poses per reference-dataflow batch (NoMini:208)
   int fused_chunk = 0;            // 0 = from budget
-  int mid_group = 16;             // poses sharing one LDS copy of a protein tile
+  int mid_group = 0;              // poses per mid-stage workgroup; 0 = tuned
+  int fwd_variant = -1;           // forward plane stage form; -1 = tuned
   bool use_graph = true;
   bool time_stages = false;
+  // Autotuner state.  The forward form changes FFT rounding, so it is chosen
+  // once per grid size and kept for the context's life (results then do not
+  // depend on how poses are batched); the mid group size only changes the
+  // schedule and is tuned per launch shape.
+  size_t max_dyn_lds = 64 * 1024;
+  std::map<int, int> tuned_fwd;                      // N -> variant
+  std::map<std::array<int, 4>, int> tuned_group;     // (N, cmax, boxmax, chunk) -> group
+  int last_fwd = -1, last_group = 0;
   StageTimes stage_times;
   // per-launch-set working arrays; ws_generation counts reallocations so a
   // captured graph that baked in the old addresses is rebuilt

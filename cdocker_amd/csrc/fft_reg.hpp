@@ -80,6 +80,63 @@ template <int S> struct RegDFT<24, S> {
   static __device__ __forceinline__ void run(float2* v) { compose_dft<3, 8, S>(v); }
 };
 
+// DFT of R points whose upper R/2 inputs are zero (the lower half alone is
+// read).  The first radix-2 split of such a transform is a copy, which saves
+// a quarter of the adds of DFT_8 and DFT_16.  Lengths without a split here
+// clear the upper half and run the full transform.
+template <int R, int S> struct RegDFTHalf {
+  static __device__ __forceinline__ void run(float2* v) {
+#pragma unroll
+    for (int i = R / 2; i < R; ++i) v[i] = make_float2(0.f, 0.f);
+    RegDFT<R, S>::run(v);
+  }
+};
+
+template <int S> struct RegDFTHalf<8, S> {
+  static __device__ __forceinline__ void run(float2* v) {
+    const float kR = 0.70710678118654752440f;
+    // DFT_4 of (a, b, 0, 0) = (a + b, a + S i b, a - b, a - S i b)
+    const float2 ib0 = c_muli<S>(v[2]), ib1 = c_muli<S>(v[3]);
+    const float2 e0 = c_add(v[0], v[2]), e1 = c_add(v[0], ib0), e2 = c_sub(v[0], v[2]), e3 = c_sub(v[0], ib0);
+    const float2 o0 = c_add(v[1], v[3]), oa = c_add(v[1], ib1), o2r = c_sub(v[1], v[3]), ob = c_sub(v[1], ib1);
+    const float2 o1 = c_scale(c_add(oa, c_muli<S>(oa)), kR);
+    const float2 o2 = c_muli<S>(o2r);
+    const float2 o3 = c_scale(c_sub(c_muli<S>(ob), ob), kR);
+    v[0] = c_add(e0, o0);
+    v[4] = c_sub(e0, o0);
+    v[1] = c_add(e1, o1);
+    v[5] = c_sub(e1, o1);
+    v[2] = c_add(e2, o2);
+    v[6] = c_sub(e2, o2);
+    v[3] = c_add(e3, o3);
+    v[7] = c_sub(e3, o3);
+  }
+};
+
+template <int S> struct RegDFTHalf<16, S> {
+  static __device__ __forceinline__ void run(float2* v) {
+    const float C[8] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                        0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f};
+    const float Sn[8] = {0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
+                         1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f};
+    float2 e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    RegDFTHalf<8, S>::run(e);
+    RegDFTHalf<8, S>::run(o);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float2 w = make_float2(C[k], S > 0 ? Sn[k] : -Sn[k]);
+      const float2 t = (k == 0) ? o[0] : c_mul(o[k], w);
+      v[k] = c_add(e[k], t);
+      v[k + 8] = c_sub(e[k], t);
+    }
+  }
+};
+
 // Lanes per line for each transform length.
 template <int N> struct RegPlan;
 template <> struct RegPlan<32> { static constexpr int L = 4; };
@@ -105,10 +162,14 @@ __device__ __forceinline__ void reg_twiddles(const float2* __restrict__ tw, int 
 
 // One transform of the wavefront's lines.  `lds` is the wavefront's slice of
 // V*64 float2.  ln = line within the wavefront, n2 = lane within the line.
-template <int N, int S>
+// HALF: inputs x[n] with n >= N/2 are zero (registers V/2.. are not read).
+template <int N, int S, bool HALF = false>
 __device__ __forceinline__ void reg_fft(float2* v, const float2* twl, float2* lds, int ln, int n2) {
   constexpr int L = RegShape<N>::L, V = RegShape<N>::V, LPW = RegShape<N>::LPW;
-  RegDFT<V, S>::run(v);
+  if constexpr (HALF)
+    RegDFTHalf<V, S>::run(v);
+  else
+    RegDFT<V, S>::run(v);
 #pragma unroll
   for (int k1 = 1; k1 < V; ++k1) {
     float2 w = twl[k1];

@@ -559,6 +559,178 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
   }
 }
 
+// Stage A, wavefront form: one workgroup per (pose, ligand channel) stages
+// the channel's atoms once; then its four wavefronts take the occupied
+// x-planes in turn and run with no block barriers.  Per plane a wavefront
+//   1. zeroes the packed rows (row pair p = rows 2p, 2p+1 in .x, .y) and
+//      spreads the plane's cells into them in the reference's atom order,
+//   2. z-transforms the row pairs (register four-step FFT, 64/L lines at a
+//      time, first stage pruned when the box fits in N/2) and writes each
+//      packed spectrum back over its row,
+//   3. y-transforms the half-spectrum columns, unpacking the two real rows
+//      of every pair as it loads them; columns 0 and N/2 are real along y
+//      and share one complex line,
+//   4. writes the plane spectrum A[slot][c][x][Y][ZH].
+// Rows have a padded stride (RSTR = N + pad, RSTR % 32 == L in float2
+// units) so that row loads, spectrum write-back and the column loads each
+// touch every 8-byte LDS bank slot evenly.
+template <int N>
+struct FwdWaveShape {
+  static constexpr int L = RegShape<N>::L, V = RegShape<N>::V, LPW = RegShape<N>::LPW;
+  static constexpr int RSTR = N + ((L - N % 32) + 32) % 32;
+};
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __restrict__ slots,
+                                                            const LigandDesc* __restrict__ ligs,
+                                                            const int* __restrict__ chan_atoms,
+                                                            const float* __restrict__ wv,
+                                                            const float* __restrict__ wq,
+                                                            const AtomGrid* __restrict__ ag, int astride,
+                                                            const RotMeta* __restrict__ meta, int cmax, int boxmax,
+                                                            int natom_lds, int npmax, float2* __restrict__ A,
+                                                            const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  using FS = FwdWaveShape<N>;
+  constexpr int L = FS::L, V = FS::V, LPW = FS::LPW, RSTR = FS::RSTR;
+  extern __shared__ double dyn[];
+  const int c = blockIdx.x, s = blockIdx.y;
+  const RotMeta m = meta[s];
+  if (!m.valid) return;
+  const PoseSlot ps = slots[s];
+  const LigandDesc& Ld = ligs[ps.lig];
+  if (c >= Ld.nch) return;
+  const int nc = Ld.ch_start[c + 1] - Ld.ch_start[c];
+  SpreadAtom* sa = reinterpret_cast<SpreadAtom*>(dyn);
+  {
+    const int* ca = chan_atoms + Ld.list_off + Ld.ch_start[c];
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+      const int a = ca[i];
+      const AtomGrid g = ag[(size_t)s * astride + a];
+      SpreadAtom t;
+      t.ix = g.ix;
+      t.iy = g.iy;
+      t.iz = g.iz;
+      t.w = channel_weight(Ld, c, a, wv, wq);
+      t.fx = g.fx;
+      t.fy = g.fy;
+      t.fz = g.fz;
+      sa[i] = t;
+    }
+  }
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int ln = lane % LPW, n2 = lane / LPW;
+  float2* tr = reinterpret_cast<float2*>(sa + natom_lds) + wave * (V * 64);
+  float2* rows = reinterpret_cast<float2*>(sa + natom_lds) + 4 * (V * 64) + wave * (npmax * RSTR);
+  float2 twl[V];
+  reg_twiddles<N>(twg, n2, twl);
+  const int bxm = min(m.box[0], boxmax), by = m.box[1], bz = m.box[2];
+  const int npair = (by + 1) >> 1;
+  const bool zhalf = bz <= N / 2, yhalf = by <= N / 2;
+  const int zlim = zhalf ? N / 2 : N;
+  float2* op0 = A + (((size_t)s * cmax + c) * boxmax) * (size_t)(N * ZH);
+  const int dy = (lane >> 1) & 1, dz = lane & 1;
+  for (int x = wave; x < bxm; x += 4) {
+    // 1. zero, then the ordered scatter: atoms touching plane x in ascending
+    // order, the 4 corners of one atom in lanes 0-3 (distinct cells), so each
+    // cell accumulates in the reference's order (FillLigandGrid.cpp:45-66,
+    // 97-117).
+    for (int p = 0; p < npair; ++p)
+      for (int z = lane; z < zlim; z += 64) rows[p * RSTR + z] = make_float2(0.f, 0.f);
+    WaveSync::sync();
+    float* cells = reinterpret_cast<float*>(rows);
+    for (int i = 0; i < nc; ++i) {
+      const int ddx = x - sa[i].ix;
+      if (ddx != 0 && ddx != 1) continue;  // wave-uniform
+      if (lane < 4) {
+        const int y = sa[i].iy + dy, z = sa[i].iz + dz;
+        float* cell = cells + (((y >> 1) * RSTR + z) << 1) + (y & 1);
+        *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, ddx), sa[i].fy, sa[i].fz, dy, dz);
+      }
+      WaveSync::sync();
+    }
+    // 2. z transforms of the row pairs, written back in place.
+    for (int p0 = 0; p0 < npair; p0 += LPW) {
+      const int p = p0 + ln;
+      const bool ok = p < npair;
+      float2 v[V];
+      if (zhalf) {
+#pragma unroll
+        for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = ok ? rows[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
+        reg_fft<N, -1, true>(v, twl, tr, ln, n2);
+      } else {
+#pragma unroll
+        for (int n1 = 0; n1 < V; ++n1) v[n1] = ok ? rows[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
+        reg_fft<N, -1, false>(v, twl, tr, ln, n2);
+      }
+      WaveSync::sync();
+      if (ok) {
+#pragma unroll
+        for (int q = 0; q < V; ++q) rows[p * RSTR + n2 + L * q] = v[q];
+      }
+    }
+    WaveSync::sync();
+    // 3. y transforms of the columns; line j = column j, except line 0 =
+    // columns 0 (real part) and N/2 (imaginary part).
+    float2* op = op0 + (size_t)x * (N * ZH);
+    for (int j0 = 0; j0 < N / 2; j0 += LPW) {
+      const int j = j0 + ln;
+      const bool jok = j < N / 2;
+      const int ia = j == 0 ? 0 : j, ib = j == 0 ? N / 2 : N - j;
+      float2 v[V];
+      auto col = [&](int y) {
+        const int p = y >> 1;
+        float2 r = make_float2(0.f, 0.f);
+        if (p < npair && jok) {
+          const float2 zk = rows[p * RSTR + ia], zm = rows[p * RSTR + ib];
+          if (j == 0)
+            r = (y & 1) ? make_float2(zk.y, zm.y) : make_float2(zk.x, zm.x);
+          else
+            r = (y & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
+                        : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        }
+        return r;
+      };
+      if (yhalf) {
+#pragma unroll
+        for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = col(n2 + L * n1);
+        reg_fft<N, -1, true>(v, twl, tr, ln, n2);
+      } else {
+#pragma unroll
+        for (int n1 = 0; n1 < V; ++n1) v[n1] = col(n2 + L * n1);
+        reg_fft<N, -1, false>(v, twl, tr, ln, n2);
+      }
+      // v[q] = Y[ky = n2 + L q] of line j.
+      if (j0 == 0) {
+        // Split line 0: C0(ky) = (U(ky) + conj U(-ky)) / 2,
+        //               CN(ky) = (U(ky) - conj U(-ky)) / 2i.
+        WaveSync::sync();
+        if (j == 0) {
+#pragma unroll
+          for (int q = 0; q < V; ++q) tr[n2 + L * q] = v[q];
+        }
+        WaveSync::sync();
+        if (j == 0) {
+#pragma unroll
+          for (int q = 0; q < V; ++q) {
+            const int ky = n2 + L * q;
+            const float2 u = v[q], w = tr[ky == 0 ? 0 : N - ky];
+            op[ky * ZH + 0] = make_float2(0.5f * (u.x + w.x), 0.5f * (u.y - w.y));
+            op[ky * ZH + N / 2] = make_float2(0.5f * (u.y + w.y), -0.5f * (u.x - w.x));
+          }
+        }
+      }
+      if (j > 0 && jok) {
+#pragma unroll
+        for (int q = 0; q < V; ++q) op[(n2 + L * q) * ZH + j] = v[q];
+      }
+    }
+    WaveSync::sync();
+  }
+}
+
 // Stage B: x FFT of every used channel, conj-multiply with the protein
 // spectra, channel sum, 1/G, inverse x FFT, for the (ky, kz) lines of the
 // spectrum.  Each wavefront owns 64/L consecutive lines (flattened
@@ -566,10 +738,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
 // one swizzled LDS transpose per transform).  A line's data arrives from
 // global memory already in the register layout the transform wants, and the
 // forward output layout is the inverse's input layout.  A workgroup is 4
-// wavefronts and walks a group of poses; the loads of the next (pose,
-// channel) pair are issued before the current transform.  Workgroup w runs
-// on XCD w % 8 and the line groups are split so each XCD touches one eighth
-// of the protein spectra, which then stays in that XCD's L2.  Reads
+// wavefronts and walks a group of poses as one sequence of (pose, channel)
+// items; for lines of at most 8 values per lane two register buffers
+// alternate, so the loads of item i+1 are in flight while item i is
+// transformed, with no register copies.  When the
+// pose's box is at most N/2 the upper half of every line is zero: those rows
+// are not loaded and the first DFT stage is pruned.  Workgroup w runs on
+// XCD w % 8 and the line groups are split so each XCD touches one eighth of
+// the protein spectra, which then stays in that XCD's L2.  Reads
 // A[slot][c][x][line] for x < box, writes T[slot][x'][line] for x' < window.
 #ifdef CDK_MID_WAVES
 #define CDK_MID_ATTR __attribute__((amdgpu_waves_per_eu(CDK_MID_WAVES)))
@@ -589,7 +765,6 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   constexpr int LPB = 4 * LPW;           // lines per workgroup
   constexpr int NLG = (NL + LPB - 1) / LPB;
   constexpr int PERX = (NLG + 7) / 8;    // line groups per XCD
-  constexpr bool PREFETCH = V <= 16;
   __shared__ float2 sbuf[4][V * 64];
   const int w = blockIdx.x;
   const int lg = (w & 7) * PERX + ((w >> 3) % PERX);
@@ -613,17 +788,6 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   // zero / are dropped.
   const int voff0 = lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone;
   constexpr int kRow = L * NL * (int)sizeof(float2);
-  auto load_a = [&](int s, int c, int bx, float2* ra) {
-    const __amdgpu_buffer_rsrc_t r =
-        buffer_rsrc(A + (((size_t)s * cmax + c) * boxmax) * plane + l0, (unsigned)(bx * NL * sizeof(float2)));
-#pragma unroll
-    for (int n1 = 0; n1 < V; ++n1) ra[n1] = buffer_load_f2(r, voff0 + n1 * kRow);
-  };
-  auto load_p = [&](int pc, float2* rp) {
-    const __amdgpu_buffer_rsrc_t r = buffer_rsrc(Phat + (size_t)pc * N * plane + l0, (unsigned)(N * NL * sizeof(float2)));
-#pragma unroll
-    for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(r, voff0 + m * kRow);
-  };
   // Slot-derived values are wave-uniform; readfirstlane keeps them (and the
   // descriptors built from them) in scalar registers.
   auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
@@ -631,58 +795,101 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
     while (s < s1 && !uni(meta[s].valid)) ++s;
     return s;
   };
-  int s = next_valid(sg * group);
-  if (s >= s1) return;
-  float2 ra[V], rp[V];
-  if (PREFETCH) {
-    load_a(s, 0, uni(min(meta[s].box[0], boxmax)), ra);
-    load_p(uni(ligs[slots[s].lig].pmap[0]), rp);
-  }
-  for (;;) {
-    const LigandDesc& Ld = ligs[uni(slots[s].lig)];
-    const int nch = uni(Ld.nch);
-    const int bx = uni(min(meta[s].box[0], boxmax));
-    const int wx = uni(meta[s].win[0]);
-    const int sn = next_valid(s + 1);
-    float2 acc[V];
+  // One (pose, channel) item of the sequence.
+  struct Item {
+    int s, c, nch, bx, lig;
+  };
+  auto enter = [&](Item& it, int s) {
+    it.s = s;
+    it.c = 0;
+    if (s < s1) {
+      it.lig = uni(slots[s].lig);
+      it.nch = uni(ligs[it.lig].nch);
+      it.bx = uni(min(meta[s].box[0], boxmax));
+    }
+  };
+  auto advance = [&](Item& it) {
+    if (++it.c >= it.nch) enter(it, next_valid(it.s + 1));
+  };
+  auto load = [&](const Item& it, float2* ra, float2* rp) {
+    const __amdgpu_buffer_rsrc_t r = buffer_rsrc(A + (((size_t)it.s * cmax + it.c) * boxmax) * plane + l0,
+                                                 (unsigned)(it.bx * NL * sizeof(float2)));
+    const int pc = uni(ligs[it.lig].pmap[it.c]);
+    const __amdgpu_buffer_rsrc_t q =
+        buffer_rsrc(Phat + (size_t)pc * N * plane + l0, (unsigned)(N * NL * sizeof(float2)));
+    if (it.bx <= N / 2) {
 #pragma unroll
-    for (int i = 0; i < V; ++i) acc[i] = make_float2(0.f, 0.f);
-    for (int c = 0; c < nch; ++c) {
-      float2 v[V], pc_[V];
-      if (PREFETCH) {
+      for (int n1 = 0; n1 < V / 2; ++n1) ra[n1] = buffer_load_f2(r, voff0 + n1 * kRow);
+    } else {
 #pragma unroll
-        for (int i = 0; i < V; ++i) {
-          v[i] = ra[i];
-          pc_[i] = rp[i];
-        }
-        if (c + 1 < nch) {
-          load_a(s, c + 1, bx, ra);
-          load_p(uni(Ld.pmap[c + 1]), rp);
-        } else if (sn < s1) {
-          load_a(sn, 0, uni(min(meta[sn].box[0], boxmax)), ra);
-          load_p(uni(ligs[uni(slots[sn].lig)].pmap[0]), rp);
-        }
-      } else {
-        load_a(s, c, bx, v);
-        load_p(uni(Ld.pmap[c]), pc_);
-      }
-      reg_fft<N, -1>(v, twl, lds, ln, n2);
-#pragma unroll
-      for (int i = 0; i < V; ++i) {
-        const float2 l = v[i], p = pc_[i];
-        acc[i].x += l.x * p.x + l.y * p.y;
-        acc[i].y += l.x * p.y - l.y * p.x;
-      }
+      for (int n1 = 0; n1 < V; ++n1) ra[n1] = buffer_load_f2(r, voff0 + n1 * kRow);
     }
 #pragma unroll
-    for (int i = 0; i < V; ++i) acc[i] = make_float2(acc[i].x * invG, acc[i].y * invG);
-    reg_fft<N, 1>(acc, twl, lds, ln, n2);
-    const __amdgpu_buffer_rsrc_t rt =
-        buffer_rsrc(T + (size_t)s * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
+    for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(q, voff0 + m * kRow);
+  };
+  float2 acc[V];
 #pragma unroll
-    for (int m = 0; m < V; ++m) buffer_store_f2(acc[m], rt, voff0 + m * kRow);
-    if (sn >= s1) break;
-    s = sn;
+  for (int i = 0; i < V; ++i) acc[i] = make_float2(0.f, 0.f);
+  // Transform item `it` held in (ra, rp) and accumulate conj(L) P; after the
+  // pose's last channel run the inverse and store the window rows.
+  auto consume = [&](const Item& it, float2* ra, const float2* rp) {
+    if (it.bx <= N / 2)
+      reg_fft<N, -1, true>(ra, twl, lds, ln, n2);
+    else
+      reg_fft<N, -1, false>(ra, twl, lds, ln, n2);
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const float2 l = ra[i], p = rp[i];
+      acc[i].x = fmaf(l.x, p.x, acc[i].x);
+      acc[i].x = fmaf(l.y, p.y, acc[i].x);
+      acc[i].y = fmaf(l.x, p.y, acc[i].y);
+      acc[i].y = fmaf(-l.y, p.x, acc[i].y);
+    }
+    if (it.c + 1 == it.nch) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) acc[i] = make_float2(acc[i].x * invG, acc[i].y * invG);
+      reg_fft<N, 1>(acc, twl, lds, ln, n2);
+      const int wx = uni(meta[it.s].win[0]);
+      const __amdgpu_buffer_rsrc_t rt =
+          buffer_rsrc(T + (size_t)it.s * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
+#pragma unroll
+      for (int m = 0; m < V; ++m) buffer_store_f2(acc[m], rt, voff0 + m * kRow);
+#pragma unroll
+      for (int i = 0; i < V; ++i) acc[i] = make_float2(0.f, 0.f);
+    }
+  };
+  Item i0;
+  enter(i0, next_valid(sg * group));
+  if (i0.s >= s1) return;
+  if constexpr (V <= 8) {
+    Item i1 = i0;
+    float2 a0[V], p0[V], a1[V], p1[V];
+    load(i0, a0, p0);
+    advance(i1);
+    if (i1.s < s1) load(i1, a1, p1);
+    for (;;) {
+      consume(i0, a0, p0);
+      if (i1.s >= s1) break;
+      Item i2 = i1;
+      advance(i2);
+      if (i2.s < s1) load(i2, a0, p0);
+      consume(i1, a1, p1);
+      if (i2.s >= s1) break;
+      Item i3 = i2;
+      advance(i3);
+      if (i3.s < s1) load(i3, a1, p1);
+      i0 = i2;
+      i1 = i3;
+    }
+  } else {
+    // Long lines: one buffer (two would halve the waves per SIMD).
+    float2 a0[V], p0[V];
+    for (;;) {
+      load(i0, a0, p0);
+      consume(i0, a0, p0);
+      advance(i0);
+      if (i0.s >= s1) break;
+    }
   }
 }
 
@@ -872,12 +1079,58 @@ hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* met
   return hipGetLastError();
 }
 
-hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
-                             const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
-                             int astride, const RotMeta* meta, int cmax, int boxmax, int max_chan_atoms, float2* A,
-                             const float2* tw) {
+size_t fwd_wave_lds_bytes(int N, int boxmax, int max_chan_atoms) {
+  int V = 0, RSTR = 0;
+  switch (N) {
+    case 32: V = FwdWaveShape<32>::V; RSTR = FwdWaveShape<32>::RSTR; break;
+    case 48: V = FwdWaveShape<48>::V; RSTR = FwdWaveShape<48>::RSTR; break;
+    case 64: V = FwdWaveShape<64>::V; RSTR = FwdWaveShape<64>::RSTR; break;
+    case 96: V = FwdWaveShape<96>::V; RSTR = FwdWaveShape<96>::RSTR; break;
+    case 128: V = FwdWaveShape<128>::V; RSTR = FwdWaveShape<128>::RSTR; break;
+    default: return 0;
+  }
+  const int npmax = (boxmax + 1) / 2;
+  return (size_t)std::max(1, max_chan_atoms) * sizeof(SpreadAtom) + 4 * (size_t)V * 64 * sizeof(float2) +
+         4 * (size_t)npmax * RSTR * sizeof(float2);
+}
+
+namespace {
+size_t g_fwd_wave_max_lds = 64 * 1024;
+}
+
+hipError_t configure_fwd_wave(size_t max_dyn_lds) {
+  g_fwd_wave_max_lds = max_dyn_lds;
+  const int sizes[] = {32, 48, 64, 96, 128};
+  for (int N : sizes) {
+    hipError_t e = hipSuccess;
+    CDK_DISPATCH_N(N, {
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_planes_wave<PL>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
+    });
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+bool fwd_wave_supported(int N) { return fwd_wave_lds_bytes(N, N, kMaxAtoms) <= g_fwd_wave_max_lds; }
+
+hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot* slots, int nslots,
+                             const LigandDesc* ligs, const int* chan_atoms, const float* wv, const float* wq,
+                             const AtomGrid* ag, int astride, const RotMeta* meta, int cmax, int boxmax,
+                             int max_chan_atoms, float2* A, const float2* tw) {
   if (nslots <= 0) return hipSuccess;
-  const size_t dyn = (size_t)std::max(1, max_chan_atoms) * sizeof(SpreadAtom);
+  const int natom = std::max(1, max_chan_atoms);
+  if (variant == kFwdWave) {
+    const size_t dyn = fwd_wave_lds_bytes(N, boxmax, natom);
+    if (dyn == 0 || dyn > g_fwd_wave_max_lds) return hipErrorInvalidValue;
+    const int npmax = (boxmax + 1) / 2;
+    CDK_DISPATCH_N(N, {
+      hipLaunchKernelGGL(k_fwd_planes_wave<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms,
+                         wv, wq, ag, astride, meta, cmax, boxmax, natom, npmax, A, tw);
+    });
+    return hipGetLastError();
+  }
+  const size_t dyn = (size_t)natom * sizeof(SpreadAtom);
   CDK_DISPATCH_N(N, {
     hipLaunchKernelGGL(k_fwd_planes<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms, wv, wq,
                        ag, astride, meta, cmax, boxmax, A, tw);

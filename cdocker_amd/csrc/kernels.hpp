@@ -26,10 +26,20 @@ hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* met
                               int N, int nb);
 
 // Fused, pruned pipeline stages.
-hipError_t launch_fwd_planes(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
-                             const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag,
-                             int astride, const RotMeta* meta, int cmax, int boxmax, int max_chan_atoms, float2* A,
-                             const float2* tw);
+// Forward plane stage variants: one workgroup per (pose, channel) plane by
+// plane with block-wide LDS FFTs, or its four wavefronts on separate planes
+// with register FFTs.  The wavefront form needs fwd_wave_lds_bytes() of
+// dynamic LDS; configure_fwd_wave() sets the device's per-workgroup limit,
+// and fwd_wave_supported(N) says whether every box and atom count of grid
+// size N fits under it (the variant is then usable for any batch).
+constexpr int kFwdBlock = 0, kFwdWave = 1;
+size_t fwd_wave_lds_bytes(int N, int boxmax, int max_chan_atoms);
+hipError_t configure_fwd_wave(size_t max_dyn_lds);
+bool fwd_wave_supported(int N);
+hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot* slots, int nslots,
+                             const LigandDesc* ligs, const int* chan_atoms, const float* wv, const float* wq,
+                             const AtomGrid* ag, int astride, const RotMeta* meta, int cmax, int boxmax,
+                             int max_chan_atoms, float2* A, const float2* tw);
 hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
                       int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, int npch,
                       float2* T, float invG, const float2* tw);

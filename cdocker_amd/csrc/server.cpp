@@ -7,26 +7,6 @@
 
 namespace cdk {
 
-int conformer_box_bound(const float* xyz, int natom, float h, int N) {
-  double d2 = 0;
-  for (int i = 0; i < natom; ++i)
-    for (int j = i + 1; j < natom; ++j) {
-      double s = 0;
-      for (int k = 0; k < 3; ++k) {
-        const double d = (double)xyz[i * 3 + k] - (double)xyz[j * 3 + k];
-        s += d * d;
-      }
-      d2 = std::max(d2, s);
-    }
-  const int b = (int)std::floor(std::sqrt(d2) / (double)h) + 3;
-  return std::min(N, std::max(b, 2));
-}
-
-size_t fused_bytes_per_pose(int N, int nch, int boxmax, int natom) {
-  const size_t plane = (size_t)N * (N / 2 + 1) * 8;
-  return (size_t)nch * boxmax * plane + (size_t)N * plane + (size_t)natom * 52 + 128;
-}
-
 void Batch::run(Engine engine) {
   if (!prepared || dirty) {
     prepare();
@@ -54,6 +34,11 @@ void Batch::prepare() {
 
 Server::Server(Context& ctx, std::vector<Protein*> prots, size_t budget, int max_batch)
     : ctx_(ctx), prots_(std::move(prots)), budget_(budget), max_batch_(std::max(1, max_batch)) {
+  // The fused engine's working arrays never exceed the context's workspace
+  // budget; that much of the HBM budget is reserved up front.
+  reserved_ = ctx_.ws_budget;
+  if (budget_ <= reserved_)
+    throw Error(kErrBudget, "server HBM budget must exceed the context workspace budget");
   th_ = std::thread([this] { loop(); });
 }
 
@@ -80,14 +65,13 @@ int64_t Server::submit(int prot, int natom, const float* charge, const float* ep
   r->nquat = nquat;
   r->shared = shared;
   r->npose = (size_t)nconf * nquat;
-  int boxmax = 2;
-  for (int c = 0; c < nconf; ++c)
-    boxmax = std::max(boxmax, conformer_box_bound(r->coords.data() + (size_t)c * natom * 3, natom, P.h, P.N));
-  // Footprint: its share of a launch set plus its tables and results.
-  const size_t in_flight = std::min<size_t>(r->npose, (size_t)max_batch_);
-  r->bytes = in_flight * fused_bytes_per_pose(P.N, r->lig->nch(), boxmax, natom) + r->coords.size() * 4 +
-             r->quats.size() * 4 + r->npose * (sizeof(PoseSlot) + sizeof(RotMeta) + 8);
-  if (r->bytes > budget_) throw Error(kErrBudget, "request working set exceeds the server's HBM budget");
+  // Device footprint of the request while admitted: its pose slots,
+  // conformer coordinates, per-pose results and quaternions.  The launch-set
+  // working arrays are the context's fixed workspace, reserved at creation.
+  r->bytes = r->coords.size() * 4 + r->quats.size() * 4 +
+             r->npose * (sizeof(PoseSlot) + sizeof(RotMeta) + sizeof(unsigned long long));
+  if (r->bytes > budget_ - reserved_)
+    throw Error(kErrBudget, "request footprint exceeds the server's HBM budget");
   r->res.energy.resize(r->npose);
   r->res.index.resize(r->npose * 3);
   r->res.mincoor.resize(r->npose * 3);
@@ -134,9 +118,9 @@ void Server::stats(double* out) {
   out[2] = (double)launches_;
   out[3] = launches_ ? (double)poses_done_ / (double)launches_ : 0.0;
   out[4] = busy_s_;
-  out[5] = (double)admitted_;
+  out[5] = (double)(reserved_ + admitted_);
   out[6] = (double)queue_.size();
-  out[7] = (double)peak_;
+  out[7] = (double)(reserved_ + peak_);
 }
 
 void Server::loop() {
@@ -150,7 +134,7 @@ void Server::loop() {
         // Admission by device footprint, in arrival order.
         while (!queue_.empty()) {
           Request* r = reqs_[queue_.front()].get();
-          if (!active_.empty() && admitted_ + r->bytes > budget_) break;
+          if (!active_.empty() && reserved_ + admitted_ + r->bytes > budget_) break;
           admitted_ += r->bytes;
           peak_ = std::max(peak_, admitted_);
           r->state = Request::kActive;
@@ -194,12 +178,12 @@ void Server::loop() {
         owners.push_back(r);
       }
     }
-    std::vector<PoseResults> out;
     const auto t0 = std::chrono::steady_clock::now();
     int code = kOk;
     std::string msg;
     try {
-      run_search(ctx_, *prots_[prot], jobs, kEngineFused, out);
+      prepare_search(ctx_, *prots_[prot], jobs, prep_);
+      execute_search(ctx_, prep_, kEngineFused, false);
     } catch (const Error& e) {
       code = e.code;
       msg = e.what();
@@ -216,9 +200,10 @@ void Server::loop() {
         Request* r = owners[i];
         const size_t b = jobs[i].pose_begin, n = jobs[i].npose();
         if (code == kOk) {
-          std::copy(out[i].energy.begin(), out[i].energy.end(), r->res.energy.begin() + b);
-          std::copy(out[i].index.begin(), out[i].index.end(), r->res.index.begin() + b * 3);
-          std::copy(out[i].mincoor.begin(), out[i].mincoor.end(), r->res.mincoor.begin() + b * 3);
+          const PoseResults& o = prep_.results[i];
+          std::copy(o.energy.begin(), o.energy.end(), r->res.energy.begin() + b);
+          std::copy(o.index.begin(), o.index.end(), r->res.index.begin() + b * 3);
+          std::copy(o.mincoor.begin(), o.mincoor.end(), r->res.mincoor.begin() + b * 3);
           poses_done_ += (int64_t)n;
         }
         r->done += n;

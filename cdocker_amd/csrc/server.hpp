@@ -13,10 +13,6 @@
 
 namespace cdk {
 
-// Device working-set bytes of one pose in a fused launch set.
-size_t fused_bytes_per_pose(int N, int nch, int boxmax, int natom);
-int conformer_box_bound(const float* xyz, int natom, float h, int N);
-
 // Many jobs (ligands x conformers x quaternions) searched together; the
 // fused engine mixes their poses in shared launches.
 struct Batch {
@@ -75,7 +71,8 @@ class Server {
   std::vector<Request*> active_;
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs_;
   int64_t next_id_ = 1;
-  size_t admitted_ = 0, peak_ = 0;
+  size_t reserved_ = 0, admitted_ = 0, peak_ = 0;
+  Prepared prep_;  // device tables reused by every launch set
   bool stop_ = false;
   double busy_s_ = 0;
   int64_t n_done_ = 0, poses_done_ = 0, launches_ = 0;

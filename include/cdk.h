@@ -55,9 +55,16 @@ int cdk_context_create(int device, cdk_context** out);
 int cdk_context_destroy(cdk_context* ctx);
 /* Options: "ws_budget_bytes", "ref_batch" (B, default 50 as NoMini:208),
  * "fused_chunk" (poses per fused launch set, 0 = from budget), "mid_group"
- * (poses sharing one LDS copy of the protein spectra), "use_graph",
+ * (poses per mid-stage workgroup, 0 = autotuned), "fwd_variant" (forward
+ * plane stage: 0 block-wide LDS FFTs, 1 one wavefront per plane with
+ * register FFTs, -1 = autotuned once per grid size), "use_graph",
  * "time_stages". */
 int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t value);
+/* Reads an option back; also "last_fwd_variant" and "last_mid_group" (what
+ * the autotuner chose for the last fused search) and "max_dyn_lds" (the
+ * device's dynamic LDS limit per workgroup, bytes).  No reference
+ * counterpart. */
+int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value);
 /* Per-stage device time (HIP events on the context stream) accumulated while
  * "time_stages" is on: ms[8], launches[8].  Stage ids: reference engine
  * 0 prep, 1 spread, 2 R2C, 3 ConjMult, 4 SumGrids, 5 C2R, 6 argmin; fused
@@ -124,10 +131,14 @@ int cdk_batch_geometry(cdk_batch* b, int job_id, int* box, int* windows);
 int cdk_batch_destroy(cdk_batch* b);
 
 /* ---- serving: continuous batching over resident proteins ------------ */
-/* Requests are admitted while their working set fits hbm_budget_bytes, and
- * their poses are batched with whatever else is in flight (at most
- * max_batch_poses per launch set).  Results of a request do not depend on
- * what it was batched with. */
+/* hbm_budget_bytes covers the context's workspace (its "ws_budget_bytes",
+ * reserved at creation; CDK_ERR_BUDGET if the budget does not exceed it) plus
+ * the resident state of admitted requests (pose slots, conformer coordinates,
+ * quaternions and per-pose results, about 104 B per pose).  Requests are
+ * admitted in arrival order while that fits, and their poses are batched
+ * with whatever else is in flight (at most max_batch_poses per launch set,
+ * shared fairly).  Results of a request do not depend on what it was batched
+ * with. */
 int cdk_server_create(cdk_context* ctx, int nprot, cdk_protein* const* prots, int64_t hbm_budget_bytes,
                       int max_batch_poses, cdk_server** out);
 int cdk_server_submit(cdk_server* s, int protein, int natom, const float* charge, const float* epsilon,

@@ -65,7 +65,7 @@ def test_chunking_does_not_change_results(ctx, prots):
     ctx.set_option("mid_group", 3)
     E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"], q)
     ctx.set_option("fused_chunk", 0)
-    ctx.set_option("mid_group", 16)
+    ctx.set_option("mid_group", 0)
     assert same(E0, E1) and np.array_equal(i0, i1)
 
 
@@ -82,14 +82,14 @@ def test_server_alone_equals_in_mix(ctx, prots):
         L["protein_index"] = k % 2
         reqs.append(L)
     # alone: one request at a time through a server
-    srv = cdk.Server(ctx, [p for _, p in prots], 8 << 30, 64)
+    srv = cdk.Server(ctx, [p for _, p in prots], 16 << 30, 64)
     alone = []
     for L in reqs:
         rid = srv.submit(L["protein_index"], L["charge"], L["epsilon"], L["radius"], L["conformers"], L["quats"])
         alone.append(srv.result(rid))
     # mixed: all submitted at once, small launch sets so they interleave
     srv.close()
-    srv = cdk.Server(ctx, [p for _, p in prots], 8 << 30, 24)
+    srv = cdk.Server(ctx, [p for _, p in prots], 16 << 30, 24)
     ids = [srv.submit(L["protein_index"], L["charge"], L["epsilon"], L["radius"], L["conformers"], L["quats"])
            for L in reqs]
     mixed = [srv.result(i) for i in ids]
@@ -108,18 +108,25 @@ def test_server_alone_equals_in_mix(ctx, prots):
 
 def test_server_budget_rejects_oversized(ctx, prots):
     say("server budget")
-    srv = cdk.Server(ctx, [prots[1][1]], 1 << 20, 64)
-    L = workloads.ligand(900, 30, 2)
-    q = workloads.quaternion_sets(901, 2, 100)
+    ws = 8 << 30  # the context's default workspace budget
     with pytest.raises(cdk.CdkError) as ei:
-        srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], q)
+        cdk.Server(ctx, [prots[1][1]], ws, 64)  # nothing left for requests
+    assert ei.value.code == 6
+    srv = cdk.Server(ctx, [prots[1][1]], ws + (64 << 10), 64)
+    L = workloads.ligand(900, 30, 2)
+    small = workloads.quaternion_sets(901, 2, 50)  # ~10 KB of state: admitted
+    big = workloads.quaternion_sets(902, 2, 2000)  # ~420 KB: refused
+    rid = srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], small)
+    assert srv.result(rid) is not None
+    with pytest.raises(cdk.CdkError) as ei:
+        srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], big)
     assert ei.value.code == 6
     srv.close()
 
 
 def test_server_poll_pending_then_done(ctx, prots):
     say("server poll")
-    srv = cdk.Server(ctx, [prots[1][1]], 8 << 30, 256)
+    srv = cdk.Server(ctx, [prots[1][1]], 16 << 30, 256)
     L = workloads.ligand(910, 40, 4)
     q = workloads.quaternion_sets(911, 4, 500)
     rid = srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], q)

@@ -126,3 +126,62 @@ def test_rank_poses_rebuild(ctx, p48):
     for i, rq in enumerate(rank):
         shift = gmin - mc[0, rq].astype(np.float64) + (idx[0, rq].astype(np.float32) * np.float32(prot.spacing))
         assert np.allclose(pose[i], rot[rq].astype(np.float64) + shift, atol=1e-9)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_forward_plane_forms_vs_oracle(ctx, p48, p64, variant):
+    """Both forms of the forward plane stage (block-wide LDS FFTs and one
+    wavefront per plane with register FFTs) against the oracle."""
+    ctx.set_option("fwd_variant", variant)
+    try:
+        for (g, prot), w, nq in ((p48, workloads.config1(), 40), (p64, workloads.config2(nconf=1, nquat=10), 10)):
+            say("forward form", variant, "grid", g["n"])
+            L = w["ligands"][0]
+            lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+            quats = L["quats"][:1, :nq]
+            E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"][:1], quats)
+            assert ctx.get_option("last_fwd_variant") == variant
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+            stats = check_search(prob, L["conformers"][:1], quats, E, idx, mc)
+            say("forward form", variant, stats)
+            assert stats.get("exact", 0) >= nq - 2
+    finally:
+        ctx.set_option("fwd_variant", -1)
+
+
+def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
+    say("autotuner")
+    g, prot = p64
+    w = workloads.config2(nconf=3, nquat=50)
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+    v = ctx.get_option("last_fwd_variant")
+    assert v in (0, 1) and ctx.get_option("last_mid_group") in (4, 8, 16, 32)
+    # a different batch shape keeps the forward form of this grid size
+    E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"][:1], L["quats"][:1, :7])
+    assert ctx.get_option("last_fwd_variant") == v
+    assert np.array_equal(E1.view(np.uint32), E0[:1, :7].view(np.uint32)) and np.array_equal(i1, i0[:1, :7])
+    ctx.set_option("fwd_variant", v)
+    E2, i2, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+    ctx.set_option("fwd_variant", -1)
+    assert np.array_equal(E2.view(np.uint32), E0.view(np.uint32)) and np.array_equal(i2, i0)
+
+
+@pytest.mark.parametrize("name", ["p96", "p128"])
+def test_large_grids_vs_oracle(ctx, name):
+    """96^3 and 128^3 grids, both engines, a few poses each."""
+    g = workloads.protein_grids(name)
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    L = workloads.ligand(77, 18, 1)
+    quats = workloads.quaternion_sets(78, 1, 4)
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    for engine in (cdk.ENGINE_REFERENCE, cdk.ENGINE_FUSED):
+        say("grid", name, "engine", engine)
+        E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats, engine=engine)
+        stats = check_search(prob, L["conformers"], quats, E, idx, mc)
+        say("grid", name, "engine", engine, stats)
+        assert stats.get("exact", 0) >= 3
+    lig.close()
+    prot.close()
