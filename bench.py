@@ -47,7 +47,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def fused_kernel_bytes(geoms, npch, N, group):
+def fused_kernel_bytes(geoms, npch, N, mid_launches):
     """Algorithmic HBM bytes of each fused kernel over a set of poses.
 
     geoms: [(box, win, nch)] per job, box/win [..., 3] per pose, nch the
@@ -55,7 +55,7 @@ def fused_kernel_bytes(geoms, npch, N, group):
     half-spectrum plane).
       forward planes: writes nch*box_x planes per pose (the pruned y/z spectra)
       mid lines:      reads those planes, writes win_x planes (x-inverse done),
-                      reads the protein spectra (npch*H*8) once per group of poses
+                      and reads the protein spectra (npch*H*8) once per launch
       inverse+argmin: reads win_x planes per pose, writes one 8-byte key
     """
     plane = N * (N // 2 + 1) * 8
@@ -67,8 +67,9 @@ def fused_kernel_bytes(geoms, npch, N, group):
         valid = (win.reshape(-1, 3) > 0).all(axis=1)
         bx, wx = bx[valid], wx[valid]
         fwd += float((nch * bx * plane).sum())
-        mid += float((nch * bx * plane + wx * plane).sum()) + npch * H8 * bx.size / group
+        mid += float((nch * bx * plane + wx * plane).sum())
         inv += float((wx * plane + 8).sum())
+    mid += npch * H8 * mid_launches
     return {"prep": 0.0, "fwd_planes": fwd, "mid_lines": mid, "inv_argmin": inv}
 
 
@@ -173,7 +174,7 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     ctx.set_option("time_stages", 0)
     group = ctx.get_option("last_mid_group")
     names = ["prep", "fwd_planes", "mid_lines", "inv_argmin"]
-    kb = fused_kernel_bytes(geoms, npch, N, group)
+    kb = fused_kernel_bytes(geoms, npch, N, int(launches[2]))
     kernels = {}
     for i, nm in enumerate(names):
         if launches[i]:
@@ -189,6 +190,7 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
             "algorithmic_bytes_per_launch": round(dom_bytes)}
     kernels["tuned"] = {"fwd_variant": ["block", "wavefront"][ctx.get_option("last_fwd_variant")],
+                        "inv_variant": ["block", "register"][ctx.get_option("last_inv_variant")],
                         "mid_group": group}
     return kernels, roof
 
@@ -274,7 +276,8 @@ def run_single(args, cdk, workloads):
     w = workloads.config2(nconf=args.conformers, nquat=args.quaternions)
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
-    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd)
+    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
+                      inv_variant=args.inv)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     nu, used, _ = lig.channels()
@@ -323,7 +326,8 @@ def run_batched(args, cdk, workloads):
     g = workloads.protein_grids("p64")
     w = workloads.config3()
     log("inputs ready in %.1fs" % (time.time() - t0))
-    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd)
+    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
+                      inv_variant=args.inv)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     N = g["n"]
     ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
@@ -376,7 +380,8 @@ def run_large(args, cdk, workloads):
     w = workloads.config5()
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
-    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd)
+    ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
+                      inv_variant=args.inv)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     C = lig.channels()[0] + 1
@@ -454,6 +459,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
     ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
     ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 wavefront, -1 autotuned")
+    ap.add_argument("--inv", type=int, default=-1, help="inverse plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="Poisson rate as a fraction of capacity")
     ap.add_argument("--serve-batch", type=int, default=4096, help="poses per server launch set")

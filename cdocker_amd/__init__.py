@@ -14,6 +14,8 @@ from .api import (  # noqa: F401
     Rng,
     Server,
     rank_poses,
+    refine_poses,
+    rescore_poses,
     read_grids,
     search,
     sort_index,

@@ -75,6 +75,8 @@ _SIGS = {
     "cdk_ligand_channels": [P, IP, IP, IP],
     "cdk_search": [P, P, P, I, FP, I, FP, I, I, FP, IP, FP],
     "cdk_rank_poses": [P, I, I, FP, FP, IP, FP, I, IP, DP],
+    "cdk_rescore_poses": [P, P, I, DP, DP, DP, IP],
+    "cdk_refine_poses": [P, P, I, DP, I, C.c_double, DP],
     "cdk_batch_create": [P, P, PP],
     "cdk_batch_add": [P, P, I, FP, I, FP, I, IP],
     "cdk_batch_run": [P, I],

@@ -287,6 +287,29 @@ def rank_poses(protein, rotated, energy, index, mincoor, n_lowest):
     return rank[:n], pose[:n]
 
 
+def rescore_poses(protein, ligand, poses, want_grad=False):
+    """FP64 grid score of explicit poses [npose][natom][3] (A).  Returns
+    (energy [npose], grad [npose][natom][3] or None, atoms on the grid)."""
+    x = np.ascontiguousarray(np.asarray(poses, np.float64).reshape(-1, ligand.natom, 3))
+    n = x.shape[0]
+    e = np.empty(n, np.float64)
+    g = np.empty_like(x) if want_grad else None
+    ins = np.empty(n, np.int32)
+    check(lib.cdk_rescore_poses(protein.h, ligand.h, n, _dp(x), _dp(e), _dp(g) if want_grad else None, _ip(ins)),
+          "cdk_rescore_poses")
+    return e, g, ins
+
+
+def refine_poses(protein, ligand, poses, steps=50, off_grid_penalty=10.0):
+    """Rigid-body descent on the grid score; returns (refined poses, energy)."""
+    x = np.array(np.asarray(poses, np.float64).reshape(-1, ligand.natom, 3), order="C", copy=True)
+    n = x.shape[0]
+    e = np.empty(n, np.float64)
+    check(lib.cdk_refine_poses(protein.h, ligand.h, n, _dp(x), int(steps), float(off_grid_penalty), _dp(e)),
+          "cdk_refine_poses")
+    return x, e
+
+
 class Batch:
     """Many ligands searched together in shared launches."""
 

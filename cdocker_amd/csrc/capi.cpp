@@ -158,6 +158,9 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
     } else if (k == "fwd_variant") {
       require(v >= -1 && v <= 1, "fwd_variant must be -1 (tuned), 0 (block) or 1 (wavefront)");
       c.fwd_variant = (int)v;
+    } else if (k == "inv_variant") {
+      require(v >= -1 && v <= 1, "inv_variant must be -1 (tuned), 0 (block) or 1 (register)");
+      c.inv_variant = (int)v;
     } else if (k == "use_graph") {
       c.use_graph = v != 0;
     } else if (k == "time_stages") {
@@ -179,6 +182,8 @@ int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value) {
     else if (k == "fused_chunk") *value = c.fused_chunk;
     else if (k == "mid_group") *value = c.mid_group;
     else if (k == "fwd_variant") *value = c.fwd_variant;
+    else if (k == "inv_variant") *value = c.inv_variant;
+    else if (k == "last_inv_variant") *value = c.last_inv;
     else if (k == "use_graph") *value = c.use_graph;
     else if (k == "time_stages") *value = c.time_stages;
     else if (k == "last_fwd_variant") *value = c.last_fwd;
@@ -326,6 +331,27 @@ int cdk_rank_poses(cdk_protein* p, int natom, int nquat, const float* rotated, c
           out_pose[((size_t)i * natom + a) * 3 + k] = (double)rotated[((size_t)r * natom + a) * 3 + k] + shift;
       }
     }
+  });
+}
+
+int cdk_rescore_poses(cdk_protein* p, const cdk_ligand* l, int npose, const double* coords, double* energy,
+                      double* grad, int* n_inside) {
+  return guard([&] {
+    require(p && l && energy && (coords || npose == 0), "NULL argument");
+    require(l->prot == p, "ligand was set up against another protein");
+    require(npose >= 0, "bad pose count");
+    cdk::rescore_poses(*p->p.ctx, p->p, *l->l, npose, coords, energy, grad, n_inside);
+  });
+}
+
+int cdk_refine_poses(cdk_protein* p, const cdk_ligand* l, int npose, double* coords, int steps,
+                     double off_grid_penalty, double* energy) {
+  return guard([&] {
+    require(p && l && energy && (coords || npose == 0), "NULL argument");
+    require(l->prot == p, "ligand was set up against another protein");
+    require(npose >= 0 && steps >= 0 && steps <= 100000, "bad pose or step count");
+    require(std::isfinite(off_grid_penalty) && off_grid_penalty >= 0, "penalty must be finite and >= 0");
+    cdk::refine_poses(*p->p.ctx, p->p, *l->l, npose, coords, steps, off_grid_penalty, energy);
   });
 }
 

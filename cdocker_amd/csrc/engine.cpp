@@ -8,6 +8,7 @@
 #include <limits>
 
 #include "kernels.hpp"
+#include "rescore.hpp"
 
 namespace cdk {
 
@@ -416,18 +417,31 @@ void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, siz
 }
 
 struct Tuned {
-  int fwd = kFwdBlock, group = 16;
+  int fwd = kFwdBlock, inv = kInvBlock, group = 16;
 };
 
-// Autotuner: picks the forward plane form (once per grid size, among the
-// forms that fit every batch of that size) and the mid-stage group size (per
-// launch shape) by timing them on the first launch set of this search.
+// Autotuner: picks the forward and inverse plane forms (once per grid size,
+// among the forms that fit every batch of that size) and the mid-stage group
+// size (per launch shape) by timing them on the first launch set of this
+// search.
 Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   const int N = prot.N;
   Tuned tu;
   const bool wave_ok = fwd_wave_supported(N);
+  const bool reg_ok = inv_reg_supported(N);
   const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)chunk};
-  bool need_fwd = false, need_group = false;
+  bool need_fwd = false, need_inv = false, need_group = false;
+  if (ctx.inv_variant >= 0) {
+    tu.inv = (ctx.inv_variant == kInvReg && reg_ok) ? kInvReg : kInvBlock;
+  } else {
+    auto it = ctx.tuned_inv.find(N);
+    if (it != ctx.tuned_inv.end())
+      tu.inv = it->second;
+    else if (reg_ok)
+      need_inv = true;
+    else
+      ctx.tuned_inv[N] = tu.inv = kInvBlock;
+  }
   if (ctx.fwd_variant >= 0) {
     tu.fwd = (ctx.fwd_variant == kFwdWave && wave_ok) ? kFwdWave : kFwdBlock;
   } else {
@@ -448,7 +462,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     else
       need_group = true;
   }
-  if (!need_fwd && !need_group) return tu;
+  if (!need_fwd && !need_inv && !need_group) return tu;
 
   const int n = (int)std::min(chunk, t.slots.size());
   const float2* tw = ctx.twiddle(N);
@@ -504,6 +518,17 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       }
       ctx.tuned_group[gkey] = tu.group;
     }
+    if (need_inv) {
+      auto inv = [&](int variant) {
+        hip_check(launch_inv_argmin(ctx.stream, N, variant, d_meta, n, N - 2, ctx.d_T.as<float2>(),
+                                    t.d_keys.as<unsigned long long>(), tw),
+                  "tune inverse");
+      };
+      const float tb = timed([&] { inv(kInvBlock); });
+      const float tr = timed([&] { inv(kInvReg); });
+      tu.inv = tr < tb ? kInvReg : kInvBlock;
+      ctx.tuned_inv[N] = tu.inv;
+    }
   } catch (...) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -547,7 +572,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
               "mid");
     timer.end(e);
     e = timer.begin(3);
-    hip_check(launch_inv_argmin(ctx.stream, N, d_meta + s, n, N - 2, ctx.d_T.as<float2>(), d_keys + s, tw),
+    hip_check(launch_inv_argmin(ctx.stream, N, tu.inv, d_meta + s, n, N - 2, ctx.d_T.as<float2>(), d_keys + s, tw),
               "inverse + argmin");
     timer.end(e);
   }
@@ -609,10 +634,11 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
     (void)ctx.twiddle(prot.N);
     const Tuned tu = tune_fused(ctx, prot, t, chunk);
     ctx.last_fwd = tu.fwd;
+    ctx.last_inv = tu.inv;
     ctx.last_group = tu.group;
     if (use_graph && !ctx.time_stages) {
       if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.group ||
-          t.graph_fwd != tu.fwd) {
+          t.graph_fwd != tu.fwd || t.graph_inv != tu.inv) {
         t.drop_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -631,6 +657,7 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
         t.graph_chunk = chunk;
         t.graph_group = tu.group;
         t.graph_fwd = tu.fwd;
+        t.graph_inv = tu.inv;
       }
       hip_check(hipGraphLaunch(t.graph, ctx.stream), "hipGraphLaunch");
     } else {
@@ -687,6 +714,87 @@ void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs,
   prepare_search(ctx, prot, jobs, t);
   execute_search(ctx, t, engine, false);
   out = std::move(t.results);
+}
+
+namespace {
+
+// Per-atom protein channel (vdW probe of the atom, -1 if none) and weights.
+void atom_channels(const Protein& prot, const Ligand& lig, std::vector<int>& pch) {
+  pch.assign(lig.natom, -1);
+  for (int a = 0; a < lig.natom; ++a) {
+    const int c = lig.atom_channel[a];
+    if (c >= 0 && c < lig.num_vdw_used) pch[a] = lig.idx_vdw_used[c];
+  }
+  (void)prot;
+}
+
+struct PoseScratch {
+  DevBuf coords, pch, wv, wq, energy, grad, inside;
+};
+
+void stage_pose_inputs(Context& ctx, const Protein& prot, const Ligand& lig, int npose, const double* coords,
+                       PoseScratch& w) {
+  std::vector<int> pch;
+  atom_channels(prot, lig, pch);
+  std::vector<float> wv(lig.natom), wq(lig.natom);
+  for (int a = 0; a < lig.natom; ++a) {
+    wv[a] = std::sqrt(lig.eps[a]);
+    wq[a] = lig.charge[a];
+  }
+  upload(ctx, w.pch, pch);
+  upload(ctx, w.wv, wv);
+  upload(ctx, w.wq, wq);
+  const size_t n = (size_t)npose * lig.natom * 3;
+  w.coords.ensure(std::max<size_t>(n, 1) * sizeof(double));
+  if (n) hip_check(hipMemcpyAsync(w.coords.p, coords, n * sizeof(double), hipMemcpyHostToDevice, ctx.stream), "coords");
+  w.energy.ensure(std::max(npose, 1) * sizeof(double));
+}
+
+}  // namespace
+
+void rescore_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npose, const double* coords,
+                   double* energy, double* grad, int* inside) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  if (npose <= 0) return;
+  PoseScratch w;
+  stage_pose_inputs(ctx, prot, lig, npose, coords, w);
+  const size_t n = (size_t)npose * lig.natom * 3;
+  if (grad) w.grad.ensure(n * sizeof(double));
+  if (inside) w.inside.ensure((size_t)npose * sizeof(int));
+  const double gmin[3] = {prot.grid_min(0), prot.grid_min(1), prot.grid_min(2)};
+  hip_check(launch_rescore(ctx.stream, npose, lig.natom, w.coords.as<double>(), w.pch.as<int>(), w.wv.as<float>(),
+                           w.wq.as<float>(), prot.grid.as<float>(), prot.N, prot.nch - 1, gmin, (double)prot.h,
+                           w.energy.as<double>(), grad ? w.grad.as<double>() : nullptr,
+                           inside ? w.inside.as<int>() : nullptr),
+            "rescore");
+  hip_check(hipMemcpyAsync(energy, w.energy.p, (size_t)npose * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+            "energy D2H");
+  if (grad)
+    hip_check(hipMemcpyAsync(grad, w.grad.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream), "grad D2H");
+  if (inside)
+    hip_check(hipMemcpyAsync(inside, w.inside.p, (size_t)npose * sizeof(int), hipMemcpyDeviceToHost, ctx.stream),
+              "inside D2H");
+  hip_check(hipStreamSynchronize(ctx.stream), "rescore sync");
+}
+
+void refine_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npose, double* coords, int steps,
+                  double penalty, double* energy) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  if (npose <= 0) return;
+  PoseScratch w;
+  stage_pose_inputs(ctx, prot, lig, npose, coords, w);
+  const size_t n = (size_t)npose * lig.natom * 3;
+  const double gmin[3] = {prot.grid_min(0), prot.grid_min(1), prot.grid_min(2)};
+  hip_check(launch_refine(ctx.stream, npose, lig.natom, w.coords.as<double>(), w.pch.as<int>(), w.wv.as<float>(),
+                          w.wq.as<float>(), prot.grid.as<float>(), prot.N, prot.nch - 1, gmin, (double)prot.h, steps,
+                          penalty, w.energy.as<double>()),
+            "refine");
+  hip_check(hipMemcpyAsync(coords, w.coords.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream), "coords D2H");
+  hip_check(hipMemcpyAsync(energy, w.energy.p, (size_t)npose * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+            "energy D2H");
+  hip_check(hipStreamSynchronize(ctx.stream), "refine sync");
 }
 
 }  // namespace cdk

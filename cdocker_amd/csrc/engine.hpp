@@ -137,7 +137,7 @@ struct Prepared {
   hipGraphExec_t graph = nullptr;
   uint64_t graph_gen = 0;
   size_t graph_chunk = 0;
-  int graph_group = 0, graph_fwd = -1;
+  int graph_group = 0, graph_fwd = -1, graph_inv = -1;
   std::vector<PoseResults> results;
   Prepared() = default;
   Prepared(const Prepared&) = delete;
@@ -157,16 +157,19 @@ struct Context {
   int fused_chunk = 0;            // 0 = from budget
   int mid_group = 0;              // poses per mid-stage workgroup; 0 = tuned
   int fwd_variant = -1;           // forward plane stage form; -1 = tuned
+  int inv_variant = -1;           // inverse plane stage form; -1 = tuned
   bool use_graph = true;
   bool time_stages = false;
-  // Autotuner state.  The forward form changes FFT rounding, so it is chosen
-  // once per grid size and kept for the context's life (results then do not
-  // depend on how poses are batched); the mid group size only changes the
-  // schedule and is tuned per launch shape.
+  // Autotuner state.  The forward and inverse plane forms change FFT
+  // rounding, so each is chosen once per grid size and kept for the
+  // context's life (results then do not depend on how poses are batched);
+  // the mid group size only changes the schedule and is tuned per launch
+  // shape.
   size_t max_dyn_lds = 64 * 1024;
   std::map<int, int> tuned_fwd;                      // N -> variant
+  std::map<int, int> tuned_inv;                      // N -> variant
   std::map<std::array<int, 4>, int> tuned_group;     // (N, cmax, boxmax, chunk) -> group
-  int last_fwd = -1, last_group = 0;
+  int last_fwd = -1, last_inv = -1, last_group = 0;
   StageTimes stage_times;
   // per-launch-set working arrays; ws_generation counts reallocations so a
   // captured graph that baked in the old addresses is rebuilt
@@ -195,6 +198,14 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph);
 // prepare + execute without a graph.
 void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,
                 std::vector<PoseResults>& out);
+
+// Grid score of explicit poses (FP64 trilinear, see rescore.hip): coords
+// [npose][natom][3] in A; energy [npose]; grad [npose][natom][3] and inside
+// [npose] optional.  refine_poses moves each pose rigidly downhill in place.
+void rescore_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npose, const double* coords,
+                   double* energy, double* grad, int* inside);
+void refine_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npose, double* coords, int steps,
+                  double penalty, double* energy);
 
 // Decode helpers shared with tests.
 float key_energy(unsigned long long key);

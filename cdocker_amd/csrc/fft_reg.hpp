@@ -4,12 +4,18 @@
 // x[n2 + L*n1] in register n1.  The transform is
 //   1. DFT_V over the lane's registers,
 //   2. twiddle W_N^(n2*k1),
-//   3. one transpose through LDS (XOR-swizzled so that both the write and
-//      the read touch every 8-byte bank slot exactly twice per instruction),
+//   3. one transpose through LDS,
 //   4. V/L DFT_L per lane over the values gathered from the L lanes,
 // and leaves lane (line, j) holding X[j + L*m] in register m -- the same
 // layout as the input, so a forward transform, a pointwise product and an
 // inverse transform chain with no reordering.  64/L lines per wavefront.
+//
+// The transpose buffer has rows of LPW + PAD float2 (row k1*L + n2 holds the
+// wavefront's lines for that value).  The padding makes every lane's write
+// and read addresses a per-lane base plus a compile-time offset (one address
+// register each way) while keeping the 8-byte bank slots evenly used: with
+// L = 8 a row is 9 slots, so the 8 lanes that read one line from 8 rows
+// j*L + m2 land 8 slots apart; with L = 4 a row is 20 slots (rows 16 apart).
 #pragma once
 #include "fft_lds.hpp"
 
@@ -149,7 +155,11 @@ template <int N> struct RegShape {
   static constexpr int L = RegPlan<N>::L;
   static constexpr int V = N / L;
   static constexpr int LPW = 64 / L;  // lines per wavefront
+  static constexpr int PAD = L == 8 ? 1 : 4;
+  static constexpr int ROW = LPW + PAD;
+  static constexpr int SLICE = V * L * ROW;  // float2 per wavefront
   static_assert(V % L == 0, "four-step split needs V divisible by L");
+  static_assert(L == 8 || L == 4, "padding chosen for 4 or 8 lanes per line");
 };
 
 // Per-lane twiddles W_N^(n2*k1), k1 < V, from the forward table
@@ -161,11 +171,12 @@ __device__ __forceinline__ void reg_twiddles(const float2* __restrict__ tw, int 
 }
 
 // One transform of the wavefront's lines.  `lds` is the wavefront's slice of
-// V*64 float2.  ln = line within the wavefront, n2 = lane within the line.
+// RegShape<N>::SLICE float2.  ln = line within the wavefront, n2 = lane
+// within the line.
 // HALF: inputs x[n] with n >= N/2 are zero (registers V/2.. are not read).
 template <int N, int S, bool HALF = false>
 __device__ __forceinline__ void reg_fft(float2* v, const float2* twl, float2* lds, int ln, int n2) {
-  constexpr int L = RegShape<N>::L, V = RegShape<N>::V, LPW = RegShape<N>::LPW;
+  constexpr int L = RegShape<N>::L, V = RegShape<N>::V;
   if constexpr (HALF)
     RegDFTHalf<V, S>::run(v);
   else
@@ -176,16 +187,18 @@ __device__ __forceinline__ void reg_fft(float2* v, const float2* twl, float2* ld
     if (S > 0) w.y = -w.y;
     v[k1] = c_mul(v[k1], w);
   }
+  constexpr int ROW = RegShape<N>::ROW;
   WaveSync::sync();  // previous reads of the slice are done
+  float2* wb = lds + n2 * ROW + ln;
 #pragma unroll
-  for (int k1 = 0; k1 < V; ++k1) lds[(k1 * L + (n2 ^ (k1 % L))) * LPW + ln] = v[k1];
+  for (int k1 = 0; k1 < V; ++k1) wb[k1 * L * ROW] = v[k1];
   WaveSync::sync();
-  const int j = n2;
+  const float2* rb = lds + n2 * L * ROW + ln;  // row (j*L + m2), j = n2
 #pragma unroll
   for (int q = 0; q < V / L; ++q) {
     float2 u[L];
 #pragma unroll
-    for (int m2 = 0; m2 < L; ++m2) u[m2] = lds[((j + L * q) * L + (m2 ^ j)) * LPW + ln];
+    for (int m2 = 0; m2 < L; ++m2) u[m2] = rb[(q * L * L + m2) * ROW];
     SmallDFT<L, S>::run(u);
 #pragma unroll
     for (int k2 = 0; k2 < L; ++k2) v[q + (V / L) * k2] = u[k2];

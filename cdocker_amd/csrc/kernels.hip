@@ -537,15 +537,20 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
       const int lane = threadIdx.x;
       const int dy = (lane >> 1) & 1, dz = lane & 1;
       float* cells = reinterpret_cast<float*>(sA);
-      for (int i = 0; i < nc; ++i) {
-        const int dx = x - sa[i].ix;
-        if (dx != 0 && dx != 1) continue;  // wave-uniform
-        if (lane < 4) {
-          const int y = sa[i].iy + dy, z = sa[i].iz + dz;
-          float* cell = cells + (((y >> 1) * N + z) << 1) + (y & 1);
-          *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, dx), sa[i].fy, sa[i].fz, dy, dz);
+      for (int base = 0; base < nc; base += 64) {
+        const int aix = base + lane < nc ? sa[base + lane].ix : -4;
+        unsigned long long mk = __ballot(x - aix == 0 || x - aix == 1);
+        while (mk) {
+          const int i = base + __ffsll((long long)mk) - 1;
+          mk &= mk - 1;
+          const int dx = x - sa[i].ix;
+          if (lane < 4) {
+            const int y = sa[i].iy + dy, z = sa[i].iz + dz;
+            float* cell = cells + (((y >> 1) * N + z) << 1) + (y & 1);
+            *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, dx), sa[i].fy, sa[i].fz, dy, dz);
+          }
+          WaveSync::sync();
         }
-        WaveSync::sync();
       }
     }
     __syncthreads();
@@ -622,8 +627,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int ln = lane % LPW, n2 = lane / LPW;
-  float2* tr = reinterpret_cast<float2*>(sa + natom_lds) + wave * (V * 64);
-  float2* rows = reinterpret_cast<float2*>(sa + natom_lds) + 4 * (V * 64) + wave * (npmax * RSTR);
+  constexpr int SLICE = RegShape<N>::SLICE;
+  float2* tr = reinterpret_cast<float2*>(sa + natom_lds) + wave * SLICE;
+  float2* rows = reinterpret_cast<float2*>(sa + natom_lds) + 4 * SLICE + wave * (npmax * RSTR);
   float2 twl[V];
   reg_twiddles<N>(twg, n2, twl);
   const int bxm = min(m.box[0], boxmax), by = m.box[1], bz = m.box[2];
@@ -632,6 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
   const int zlim = zhalf ? N / 2 : N;
   float2* op0 = A + (((size_t)s * cmax + c) * boxmax) * (size_t)(N * ZH);
   const int dy = (lane >> 1) & 1, dz = lane & 1;
+  const int ix0 = lane < nc ? sa[lane].ix : -4;
   for (int x = wave; x < bxm; x += 4) {
     // 1. zero, then the ordered scatter: atoms touching plane x in ascending
     // order, the 4 corners of one atom in lanes 0-3 (distinct cells), so each
@@ -641,15 +648,22 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
       for (int z = lane; z < zlim; z += 64) rows[p * RSTR + z] = make_float2(0.f, 0.f);
     WaveSync::sync();
     float* cells = reinterpret_cast<float*>(rows);
-    for (int i = 0; i < nc; ++i) {
-      const int ddx = x - sa[i].ix;
-      if (ddx != 0 && ddx != 1) continue;  // wave-uniform
-      if (lane < 4) {
-        const int y = sa[i].iy + dy, z = sa[i].iz + dz;
-        float* cell = cells + (((y >> 1) * RSTR + z) << 1) + (y & 1);
-        *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, ddx), sa[i].fy, sa[i].fz, dy, dz);
+    for (int base = 0; base < nc; base += 64) {
+      // Atoms touching plane x, as a mask over this chunk of 64; set bits
+      // are taken lowest first, which keeps ascending atom order.
+      const int aix = base == 0 ? ix0 : (base + lane < nc ? sa[base + lane].ix : -4);
+      unsigned long long mk = __ballot(x - aix == 0 || x - aix == 1);
+      while (mk) {
+        const int i = base + __ffsll((long long)mk) - 1;
+        mk &= mk - 1;
+        const int ddx = x - sa[i].ix;
+        if (lane < 4) {
+          const int y = sa[i].iy + dy, z = sa[i].iz + dz;
+          float* cell = cells + (((y >> 1) * RSTR + z) << 1) + (y & 1);
+          *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, ddx), sa[i].fy, sa[i].fz, dy, dz);
+        }
+        WaveSync::sync();
       }
-      WaveSync::sync();
     }
     // 2. z transforms of the row pairs, written back in place.
     for (int p0 = 0; p0 < npair; p0 += LPW) {
@@ -765,11 +779,18 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   constexpr int LPB = 4 * LPW;           // lines per workgroup
   constexpr int NLG = (NL + LPB - 1) / LPB;
   constexpr int PERX = (NLG + 7) / 8;    // line groups per XCD
-  __shared__ float2 sbuf[4][V * 64];
+  // Long lines read their twiddles from LDS (registers go to the data).
+  constexpr bool TW_LDS = V >= 16;
+  __shared__ float2 sbuf[4][RS::SLICE];
+  __shared__ float2 stw[TW_LDS ? L * V : 1];
   const int w = blockIdx.x;
   const int lg = (w & 7) * PERX + ((w >> 3) % PERX);
   const int sg = (w >> 3) / PERX;
   if (lg >= NLG) return;
+  if constexpr (TW_LDS) {
+    for (int t = threadIdx.x; t < L * V; t += blockDim.x) stw[t] = twg[(t / V) * (t % V)];
+    __syncthreads();
+  }
   // Wave index and line offset are wave-uniform; say so, so the buffer
   // descriptors built from them live in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -779,8 +800,9 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   if (l0 >= NL) return;
   const bool lok = l0 + ln < NL;
   const size_t plane = (size_t)NL;
-  float2 twl[V];
-  reg_twiddles<N>(twg, n2, twl);
+  float2 twr[TW_LDS ? 1 : V];
+  if constexpr (!TW_LDS) reg_twiddles<N>(twg, n2, twr);
+  const float2* twl = TW_LDS ? stw + n2 * V : twr;
   float2* lds = sbuf[wave];
   const int s1 = min(nslots, sg * group + group);
   // Lane offset of row x = n2 within a plane-major block; row n1 adds L*n1
@@ -923,6 +945,147 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin(const RotMeta* __restrict
     const float v = (y & 1) ? v2.y : v2.x;
     const unsigned long long k = energy_key(v, (unsigned)((xp * N + y) * N + z));
     best = k < best ? k : best;
+  }
+  block_min_commit(best, keys + s);
+}
+
+// Stage C, register form: one workgroup per (pose, group of XPB window
+// planes x').  Per plane the four wavefronts
+//   1. inverse-transform the half-spectrum columns along y (register
+//      four-step FFT; columns 0 and N/2 are Hermitian along y, so their
+//      transforms are real and share one complex line),
+//   2. write rows y < window into an LDS tile [y][kz] (one block barrier),
+//   3. C2R the row pairs along z (two real rows per complex line, the
+//      Hermitian half extended on the fly),
+// keep a running (energy, index) key minimum over the window, and commit it
+// once per workgroup with atomicMin.  The next plane's column loads are
+// issued before step 3.  Tile rows have stride RS = ZH + pad with RS % 32 =
+// 4 (8 lanes per line) or 2 (4 lanes per line) float2, so both the row
+// writes of step 2 and the paired row reads of step 3 spread over the
+// 8-byte bank slots.
+template <int N>
+struct InvRegShape {
+  static constexpr int L = RegShape<N>::L;
+  static constexpr int ZH = N / 2 + 1;
+  static constexpr int T32 = L == 8 ? 4 : 2;
+  static constexpr int RS = ZH + ((T32 - ZH % 32) + 32) % 32;
+  static constexpr int XPB = 4;  // window planes per workgroup
+};
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_inv_argmin_reg(const RotMeta* __restrict__ meta,
+                                                           const float2* __restrict__ T,
+                                                           unsigned long long* __restrict__ keys,
+                                                           const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  using RS_ = RegShape<N>;
+  using IS = InvRegShape<N>;
+  constexpr int L = RS_::L, V = RS_::V, LPW = RS_::LPW, RS = IS::RS, XPB = IS::XPB;
+  constexpr int NYL = N / 2;  // y lines: columns 1..N/2-1 and the packed {0, N/2}
+  __shared__ float2 tile[N * RS];
+  __shared__ float2 sbuf[4][RS_::SLICE];
+  const int s = blockIdx.y;
+  const RotMeta m = meta[s];
+  const int x0 = blockIdx.x * XPB;
+  if (!m.valid || x0 >= m.win[0]) return;
+  const int x1 = min(x0 + XPB, m.win[0]);
+  const int wy = m.win[1], wz = m.win[2];
+  const int npw = (wy + 1) >> 1;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int ln = lane % LPW, n2 = lane / LPW;
+  float2 twl[V];
+  reg_twiddles<N>(twg, n2, twl);
+  float2* lds = sbuf[wave];
+  unsigned long long best = ~0ull;
+  auto load_cols = [&](int xp, int pass, float2* v) {
+    const int j = (pass * 4 + wave) * LPW + ln;
+    const float2* ip = T + ((size_t)s * N + xp) * (size_t)(N * ZH);
+#pragma unroll
+    for (int n1 = 0; n1 < V; ++n1) {
+      const int ky = n2 + L * n1;
+      float2 r = make_float2(0.f, 0.f);
+      if (j == 0) {
+        const float2 a = ip[ky * ZH], b = ip[ky * ZH + N / 2];
+        r = make_float2(a.x - b.y, a.y + b.x);
+      } else if (j < NYL) {
+        r = ip[ky * ZH + j];
+      }
+      v[n1] = r;
+    }
+  };
+  constexpr int YPASS = (NYL + 4 * LPW - 1) / (4 * LPW);
+  float2 nxt[V];
+  load_cols(x0, 0, nxt);
+  for (int xp = x0; xp < x1; ++xp) {
+    // 1-2. y inverse transforms into the tile.
+#pragma unroll
+    for (int pass = 0; pass < YPASS; ++pass) {
+      float2 v[V];
+      if (pass == 0) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[i] = nxt[i];
+      } else {
+        load_cols(xp, pass, v);
+      }
+      reg_fft<N, 1>(v, twl, lds, ln, n2);
+      const int j = (pass * 4 + wave) * LPW + ln;
+      if (j < NYL) {
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int y = n2 + L * q;
+          if (y < 2 * npw) {
+            if (j == 0) {
+              tile[y * RS] = make_float2(v[q].x, 0.f);
+              tile[y * RS + N / 2] = make_float2(v[q].y, 0.f);
+            } else {
+              tile[y * RS + j] = v[q];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (xp + 1 < x1) load_cols(xp + 1, 0, nxt);
+    // 3. z C2R of the row pairs and the window minimum.
+    for (int p0 = wave * LPW; p0 < npw; p0 += 4 * LPW) {
+      const int p = p0 + ln;
+      const bool ok = p < npw;
+      float2 z[V];
+#pragma unroll
+      for (int n1 = 0; n1 < V; ++n1) {
+        const int k = n2 + L * n1;
+        float2 r = make_float2(0.f, 0.f);
+        if (ok) {
+          const int kk = k <= N / 2 ? k : N - k;
+          float2 A = tile[(2 * p) * RS + kk], B = tile[(2 * p + 1) * RS + kk];
+          if (k > N / 2) {
+            A.y = -A.y;
+            B.y = -B.y;
+          }
+          r = make_float2(A.x - B.y, A.y + B.x);
+        }
+        z[n1] = r;
+      }
+      reg_fft<N, 1>(z, twl, lds, ln, n2);
+      if (ok) {
+        const int y0 = 2 * p;
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int zz = n2 + L * q;
+          if (zz < wz) {
+            const unsigned lin0 = (unsigned)((xp * N + y0) * N + zz);
+            const unsigned long long k0 = energy_key(z[q].x, lin0);
+            best = k0 < best ? k0 : best;
+            if (y0 + 1 < wy) {
+              const unsigned long long k1 = energy_key(z[q].y, lin0 + (unsigned)N);
+              best = k1 < best ? k1 : best;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // tile reads done before the next plane's writes
   }
   block_min_commit(best, keys + s);
 }
@@ -1080,17 +1243,17 @@ hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* met
 }
 
 size_t fwd_wave_lds_bytes(int N, int boxmax, int max_chan_atoms) {
-  int V = 0, RSTR = 0;
+  int SLICE = 0, RSTR = 0;
   switch (N) {
-    case 32: V = FwdWaveShape<32>::V; RSTR = FwdWaveShape<32>::RSTR; break;
-    case 48: V = FwdWaveShape<48>::V; RSTR = FwdWaveShape<48>::RSTR; break;
-    case 64: V = FwdWaveShape<64>::V; RSTR = FwdWaveShape<64>::RSTR; break;
-    case 96: V = FwdWaveShape<96>::V; RSTR = FwdWaveShape<96>::RSTR; break;
-    case 128: V = FwdWaveShape<128>::V; RSTR = FwdWaveShape<128>::RSTR; break;
+    case 32: SLICE = RegShape<32>::SLICE; RSTR = FwdWaveShape<32>::RSTR; break;
+    case 48: SLICE = RegShape<48>::SLICE; RSTR = FwdWaveShape<48>::RSTR; break;
+    case 64: SLICE = RegShape<64>::SLICE; RSTR = FwdWaveShape<64>::RSTR; break;
+    case 96: SLICE = RegShape<96>::SLICE; RSTR = FwdWaveShape<96>::RSTR; break;
+    case 128: SLICE = RegShape<128>::SLICE; RSTR = FwdWaveShape<128>::RSTR; break;
     default: return 0;
   }
   const int npmax = (boxmax + 1) / 2;
-  return (size_t)std::max(1, max_chan_atoms) * sizeof(SpreadAtom) + 4 * (size_t)V * 64 * sizeof(float2) +
+  return (size_t)std::max(1, max_chan_atoms) * sizeof(SpreadAtom) + 4 * (size_t)SLICE * sizeof(float2) +
          4 * (size_t)npmax * RSTR * sizeof(float2);
 }
 
@@ -1160,9 +1323,21 @@ hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const Ligand
   return hipGetLastError();
 }
 
-hipError_t launch_inv_argmin(hipStream_t st, int N, const RotMeta* meta, int nslots, int wmax, const float2* T,
-                             unsigned long long* keys, const float2* tw) {
+bool inv_reg_supported(int N) { return N <= 64; }
+
+hipError_t launch_inv_argmin(hipStream_t st, int N, int variant, const RotMeta* meta, int nslots, int wmax,
+                             const float2* T, unsigned long long* keys, const float2* tw) {
   if (nslots <= 0) return hipSuccess;
+  if (variant == kInvReg) {
+    if (!inv_reg_supported(N)) return hipErrorInvalidValue;
+    CDK_DISPATCH_N(N, {
+      if constexpr (PL::N <= 64) {
+        const int gx = (wmax + InvRegShape<PL::N>::XPB - 1) / InvRegShape<PL::N>::XPB;
+        hipLaunchKernelGGL(k_inv_argmin_reg<PL>, dim3(gx, nslots), dim3(kBlock), 0, st, meta, T, keys, tw);
+      }
+    });
+    return hipGetLastError();
+  }
   CDK_DISPATCH_N(N, {
     hipLaunchKernelGGL(k_inv_argmin<PL>, dim3(wmax, nslots), dim3(kBlock), 0, st, meta, T, keys, tw);
   });

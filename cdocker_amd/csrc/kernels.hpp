@@ -43,8 +43,13 @@ hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot*
 hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
                       int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, int npch,
                       float2* T, float invG, const float2* tw);
-hipError_t launch_inv_argmin(hipStream_t st, int N, const RotMeta* meta, int nslots, int wmax, const float2* T,
-                             unsigned long long* keys, const float2* tw);
+// Inverse plane stage variants: one workgroup per (pose, window plane) with
+// block-wide LDS FFTs, or one workgroup per (pose, 4 window planes) with
+// register FFTs and one LDS tile (grid sizes up to 64).
+constexpr int kInvBlock = 0, kInvReg = 1;
+bool inv_reg_supported(int N);
+hipError_t launch_inv_argmin(hipStream_t st, int N, int variant, const RotMeta* meta, int nslots, int wmax,
+                             const float2* T, unsigned long long* keys, const float2* tw);
 
 hipError_t launch_rotate(hipStream_t st, int nq, int natom, const float* quats, const float* coords, float* out);
 

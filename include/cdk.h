@@ -57,11 +57,13 @@ int cdk_context_destroy(cdk_context* ctx);
  * "fused_chunk" (poses per fused launch set, 0 = from budget), "mid_group"
  * (poses per mid-stage workgroup, 0 = autotuned), "fwd_variant" (forward
  * plane stage: 0 block-wide LDS FFTs, 1 one wavefront per plane with
- * register FFTs, -1 = autotuned once per grid size), "use_graph",
+ * register FFTs, -1 = autotuned once per grid size), "inv_variant" (inverse
+ * plane stage + argmin: 0 block-wide LDS FFTs per plane, 1 register FFTs over
+ * 4 planes per workgroup, -1 = autotuned once per grid size), "use_graph",
  * "time_stages". */
 int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t value);
-/* Reads an option back; also "last_fwd_variant" and "last_mid_group" (what
- * the autotuner chose for the last fused search) and "max_dyn_lds" (the
+/* Reads an option back; also "last_fwd_variant", "last_inv_variant" and
+ * "last_mid_group" (what the autotuner chose for the last fused search) and "max_dyn_lds" (the
  * device's dynamic LDS limit per workgroup, bytes).  No reference
  * counterpart. */
 int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value);
@@ -117,6 +119,26 @@ int cdk_search(cdk_context* ctx, cdk_protein* p, const cdk_ligand* l, int nconf,
  * cdk_rotate); out_rank[n] rotation indices, out_pose [n][natom][3] double. */
 int cdk_rank_poses(cdk_protein* p, int natom, int nquat, const float* rotated, const float* energy,
                    const int* index, const float* mincoor, int n_lowest, int* out_rank, double* out_pose);
+
+/* ---- retained poses: grid rescoring and rigid refinement ------------- */
+/* Grid score of explicit poses, in FP64: coords [npose][natom][3] absolute
+ * (A, the frame of the poses cdk_rank_poses rebuilds); energy [npose] =
+ * sum_a sqrt(eps_a) P_probe(a)(r_a) + q_a P_elec(r_a), P trilinear in the
+ * protein grids (the interpolation that the search's spreading implies, so a
+ * pose from the search scores its search energy).  grad [npose][natom][3]
+ * (dE/dr, kcal/mol/A) and n_inside [npose] (atoms on the grid; off-grid atoms
+ * add nothing) may be NULL.  Replaces the grid energy of the OpenMM grid
+ * context (TranRotaConfSearchGrid.cu:158-174, AddGridForcesToOpenMMSystem.cpp
+ * :8-130), which interpolates with splines instead. */
+int cdk_rescore_poses(cdk_protein* p, const cdk_ligand* l, int npose, const double* coords, double* energy,
+                      double* grad, int* n_inside);
+/* Rigid-body descent on that score, in place on coords: per pose `steps`
+ * trial moves along the force and the torque about the centroid, kept when
+ * the score drops.  An atom off the grid adds off_grid_penalty.  energy
+ * [npose] receives the final score.  A fast pre-minimisation before the host
+ * OpenMM step (Grid:511-532). */
+int cdk_refine_poses(cdk_protein* p, const cdk_ligand* l, int npose, double* coords, int steps,
+                     double off_grid_penalty, double* energy);
 
 /* ---- batched search: many ligands in one set of launches ------------- */
 int cdk_batch_create(cdk_context* ctx, cdk_protein* p, cdk_batch** out);

@@ -129,10 +129,13 @@ def test_rank_poses_rebuild(ctx, p48):
 
 
 @pytest.mark.parametrize("variant", [0, 1])
-def test_forward_plane_forms_vs_oracle(ctx, p48, p64, variant):
-    """Both forms of the forward plane stage (block-wide LDS FFTs and one
-    wavefront per plane with register FFTs) against the oracle."""
+def test_plane_stage_forms_vs_oracle(ctx, p48, p64, variant):
+    """Both forms of the forward plane stage (block-wide LDS FFTs, one
+    wavefront per plane with register FFTs) and of the inverse plane stage
+    (block-wide LDS FFTs per plane, register FFTs over 4 planes) against the
+    oracle."""
     ctx.set_option("fwd_variant", variant)
+    ctx.set_option("inv_variant", variant)
     try:
         for (g, prot), w, nq in ((p48, workloads.config1(), 40), (p64, workloads.config2(nconf=1, nquat=10), 10)):
             say("forward form", variant, "grid", g["n"])
@@ -141,12 +144,14 @@ def test_forward_plane_forms_vs_oracle(ctx, p48, p64, variant):
             quats = L["quats"][:1, :nq]
             E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"][:1], quats)
             assert ctx.get_option("last_fwd_variant") == variant
+            assert ctx.get_option("last_inv_variant") == variant
             prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
             stats = check_search(prob, L["conformers"][:1], quats, E, idx, mc)
             say("forward form", variant, stats)
             assert stats.get("exact", 0) >= nq - 2
     finally:
         ctx.set_option("fwd_variant", -1)
+        ctx.set_option("inv_variant", -1)
 
 
 def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
@@ -157,14 +162,17 @@ def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
     v = ctx.get_option("last_fwd_variant")
-    assert v in (0, 1) and ctx.get_option("last_mid_group") in (4, 8, 16, 32)
+    vi = ctx.get_option("last_inv_variant")
+    assert v in (0, 1) and vi in (0, 1) and ctx.get_option("last_mid_group") in (4, 8, 16, 32)
     # a different batch shape keeps the forward form of this grid size
     E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"][:1], L["quats"][:1, :7])
-    assert ctx.get_option("last_fwd_variant") == v
+    assert ctx.get_option("last_fwd_variant") == v and ctx.get_option("last_inv_variant") == vi
     assert np.array_equal(E1.view(np.uint32), E0[:1, :7].view(np.uint32)) and np.array_equal(i1, i0[:1, :7])
     ctx.set_option("fwd_variant", v)
+    ctx.set_option("inv_variant", vi)
     E2, i2, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
     ctx.set_option("fwd_variant", -1)
+    ctx.set_option("inv_variant", -1)
     assert np.array_equal(E2.view(np.uint32), E0.view(np.uint32)) and np.array_equal(i2, i0)
 
 
