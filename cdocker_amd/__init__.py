@@ -13,6 +13,7 @@ from .api import (  # noqa: F401
     QuaternionUniformSampling,
     Rng,
     Server,
+    grid_for_openmm,
     rank_poses,
     refine_poses,
     rescore_poses,
@@ -23,4 +24,5 @@ from .api import (  # noqa: F401
     synth_ligand,
     synth_protein,
     write_grids,
+    write_poses,
 )

@@ -77,6 +77,8 @@ _SIGS = {
     "cdk_rank_poses": [P, I, I, FP, FP, IP, FP, I, IP, DP],
     "cdk_rescore_poses": [P, P, I, DP, DP, DP, IP],
     "cdk_refine_poses": [P, P, I, DP, I, C.c_double, DP],
+    "cdk_write_poses": [CP, I, I, I, DP, FP, C.POINTER(CP), C.POINTER(CP), CP],
+    "cdk_grid_for_openmm": [P, I, DP, DP],
     "cdk_batch_create": [P, P, PP],
     "cdk_batch_add": [P, P, I, FP, I, FP, I, IP],
     "cdk_batch_run": [P, I],

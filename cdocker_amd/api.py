@@ -310,6 +310,31 @@ def refine_poses(protein, ligand, poses, steps=50, off_grid_penalty=10.0):
     return x, e
 
 
+def write_poses(directory, conformer, poses, energy, atom_names=None, elements=None, energy_path=None):
+    """conformer_<c>_<i>.pdb files + energy.txt rows (NoMini:489-502)."""
+    x = np.ascontiguousarray(np.asarray(poses, np.float64))
+    n, na = x.shape[0], x.shape[1]
+    e = _f32(energy)
+
+    def strs(v):
+        if v is None:
+            return None
+        arr = (C.c_char_p * na)(*[str(t).encode() for t in v])
+        return C.cast(arr, C.POINTER(C.c_char_p))
+
+    check(lib.cdk_write_poses(str(directory).encode(), int(conformer), n, na, _dp(x), _fp(e), strs(atom_names),
+                              strs(elements), energy_path.encode() if energy_path else None), "cdk_write_poses")
+
+
+def grid_for_openmm(protein, channel):
+    """(values [Z][Y][X] float64 -- x fastest -- , bounds_nm [6])."""
+    n = protein.n
+    v = np.empty((n, n, n), np.float64)
+    b = np.empty(6, np.float64)
+    check(lib.cdk_grid_for_openmm(protein.h, int(channel), _dp(v), _dp(b)), "cdk_grid_for_openmm")
+    return v, b
+
+
 class Batch:
     """Many ligands searched together in shared launches."""
 

@@ -245,7 +245,7 @@ int cdk_protein_spectrum(cdk_protein* p, int channel, float* out) {
 int cdk_protein_bytes(cdk_protein* p, int64_t* bytes) {
   return guard([&] {
     require(p && bytes, "NULL argument");
-    *bytes = (int64_t)(p->p.grid.cap + p->p.spec.cap);
+    *bytes = (int64_t)(p->p.grid.cap + p->p.spec.cap + p->p.spec_lb.cap);
   });
 }
 
@@ -352,6 +352,23 @@ int cdk_refine_poses(cdk_protein* p, const cdk_ligand* l, int npose, double* coo
     require(npose >= 0 && steps >= 0 && steps <= 100000, "bad pose or step count");
     require(std::isfinite(off_grid_penalty) && off_grid_penalty >= 0, "penalty must be finite and >= 0");
     cdk::refine_poses(*p->p.ctx, p->p, *l->l, npose, coords, steps, off_grid_penalty, energy);
+  });
+}
+
+int cdk_write_poses(const char* dir, int conformer, int n, int natom, const double* poses, const float* energy,
+                    const char* const* atom_names, const char* const* elements, const char* energy_path) {
+  return guard([&] {
+    require(n >= 0 && natom > 0 && conformer >= 0, "bad counts");
+    require(n == 0 || (poses && energy), "NULL argument");
+    cdk::write_poses(dir, conformer, n, natom, poses, energy, atom_names, elements, energy_path);
+  });
+}
+
+int cdk_grid_for_openmm(cdk_protein* p, int channel, double* values, double* bounds_nm) {
+  return guard([&] {
+    require(p != nullptr, "NULL protein");
+    require(channel >= 0 && channel < p->p.nch, "channel out of range");
+    cdk::grid_for_openmm(*p->p.ctx, p->p, channel, values, bounds_nm);
   });
 }
 

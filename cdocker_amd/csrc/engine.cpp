@@ -103,6 +103,9 @@ void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const
             "protein upload");
   hip_check(launch_r2c_3d(ctx.stream, N, p.grid.as<float>(), p.spec.as<float2>(), nch, ctx.twiddle(N)),
             "protein R2C");
+  p.spec_lb.ensure((size_t)nch * H * sizeof(float2));
+  hip_check(launch_to_line_blocks(ctx.stream, N, p.spec.as<float2>(), p.spec_lb.as<float2>(), nch),
+            "protein spectra line blocks");
   hip_check(hipStreamSynchronize(ctx.stream), "protein R2C sync");
 }
 
@@ -508,7 +511,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       for (int g : {4, 8, 16, 32}) {
         const float ms = timed([&] {
           hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
-                               prot.spec.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)prot.G(), tw),
+                               prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)prot.G(), tw),
                     "tune mid");
         });
         if (ms < best) {
@@ -568,7 +571,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
     timer.end(e);
     e = timer.begin(2);
     hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
-                         prot.spec.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+                         prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
               "mid");
     timer.end(e);
     e = timer.begin(3);

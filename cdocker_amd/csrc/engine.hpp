@@ -62,6 +62,7 @@ struct Protein {
   std::vector<float> radii;  // nch - 1 probe radii
   DevBuf grid;               // float [nch][G]
   DevBuf spec;               // float2 [nch][H]
+  DevBuf spec_lb;            // the same spectra in the fused engine's line-blocked layout
   size_t G() const { return (size_t)N * N * N; }
   size_t H() const { return (size_t)N * N * (N / 2 + 1); }
   // gridMin = mid - len / 2 evaluated in float, then widened (NoMini:149-151).
@@ -206,6 +207,15 @@ void rescore_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npo
                    double* energy, double* grad, int* inside);
 void refine_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npose, double* coords, int steps,
                   double penalty, double* energy);
+
+// Retained-pose export (export.cpp).
+std::string pose_file_name(int conformer, int rank);
+std::string energy_row(int conformer, int rank, float energy);
+void write_poses(const char* dir, int conformer, int n, int natom, const double* poses, const float* energy,
+                 const char* const* names, const char* const* elements, const char* energy_path);
+// One protein channel in OpenMM's x-fastest order (double) and the grid
+// bounds in nm: min x, y, z then max x, y, z.
+void grid_for_openmm(Context& ctx, const Protein& prot, int channel, double* values, double* bounds_nm);
 
 // Decode helpers shared with tests.
 float key_energy(unsigned long long key);

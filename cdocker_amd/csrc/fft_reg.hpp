@@ -10,12 +10,15 @@
 // layout as the input, so a forward transform, a pointwise product and an
 // inverse transform chain with no reordering.  64/L lines per wavefront.
 //
-// The transpose buffer has rows of LPW + PAD float2 (row k1*L + n2 holds the
-// wavefront's lines for that value).  The padding makes every lane's write
-// and read addresses a per-lane base plus a compile-time offset (one address
-// register each way) while keeping the 8-byte bank slots evenly used: with
-// L = 8 a row is 9 slots, so the 8 lanes that read one line from 8 rows
-// j*L + m2 land 8 slots apart; with L = 4 a row is 20 slots (rows 16 apart).
+// The transpose buffer holds V groups (one per k1) of L rows x LPW lines,
+// each group padded to K = L*LPW + PAD float2.  Lane (line, n2) writes
+// value k1 at k1*K + n2*LPW + line and reads row m2 of group j + L*q at
+// (j + L*q)*K + m2*LPW + line, so every address is a per-lane base plus a
+// compile-time offset (one address register each way).  PAD = 8 (L = 8) or
+// 16 (L = 4) shifts consecutive groups by half a bank row: the 16 or 32
+// lanes served together then hit distinct 8-byte bank slots for the writes
+// (consecutive lines of one group) and for the reads (one row from each of
+// the lanes' groups).
 #pragma once
 #include "fft_lds.hpp"
 
@@ -155,9 +158,9 @@ template <int N> struct RegShape {
   static constexpr int L = RegPlan<N>::L;
   static constexpr int V = N / L;
   static constexpr int LPW = 64 / L;  // lines per wavefront
-  static constexpr int PAD = L == 8 ? 1 : 4;
-  static constexpr int ROW = LPW + PAD;
-  static constexpr int SLICE = V * L * ROW;  // float2 per wavefront
+  static constexpr int PAD = L == 8 ? 8 : 16;
+  static constexpr int K = L * LPW + PAD;  // float2 per k1 group
+  static constexpr int SLICE = V * K;      // float2 per wavefront
   static_assert(V % L == 0, "four-step split needs V divisible by L");
   static_assert(L == 8 || L == 4, "padding chosen for 4 or 8 lanes per line");
 };
@@ -187,18 +190,18 @@ __device__ __forceinline__ void reg_fft(float2* v, const float2* twl, float2* ld
     if (S > 0) w.y = -w.y;
     v[k1] = c_mul(v[k1], w);
   }
-  constexpr int ROW = RegShape<N>::ROW;
+  constexpr int K = RegShape<N>::K, LPW = RegShape<N>::LPW;
   WaveSync::sync();  // previous reads of the slice are done
-  float2* wb = lds + n2 * ROW + ln;
+  float2* wb = lds + n2 * LPW + ln;
 #pragma unroll
-  for (int k1 = 0; k1 < V; ++k1) wb[k1 * L * ROW] = v[k1];
+  for (int k1 = 0; k1 < V; ++k1) wb[k1 * K] = v[k1];
   WaveSync::sync();
-  const float2* rb = lds + n2 * L * ROW + ln;  // row (j*L + m2), j = n2
+  const float2* rb = lds + n2 * K + ln;  // group j = n2
 #pragma unroll
   for (int q = 0; q < V / L; ++q) {
     float2 u[L];
 #pragma unroll
-    for (int m2 = 0; m2 < L; ++m2) u[m2] = rb[(q * L * L + m2) * ROW];
+    for (int m2 = 0; m2 < L; ++m2) u[m2] = rb[q * L * K + m2 * LPW];
     SmallDFT<L, S>::run(u);
 #pragma unroll
     for (int k2 = 0; k2 < L; ++k2) v[q + (V / L) * k2] = u[k2];

@@ -480,6 +480,47 @@ struct PlaneAtom {
   double wx, fy, fz;
 };
 
+// Line-blocked layout of the fused engine's spectra (ligand planes A,
+// correlation planes T, and the protein copy the mid stage reads): lines of
+// the (ky, kz) plane in blocks of B = 64/L consecutive lines, the rows x of
+// one block contiguous.  Element (x, line) of an R-row volume sits at
+// ((line / B) * R + x) * B + line % B, so the 64/L lines a mid-stage
+// wavefront owns are 8 x 64/L contiguous bytes per row and its 8 rows of one
+// load are 512 contiguous bytes.  N * (N/2 + 1) is a multiple of B for every
+// supported N.
+#ifndef CDK_A_LB
+#define CDK_A_LB 1
+#endif
+#ifndef CDK_T_LB
+#define CDK_T_LB 1
+#endif
+template <int N>
+__device__ __forceinline__ int lb_index(int R, int x, int line) {
+  constexpr int B = RegShape<N>::LPW;  // a power of two
+  return ((line / B) * R + x) * B + (line & (B - 1));  // < 2^31 per (slot, channel)
+}
+template <int N>
+__device__ __forceinline__ int a_index(int R, int x, int line) {
+  return CDK_A_LB ? lb_index<N>(R, x, line) : x * (N * (N / 2 + 1)) + line;
+}
+template <int N>
+__device__ __forceinline__ int t_index(int x, int line) {
+  return CDK_T_LB ? lb_index<N>(N, x, line) : x * (N * (N / 2 + 1)) + line;
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_to_line_blocks(const float2* __restrict__ in, float2* __restrict__ out,
+                                                           int nch) {
+  constexpr int N = P::N, NL = N * P::ZH;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per = (size_t)N * NL;
+  if (t >= per * nch) return;
+  const int c = (int)(t / per);
+  const size_t r = t - (size_t)c * per;
+  const int x = (int)(r / NL), line = (int)(r % NL);
+  out[(size_t)c * per + (size_t)lb_index<N>(N, x, line)] = in[t];
+}
+
 // Stage A: one workgroup per (pose, ligand channel).  The channel's atoms are
 // staged in LDS once; then for every occupied x-plane: spread the plane in
 // LDS (FillLigandGrid arithmetic), z R2C of the occupied rows (two real rows
@@ -559,8 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
     unpack_rows<N>(r, u, 2 * npair);
     __syncthreads();
     const float2* res = lds_fft<P, -1, true, ZH>(u, r, ZH, ZH, 1, tw);
-    float2* op = op0 + (size_t)x * (N * ZH);
-    for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op[t] = res[t];
+    for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op0[a_index<N>(boxmax, x, t)] = res[t];
   }
 }
 
@@ -688,7 +728,6 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
     WaveSync::sync();
     // 3. y transforms of the columns; line j = column j, except line 0 =
     // columns 0 (real part) and N/2 (imaginary part).
-    float2* op = op0 + (size_t)x * (N * ZH);
     for (int j0 = 0; j0 < N / 2; j0 += LPW) {
       const int j = j0 + ln;
       const bool jok = j < N / 2;
@@ -731,14 +770,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
           for (int q = 0; q < V; ++q) {
             const int ky = n2 + L * q;
             const float2 u = v[q], w = tr[ky == 0 ? 0 : N - ky];
-            op[ky * ZH + 0] = make_float2(0.5f * (u.x + w.x), 0.5f * (u.y - w.y));
-            op[ky * ZH + N / 2] = make_float2(0.5f * (u.y + w.y), -0.5f * (u.x - w.x));
+            op0[a_index<N>(boxmax, x, ky * ZH)] = make_float2(0.5f * (u.x + w.x), 0.5f * (u.y - w.y));
+            op0[a_index<N>(boxmax, x, ky * ZH + N / 2)] = make_float2(0.5f * (u.y + w.y), -0.5f * (u.x - w.x));
           }
         }
       }
       if (j > 0 && jok) {
 #pragma unroll
-        for (int q = 0; q < V; ++q) op[(n2 + L * q) * ZH + j] = v[q];
+        for (int q = 0; q < V; ++q) op0[a_index<N>(boxmax, x, (n2 + L * q) * ZH + j)] = v[q];
       }
     }
     WaveSync::sync();
@@ -805,11 +844,16 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   const float2* twl = TW_LDS ? stw + n2 * V : twr;
   float2* lds = sbuf[wave];
   const int s1 = min(nslots, sg * group + group);
-  // Lane offset of row x = n2 within a plane-major block; row n1 adds L*n1
-  // planes.  Rows past the descriptor's end (x >= box, x' >= window) read as
-  // zero / are dropped.
-  const int voff0 = lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone;
-  constexpr int kRow = L * NL * (int)sizeof(float2);
+  // Line blocks (lb_index): the wavefront's lines are block l0 / LPW; lane
+  // offset of row x = n2 within it, row n1 adds L*n1 rows.  Rows past the
+  // descriptor's end (x >= box, x' >= window) read as zero / are dropped.
+  const size_t blk = (size_t)(l0 / LPW) * LPW;  // block start, in rows of LPW lines
+  const int voff0 = lok ? (n2 * LPW + ln) * (int)sizeof(float2) : kVoffNone;
+  constexpr int kRow = L * LPW * (int)sizeof(float2);
+  const int voffA = CDK_A_LB ? voff0 : (lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone);
+  constexpr int kRowA = CDK_A_LB ? kRow : L * NL * (int)sizeof(float2);
+  const int voffT = CDK_T_LB ? voff0 : (lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone);
+  constexpr int kRowT = CDK_T_LB ? kRow : L * NL * (int)sizeof(float2);
   // Slot-derived values are wave-uniform; readfirstlane keeps them (and the
   // descriptors built from them) in scalar registers.
   auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
@@ -834,17 +878,20 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
     if (++it.c >= it.nch) enter(it, next_valid(it.s + 1));
   };
   auto load = [&](const Item& it, float2* ra, float2* rp) {
-    const __amdgpu_buffer_rsrc_t r = buffer_rsrc(A + (((size_t)it.s * cmax + it.c) * boxmax) * plane + l0,
-                                                 (unsigned)(it.bx * NL * sizeof(float2)));
+    const __amdgpu_buffer_rsrc_t r =
+        CDK_A_LB ? buffer_rsrc(A + (((size_t)it.s * cmax + it.c) * boxmax) * plane + blk * boxmax,
+                               (unsigned)(it.bx * LPW * sizeof(float2)))
+                 : buffer_rsrc(A + (((size_t)it.s * cmax + it.c) * boxmax) * plane + l0,
+                               (unsigned)(it.bx * NL * sizeof(float2)));
     const int pc = uni(ligs[it.lig].pmap[it.c]);
     const __amdgpu_buffer_rsrc_t q =
-        buffer_rsrc(Phat + (size_t)pc * N * plane + l0, (unsigned)(N * NL * sizeof(float2)));
+        buffer_rsrc(Phat + (size_t)pc * N * plane + blk * N, (unsigned)(N * LPW * sizeof(float2)));
     if (it.bx <= N / 2) {
 #pragma unroll
-      for (int n1 = 0; n1 < V / 2; ++n1) ra[n1] = buffer_load_f2(r, voff0 + n1 * kRow);
+      for (int n1 = 0; n1 < V / 2; ++n1) ra[n1] = buffer_load_f2(r, voffA + n1 * kRowA);
     } else {
 #pragma unroll
-      for (int n1 = 0; n1 < V; ++n1) ra[n1] = buffer_load_f2(r, voff0 + n1 * kRow);
+      for (int n1 = 0; n1 < V; ++n1) ra[n1] = buffer_load_f2(r, voffA + n1 * kRowA);
     }
 #pragma unroll
     for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(q, voff0 + m * kRow);
@@ -873,9 +920,10 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
       reg_fft<N, 1>(acc, twl, lds, ln, n2);
       const int wx = uni(meta[it.s].win[0]);
       const __amdgpu_buffer_rsrc_t rt =
-          buffer_rsrc(T + (size_t)it.s * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
+          CDK_T_LB ? buffer_rsrc(T + (size_t)it.s * N * plane + blk * N, (unsigned)(max(wx, 0) * LPW * sizeof(float2)))
+                   : buffer_rsrc(T + (size_t)it.s * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
 #pragma unroll
-      for (int m = 0; m < V; ++m) buffer_store_f2(acc[m], rt, voff0 + m * kRow);
+      for (int m = 0; m < V; ++m) buffer_store_f2(acc[m], rt, voffT + m * kRowT);
 #pragma unroll
       for (int i = 0; i < V; ++i) acc[i] = make_float2(0.f, 0.f);
     }
@@ -928,8 +976,8 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin(const RotMeta* __restrict
   const RotMeta m = meta[s];
   if (!m.valid || xp >= m.win[0]) return;
   load_twiddles<N>(tw, twg);
-  const float2* ip = T + ((size_t)s * N + xp) * (size_t)(N * ZH);
-  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) sA[t] = ip[t];
+  const float2* ip = T + (size_t)s * N * (size_t)(N * ZH);
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) sA[t] = ip[t_index<N>(xp, t)];
   __syncthreads();
   float2* r = lds_fft<P, 1, true, ZH>(sA, sB, ZH, ZH, 1, tw);
   float2* u = (r == sA) ? sB : sA;
@@ -1000,16 +1048,16 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin_reg(const RotMeta* __rest
   unsigned long long best = ~0ull;
   auto load_cols = [&](int xp, int pass, float2* v) {
     const int j = (pass * 4 + wave) * LPW + ln;
-    const float2* ip = T + ((size_t)s * N + xp) * (size_t)(N * ZH);
+    const float2* ip = T + (size_t)s * N * (size_t)(N * ZH);
 #pragma unroll
     for (int n1 = 0; n1 < V; ++n1) {
       const int ky = n2 + L * n1;
       float2 r = make_float2(0.f, 0.f);
       if (j == 0) {
-        const float2 a = ip[ky * ZH], b = ip[ky * ZH + N / 2];
+        const float2 a = ip[t_index<N>(xp, ky * ZH)], b = ip[t_index<N>(xp, ky * ZH + N / 2)];
         r = make_float2(a.x - b.y, a.y + b.x);
       } else if (j < NYL) {
-        r = ip[ky * ZH + j];
+        r = ip[t_index<N>(xp, ky * ZH + j)];
       }
       v[n1] = r;
     }
@@ -1297,6 +1345,15 @@ hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot*
   CDK_DISPATCH_N(N, {
     hipLaunchKernelGGL(k_fwd_planes<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms, wv, wq,
                        ag, astride, meta, cmax, boxmax, A, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_to_line_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch) {
+  const size_t n = (size_t)nch * N * N * (N / 2 + 1);
+  if (n == 0) return hipSuccess;
+  CDK_DISPATCH_N(N, {
+    hipLaunchKernelGGL(k_to_line_blocks<PL>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, in, out, nch);
   });
   return hipGetLastError();
 }

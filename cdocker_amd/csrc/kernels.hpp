@@ -8,6 +8,9 @@ namespace cdk {
 
 bool grid_edge_supported(int N);
 int mid_line_groups(int N);
+// Copies [nch][N][N*(N/2+1)] spectra into the fused engine's line-blocked
+// layout (blocks of 64/L lines, rows contiguous within a block).
+hipError_t launch_to_line_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
 
 hipError_t launch_prep(hipStream_t st, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
                        const float* coords, int N, float h, int astride, int boxmax, float* rot, AtomGrid* ag,

@@ -223,7 +223,27 @@ __global__ __launch_bounds__(64) void k_refine(int npose, int natom, double* __r
   if (lane == 0) energy[pose] = e;
 }
 
+// CHARMM grid order (z fastest) to OpenMM's Continuous3DFunction order
+// (x fastest), widened to double (AddGridForcesToOpenMMSystem.cpp:30-41).
+__global__ __launch_bounds__(256) void k_grid_xfast(const float* __restrict__ in, int X, int Y, int Z,
+                                                    double* __restrict__ out) {
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)X * Y * Z;
+  if (o >= n) return;
+  const int i = (int)(o % X);
+  const int j = (int)((o / X) % Y);
+  const int k = (int)(o / ((size_t)X * Y));
+  out[o] = (double)in[((size_t)i * Y + j) * Z + k];
+}
+
 }  // namespace
+
+hipError_t launch_grid_xfast(hipStream_t st, const float* in, int X, int Y, int Z, double* out) {
+  const size_t n = (size_t)X * Y * Z;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grid_xfast, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, X, Y, Z, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_rescore(hipStream_t st, int npose, int natom, const double* coords, const int* pch,
                           const float* wv, const float* wq, const float* grid, int N, int ech, const double* gmin,

@@ -16,4 +16,7 @@ hipError_t launch_refine(hipStream_t st, int npose, int natom, double* coords, c
                          const float* wq, const float* grid, int N, int ech, const double* gmin, double h, int steps,
                          double penalty, double* energy);
 
+// out[(k*Y + j)*X + i] = in[(i*Y + j)*Z + k] as double.
+hipError_t launch_grid_xfast(hipStream_t st, const float* in, int X, int Y, int Z, double* out);
+
 }  // namespace cdk

@@ -140,6 +140,19 @@ int cdk_rescore_poses(cdk_protein* p, const cdk_ligand* l, int npose, const doub
 int cdk_refine_poses(cdk_protein* p, const cdk_ligand* l, int npose, double* coords, int steps,
                      double off_grid_penalty, double* energy);
 
+/* Writes n retained poses of one conformer as conformer_<conformer>_<i>.pdb
+ * in dir (HETATM records; atom_names / elements may be NULL) and appends one
+ * row "conformer_<c>_<i>.pdb,<c>,<i>,<energy>" per pose to energy_path (if
+ * not NULL), the drivers' output (TranRotaConfSearchNoMini.cu:489-502).
+ * Host only. */
+int cdk_write_poses(const char* dir, int conformer, int n, int natom, const double* poses, const float* energy,
+                    const char* const* atom_names, const char* const* elements, const char* energy_path);
+/* One protein grid channel in the order OpenMM's Continuous3DFunction takes
+ * (x fastest, double; AddGridForcesToOpenMMSystem.cpp:30-41) and the grid
+ * bounds in nm [min x, y, z, max x, y, z] (TranRotaConfSearchGrid.cu:149-154).
+ * values (X*Y*Z doubles) or bounds_nm may be NULL. */
+int cdk_grid_for_openmm(cdk_protein* p, int channel, double* values, double* bounds_nm);
+
 /* ---- batched search: many ligands in one set of launches ------------- */
 int cdk_batch_create(cdk_context* ctx, cdk_protein* p, cdk_batch** out);
 int cdk_batch_add(cdk_batch* b, const cdk_ligand* l, int nconf, const float* conf_coords, int nquat,

@@ -90,3 +90,29 @@ def test_grid_file_round_trip(tmp_path):
     bad.write_text("a\nb\nc\n2 4 4\n")
     with pytest.raises(cdk.CdkError):
         cdk.read_grids(bad)
+
+
+def test_write_poses_files_and_energy_rows(tmp_path):
+    """conformer_<c>_<i>.pdb files and energy.txt rows as the drivers write
+    them (TranRotaConfSearchNoMini.cu:489-502); runs on the host only."""
+    rs = np.random.default_rng(2)
+    poses = rs.normal(0, 5, size=(3, 4, 3))
+    energy = np.array([-12.5, -11.25, 3.0e-5], np.float32)
+    names = ["C1", "N2", "O3", "CL4"]
+    elements = ["C", "N", "O", "Cl"]
+    epath = tmp_path / "energy.txt"
+    epath.write_text("file,conformer,rank,energy\n")
+    cdk.write_poses(tmp_path, 7, poses, energy, names, elements, str(epath))
+    rows = epath.read_text().splitlines()
+    assert rows == ["file,conformer,rank,energy", "conformer_7_0.pdb,7,0,-12.5", "conformer_7_1.pdb,7,1,-11.25",
+                    "conformer_7_2.pdb,7,2,3e-05"]
+    for i in range(3):
+        lines = (tmp_path / ("conformer_7_%d.pdb" % i)).read_text().splitlines()
+        atoms = [l for l in lines if l.startswith("HETATM")]
+        assert len(atoms) == 4 and lines[-1] == "END"
+        for a, l in enumerate(atoms):
+            assert len(l) == 78
+            assert int(l[6:11]) == a + 1 and l[12:16].strip() == names[a] and l[17:20] == "LIG"
+            xyz = [float(l[30:38]), float(l[38:46]), float(l[46:54])]
+            assert np.allclose(xyz, poses[i, a], atol=5e-4)
+            assert l[76:78].strip() == elements[a]

@@ -143,3 +143,17 @@ def test_refine_is_rigid_and_downhill(ctx, p48):
     er, _, ins = cdk.rescore_poses(prot, lig, x1)
     inside = ins == lig.natom
     assert np.allclose(er[inside], e1[inside], rtol=0, atol=1e-9)
+
+
+def test_grid_for_openmm_order_and_bounds(ctx, p48):
+    """CHARMM z-fastest grids to OpenMM's x-fastest order
+    (AddGridForcesToOpenMMSystem.cpp:30-41) and the grid bounds in nm
+    (TranRotaConfSearchGrid.cu:149-154)."""
+    say("grid for openmm")
+    g, prot = p48
+    for c in (0, prot.num_grids - 1):
+        v, b = cdk.grid_for_openmm(prot, c)
+        assert np.array_equal(v, np.transpose(g["values"][c], (2, 1, 0)).astype(np.float64))
+        gmin = prot.grid_min()
+        gmax = gmin + np.float64(np.float32(g["n"] - 1) * np.float32(g["spacing"]))
+        assert np.array_equal(b, np.concatenate([gmin * 0.1, gmax * 0.1]))
