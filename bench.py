@@ -189,9 +189,9 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
             "algorithmic_bytes_per_launch": round(dom_bytes)}
-    kernels["tuned"] = {"fwd_variant": ["block", "wavefront"][ctx.get_option("last_fwd_variant")],
+    kernels["tuned"] = {"fwd_variant": ["block", "register"][ctx.get_option("last_fwd_variant")],
                         "inv_variant": ["block", "register"][ctx.get_option("last_inv_variant")],
-                        "mid_group": group}
+                        "mid_group": group, "max_dyn_lds": ctx.get_option("max_dyn_lds")}
     return kernels, roof
 
 
@@ -458,7 +458,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
     ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
-    ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 wavefront, -1 autotuned")
+    ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--inv", type=int, default=-1, help="inverse plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="Poisson rate as a fraction of capacity")

@@ -23,6 +23,7 @@ from .api import (  # noqa: F401
     synth_conformers,
     synth_ligand,
     synth_protein,
+    synth_protein_atoms,
     write_grids,
     write_poses,
 )

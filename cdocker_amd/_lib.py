@@ -67,6 +67,9 @@ _SIGS = {
     "cdk_context_get_option": [P, CP, I64P],
     "cdk_context_stage_times": [P, DP, IP, I],
     "cdk_protein_create": [P, I, I, I, I, FP, FP, F, FP, FP, PP],
+    "cdk_protein_build": [P, I, DP, DP, DP, DP, I, FP, I, F, FP, FP, C.c_double, C.c_double, C.c_double,
+                          C.c_double, PP],
+    "cdk_protein_grids": [P, FP],
     "cdk_protein_destroy": [P],
     "cdk_protein_spectrum": [P, I, FP],
     "cdk_protein_bytes": [P, I64P],
@@ -104,6 +107,7 @@ _SIGS = {
     "cdk_quaternion_uniform_sampling": [P, I, FP],
     "cdk_filter_quaternions": [P, FP, I, I, FP, F, F, F, I, U64, IP, FP],
     "cdk_idx_of_atoms_for_vdw_radius": [I, FP, I, FP, IP, IP, IP],
+    "cdk_synth_protein_atoms": [U64, I, F, DP, DP, DP, DP],
     "cdk_synth_protein": [U64, I, F, I, F, FP, FP, FP, FP],
     "cdk_synth_ligand": [U64, I, FP, FP, FP, FP],
     "cdk_synth_conformers": [U64, I, FP, I, FP],

@@ -195,6 +195,33 @@ class Protein:
         self.num_grids, self.n, self.spacing = ng, n, float(spacing)
         self.mid, self.length, self.radii = mid, length, r
 
+    @classmethod
+    def build(cls, ctx, xyz, charge, epsilon, radius, probe_radii, n, spacing, mid, length, lj_cutoff=8.0,
+              elec_cutoff=12.0, lj_cap=3.0, elec_cap=20.0):
+        """Potential grids built on the device from atoms (cdk_protein_build)."""
+        x = np.ascontiguousarray(np.asarray(xyz, np.float64).reshape(-1, 3))
+        q = np.ascontiguousarray(np.asarray(charge, np.float64))
+        e = np.ascontiguousarray(np.asarray(epsilon, np.float64))
+        r = np.ascontiguousarray(np.asarray(radius, np.float64))
+        pr = _f32(probe_radii)
+        mid = _f32(mid)
+        length = _f32(length)
+        h = C.c_void_p()
+        check(lib.cdk_protein_build(ctx.h, x.shape[0], _dp(x), _dp(q), _dp(e), _dp(r), pr.shape[0], _fp(pr), int(n),
+                                    float(spacing), _fp(mid), _fp(length), float(lj_cutoff), float(elec_cutoff),
+                                    float(lj_cap), float(elec_cap), C.byref(h)), "cdk_protein_build")
+        self = cls.__new__(cls)
+        self.h, self.ctx = h, ctx
+        self.num_grids, self.n, self.spacing = pr.shape[0] + 1, int(n), float(spacing)
+        self.mid, self.length, self.radii = mid, length, pr
+        return self
+
+    def grids(self):
+        n = self.n
+        out = np.empty((self.num_grids, n, n, n), np.float32)
+        check(lib.cdk_protein_grids(self.h, _fp(out)), "cdk_protein_grids")
+        return out
+
     def spectrum(self, channel):
         n = self.n
         out = np.empty((n, n, n // 2 + 1), np.complex64)
@@ -488,6 +515,17 @@ def sort_index(values):
     """Energy order with ties by position (stable restatement of sort_index,
     GeneDiverseConformations.h:9-20)."""
     return np.argsort(np.asarray(values), kind="stable")
+
+
+def synth_protein_atoms(seed, natom, box):
+    """(xyz, charge, epsilon, radius) of the synthetic protein, float64."""
+    x = np.empty((natom, 3), np.float64)
+    q = np.empty(natom, np.float64)
+    e = np.empty(natom, np.float64)
+    r = np.empty(natom, np.float64)
+    check(lib.cdk_synth_protein_atoms(C.c_uint64(seed), int(natom), float(box), _dp(x), _dp(q), _dp(e), _dp(r)),
+          "cdk_synth_protein_atoms")
+    return x, q, e, r
 
 
 def synth_protein(seed, natom, box, n, spacing):

@@ -156,7 +156,7 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
       require(v >= 0 && v <= 4096, "mid_group out of range");
       c.mid_group = (int)v;
     } else if (k == "fwd_variant") {
-      require(v >= -1 && v <= 1, "fwd_variant must be -1 (tuned), 0 (block) or 1 (wavefront)");
+      require(v >= -1 && v <= 1, "fwd_variant must be -1 (tuned), 0 (block) or 1 (register)");
       c.fwd_variant = (int)v;
     } else if (k == "inv_variant") {
       require(v >= -1 && v <= 1, "inv_variant must be -1 (tuned), 0 (block) or 1 (register)");
@@ -223,6 +223,40 @@ int cdk_protein_create(cdk_context* ctx, int num_grids, int xdim, int ydim, int 
       throw;
     }
     *out = p;
+  });
+}
+
+int cdk_protein_build(cdk_context* ctx, int natom, const double* xyz, const double* charge, const double* epsilon,
+                      const double* radius, int num_probes, const float* probe_radii, int n, float spacing,
+                      const float* mid3, const float* len3, double lj_cutoff, double elec_cutoff, double lj_cap,
+                      double elec_cap, cdk_protein** out) {
+  return guard([&] {
+    require(ctx && probe_radii && out, "NULL argument");
+    require(natom == 0 || (xyz && charge && epsilon && radius), "NULL atom arrays");
+    require(lj_cutoff > 0 && elec_cutoff > 0 && lj_cap > 0 && elec_cap > 0, "cutoffs and caps must be positive");
+    auto* p = new cdk_protein();
+    p->p.ctx = ctx->c.get();
+    try {
+      std::lock_guard<std::mutex> lk(ctx->c->mu);
+      cdk::hip_check(hipSetDevice(ctx->c->device), "hipSetDevice");
+      cdk::protein_build(p->p, natom, xyz, charge, epsilon, radius, num_probes, probe_radii, n, spacing, mid3, len3,
+                         lj_cutoff, elec_cutoff, lj_cap, elec_cap);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+int cdk_protein_grids(cdk_protein* p, float* values) {
+  return guard([&] {
+    require(p && values, "NULL argument");
+    cdk::Context& ctx = *p->p.ctx;
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    cdk::hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+    cdk::hip_check(hipMemcpy(values, p->p.grid.p, (size_t)p->p.nch * p->p.G() * sizeof(float), hipMemcpyDeviceToHost),
+                   "grids D2H");
   });
 }
 
@@ -754,6 +788,15 @@ int cdk_idx_of_atoms_for_vdw_radius(int natom, const float* atom_radii, int num_
     for (int j = 0; j < num_vdw_grids; ++j)
       if (count[j] > 0) idx_vdw_used[u++] = j;
     *num_vdw_used = u;
+  });
+}
+
+int cdk_synth_protein_atoms(uint64_t seed, int natom, float box, double* xyz, double* charge, double* epsilon,
+                            double* radius) {
+  return guard([&] {
+    require(xyz && charge && epsilon && radius, "NULL argument");
+    require(natom > 0, "bad atom count");
+    cdk::synth_protein_atoms(seed, natom, box, xyz, charge, epsilon, radius);
   });
 }
 

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <limits>
 
+#include "gridbuild.hpp"
 #include "kernels.hpp"
 #include "rescore.hpp"
 
@@ -44,7 +45,7 @@ Context::Context(int dev) : device(dev) {
   int lds = 0;
   hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
   max_dyn_lds = (size_t)lds;
-  hip_check(configure_fwd_wave(max_dyn_lds), "configure forward kernels");
+  hip_check(configure_fwd_reg(max_dyn_lds), "configure forward kernels");
 }
 
 Context::~Context() {
@@ -82,12 +83,19 @@ void Context::ensure_host(size_t nslots) {
   h_cap = nslots;
 }
 
-void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const float* len, const float* radii,
-                    const float* values) {
+namespace {
+
+template <class T>
+void upload(Context& ctx, DevBuf& b, const std::vector<T>& v) {
+  b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (!v.empty())
+    hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx.stream), "upload");
+}
+
+void protein_setup(Protein& p, int nch, int N, float h, const float* mid, const float* len, const float* radii) {
   if (nch < 2 || nch > kMaxCh) throw Error(kErrInvalid, "number of grids must be in [2, 16]");
   if (!grid_edge_supported(N)) throw Error(kErrUnsupported, "grid edge must be one of 32, 48, 64, 96, 128");
   if (!(h > 0.f)) throw Error(kErrInvalid, "grid spacing must be positive");
-  Context& ctx = *p.ctx;
   p.nch = nch;
   p.N = N;
   p.h = h;
@@ -96,17 +104,57 @@ void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const
     p.len[k] = len ? len[k] : (float)(N - 1) * h;
   }
   p.radii.assign(radii, radii + (nch - 1));
-  const size_t G = p.G(), H = p.H();
-  p.grid.ensure((size_t)nch * G * sizeof(float));
-  p.spec.ensure((size_t)nch * H * sizeof(float2));
-  hip_check(hipMemcpyAsync(p.grid.p, values, (size_t)nch * G * sizeof(float), hipMemcpyHostToDevice, ctx.stream),
-            "protein upload");
-  hip_check(launch_r2c_3d(ctx.stream, N, p.grid.as<float>(), p.spec.as<float2>(), nch, ctx.twiddle(N)),
+  p.grid.ensure((size_t)nch * p.G() * sizeof(float));
+}
+
+// Forward R2C of every channel (NoMini:177-199) plus the line-blocked copy
+// the fused mid stage reads.
+void protein_spectra(Protein& p) {
+  Context& ctx = *p.ctx;
+  const size_t H = p.H();
+  p.spec.ensure((size_t)p.nch * H * sizeof(float2));
+  hip_check(launch_r2c_3d(ctx.stream, p.N, p.grid.as<float>(), p.spec.as<float2>(), p.nch, ctx.twiddle(p.N)),
             "protein R2C");
-  p.spec_lb.ensure((size_t)nch * H * sizeof(float2));
-  hip_check(launch_to_line_blocks(ctx.stream, N, p.spec.as<float2>(), p.spec_lb.as<float2>(), nch),
+  p.spec_lb.ensure((size_t)p.nch * H * sizeof(float2));
+  hip_check(launch_to_line_blocks(ctx.stream, p.N, p.spec.as<float2>(), p.spec_lb.as<float2>(), p.nch),
             "protein spectra line blocks");
   hip_check(hipStreamSynchronize(ctx.stream), "protein R2C sync");
+}
+
+}  // namespace
+
+void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const float* len, const float* radii,
+                    const float* values) {
+  protein_setup(p, nch, N, h, mid, len, radii);
+  Context& ctx = *p.ctx;
+  hip_check(hipMemcpyAsync(p.grid.p, values, (size_t)nch * p.G() * sizeof(float), hipMemcpyHostToDevice, ctx.stream),
+            "protein upload");
+  protein_spectra(p);
+}
+
+void protein_build(Protein& p, int natom, const double* xyz, const double* charge, const double* epsilon,
+                   const double* radius, int nprobe, const float* probe_radii, int N, float h, const float* mid,
+                   const float* len, double lj_cutoff, double elec_cutoff, double lj_cap, double elec_cap) {
+  if (natom < 0 || nprobe < 1 || nprobe > grid_build_max_probes())
+    throw Error(kErrInvalid, "atom count must be >= 0 and probe count in [1, 15]");
+  protein_setup(p, nprobe + 1, N, h, mid, len, probe_radii);
+  Context& ctx = *p.ctx;
+  std::vector<double> atoms((size_t)std::max(natom, 1) * 6);
+  for (int i = 0; i < natom; ++i) {
+    for (int k = 0; k < 3; ++k) atoms[(size_t)i * 6 + k] = xyz[i * 3 + k];
+    atoms[(size_t)i * 6 + 3] = charge[i];
+    atoms[(size_t)i * 6 + 4] = radius[i];
+    atoms[(size_t)i * 6 + 5] = epsilon[i];
+  }
+  std::vector<double> probe(probe_radii, probe_radii + nprobe);
+  DevBuf d_atoms, d_probe;
+  upload(ctx, d_atoms, atoms);
+  upload(ctx, d_probe, probe);
+  const double gmin[3] = {p.grid_min(0), p.grid_min(1), p.grid_min(2)};
+  hip_check(launch_build_grids(ctx.stream, natom, d_atoms.as<double>(), nprobe, d_probe.as<double>(), N, gmin,
+                               (double)h, lj_cutoff, elec_cutoff, lj_cap, elec_cap, p.grid.as<float>()),
+            "build grids");
+  protein_spectra(p);
 }
 
 std::unique_ptr<Ligand> make_ligand(const Protein& p, int natom, const float* charge, const float* eps,
@@ -288,12 +336,6 @@ void build_tables(const Protein& prot, const std::vector<Job>& jobs, Prepared& t
   t.cmax = std::max(t.cmax, 1);
 }
 
-template <class T>
-void upload(Context& ctx, DevBuf& b, const std::vector<T>& v) {
-  b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
-  if (!v.empty())
-    hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx.stream), "upload");
-}
 
 struct EventTimer {
   Context& ctx;
@@ -430,7 +472,7 @@ struct Tuned {
 Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   const int N = prot.N;
   Tuned tu;
-  const bool wave_ok = fwd_wave_supported(N);
+  const bool fwd_reg_ok = fwd_reg_supported(N);
   const bool reg_ok = inv_reg_supported(N);
   const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)chunk};
   bool need_fwd = false, need_inv = false, need_group = false;
@@ -446,12 +488,12 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       ctx.tuned_inv[N] = tu.inv = kInvBlock;
   }
   if (ctx.fwd_variant >= 0) {
-    tu.fwd = (ctx.fwd_variant == kFwdWave && wave_ok) ? kFwdWave : kFwdBlock;
+    tu.fwd = (ctx.fwd_variant == kFwdReg && fwd_reg_ok) ? kFwdReg : kFwdBlock;
   } else {
     auto it = ctx.tuned_fwd.find(N);
     if (it != ctx.tuned_fwd.end())
       tu.fwd = it->second;
-    else if (wave_ok)
+    else if (fwd_reg_ok)
       need_fwd = true;
     else
       ctx.tuned_fwd[N] = tu.fwd = kFwdBlock;
@@ -500,8 +542,8 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   try {
     if (need_fwd) {
       const float tb = timed([&] { fwd(kFwdBlock); });
-      const float twv = timed([&] { fwd(kFwdWave); });
-      tu.fwd = twv < tb ? kFwdWave : kFwdBlock;
+      const float twv = timed([&] { fwd(kFwdReg); });
+      tu.fwd = twv < tb ? kFwdReg : kFwdBlock;
       ctx.tuned_fwd[N] = tu.fwd;
     } else {
       fwd(tu.fwd);

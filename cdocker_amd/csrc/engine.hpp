@@ -190,6 +190,11 @@ struct Context {
 
 void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const float* len, const float* radii,
                     const float* values);
+// Builds the potential grids on the device from atoms (gridbuild.hip), then
+// the spectra as protein_upload does.
+void protein_build(Protein& p, int natom, const double* xyz, const double* charge, const double* epsilon,
+                   const double* radius, int nprobe, const float* probe_radii, int N, float h, const float* mid,
+                   const float* len, double lj_cutoff, double elec_cutoff, double lj_cap, double elec_cap);
 
 // Builds and uploads the tables of a set of jobs (inputs resident in HBM).
 void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Prepared& t);

@@ -82,17 +82,29 @@ void quaternion_uniform_sampling(std::mt19937_64& gen, int n, float* out) {
 // The ligand energy is grid * sqrt(eps_ligand) (vdW) and grid * q (elec),
 // the units the reference's grid forces use (AddGridForcesToOpenMMSystem.cpp:54,100).
 // ---------------------------------------------------------------------------
+void synth_protein_atoms(uint64_t seed, int natom, float box, double* xyz, double* charge, double* epsilon,
+                         double* radius) {
+  Xoshiro rng(seed);
+  for (int i = 0; i < natom; ++i) {
+    xyz[i * 3 + 0] = rng.uniform() * box;
+    xyz[i * 3 + 1] = rng.uniform() * box;
+    xyz[i * 3 + 2] = rng.uniform() * box;
+    charge[i] = 0.3 * rng.normal();
+    radius[i] = kProbeRadii[(int)(rng.uniform() * 8) & 7];
+    epsilon[i] = 0.05 + 0.20 * rng.uniform();
+  }
+}
+
 void synth_protein(uint64_t seed, int natom, float box, int n, float h, float* values, float* radii, float* mid3,
                    float* len3) {
-  Xoshiro rng(seed);
-  std::vector<double> px(natom), py(natom), pz(natom), q(natom), r(natom), se(natom);
+  std::vector<double> xyz((size_t)natom * 3), q(natom), r(natom), eps(natom);
+  synth_protein_atoms(seed, natom, box, xyz.data(), q.data(), eps.data(), r.data());
+  std::vector<double> px(natom), py(natom), pz(natom), se(natom);
   for (int i = 0; i < natom; ++i) {
-    px[i] = rng.uniform() * box;
-    py[i] = rng.uniform() * box;
-    pz[i] = rng.uniform() * box;
-    q[i] = 0.3 * rng.normal();
-    r[i] = kProbeRadii[(int)(rng.uniform() * 8) & 7];
-    se[i] = std::sqrt(0.05 + 0.20 * rng.uniform());
+    px[i] = xyz[i * 3 + 0];
+    py[i] = xyz[i * 3 + 1];
+    pz[i] = xyz[i * 3 + 2];
+    se[i] = std::sqrt(eps[i]);
   }
   for (int k = 0; k < 8; ++k) radii[k] = kProbeRadii[k];
   const float len = (float)(n - 1) * h;

@@ -18,6 +18,10 @@ struct Xoshiro {
 extern const float kProbeRadii[8];
 
 void quaternion_uniform_sampling(std::mt19937_64& gen, int n, float* out);
+// The atoms behind synth_protein: positions uniform in [0, box)^3, charge
+// N(0, 0.3), radius from the probe set, epsilon uniform [0.05, 0.25].
+void synth_protein_atoms(uint64_t seed, int natom, float box, double* xyz, double* charge, double* epsilon,
+                         double* radius);
 void synth_protein(uint64_t seed, int natom, float box, int n, float h, float* values, float* radii, float* mid3,
                    float* len3);
 void synth_ligand(uint64_t seed, int natom, float* coords, float* charge, float* eps, float* radius);

@@ -480,33 +480,22 @@ struct PlaneAtom {
   double wx, fy, fz;
 };
 
-// Line-blocked layout of the fused engine's spectra (ligand planes A,
-// correlation planes T, and the protein copy the mid stage reads): lines of
-// the (ky, kz) plane in blocks of B = 64/L consecutive lines, the rows x of
-// one block contiguous.  Element (x, line) of an R-row volume sits at
+// Line-blocked layout of the protein spectra copy the mid stage reads: lines
+// of the (ky, kz) plane in blocks of B = 64/L consecutive lines, the rows x
+// of one block contiguous.  Element (x, line) of an R-row volume sits at
 // ((line / B) * R + x) * B + line % B, so the 64/L lines a mid-stage
 // wavefront owns are 8 x 64/L contiguous bytes per row and its 8 rows of one
-// load are 512 contiguous bytes.  N * (N/2 + 1) is a multiple of B for every
-// supported N.
-#ifndef CDK_A_LB
-#define CDK_A_LB 1
-#endif
-#ifndef CDK_T_LB
-#define CDK_T_LB 1
-#endif
+// load are 512 contiguous bytes (whole cache lines; the mid stage reads the
+// protein spectra once per (pose, channel), twice the ligand's bytes).  The
+// ligand planes A and correlation planes T stay plane-major [x][line]: their
+// producers and consumers walk planes.  N * (N/2 + 1) is a multiple of B for
+// every supported N.
 template <int N>
 __device__ __forceinline__ int lb_index(int R, int x, int line) {
   constexpr int B = RegShape<N>::LPW;  // a power of two
   return ((line / B) * R + x) * B + (line & (B - 1));  // < 2^31 per (slot, channel)
 }
-template <int N>
-__device__ __forceinline__ int a_index(int R, int x, int line) {
-  return CDK_A_LB ? lb_index<N>(R, x, line) : x * (N * (N / 2 + 1)) + line;
-}
-template <int N>
-__device__ __forceinline__ int t_index(int x, int line) {
-  return CDK_T_LB ? lb_index<N>(N, x, line) : x * (N * (N / 2 + 1)) + line;
-}
+
 
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_to_line_blocks(const float2* __restrict__ in, float2* __restrict__ out,
@@ -600,45 +589,46 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes(const PoseSlot* __restric
     unpack_rows<N>(r, u, 2 * npair);
     __syncthreads();
     const float2* res = lds_fft<P, -1, true, ZH>(u, r, ZH, ZH, 1, tw);
-    for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op0[a_index<N>(boxmax, x, t)] = res[t];
+    for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) op0[(size_t)x * (N * ZH) + t] = res[t];
   }
 }
 
-// Stage A, wavefront form: one workgroup per (pose, ligand channel) stages
-// the channel's atoms once; then its four wavefronts take the occupied
-// x-planes in turn and run with no block barriers.  Per plane a wavefront
-//   1. zeroes the packed rows (row pair p = rows 2p, 2p+1 in .x, .y) and
-//      spreads the plane's cells into them in the reference's atom order,
-//   2. z-transforms the row pairs (register four-step FFT, 64/L lines at a
-//      time, first stage pruned when the box fits in N/2) and writes each
-//      packed spectrum back over its row,
-//   3. y-transforms the half-spectrum columns, unpacking the two real rows
-//      of every pair as it loads them; columns 0 and N/2 are real along y
-//      and share one complex line,
-//   4. writes the plane spectrum A[slot][c][x][Y][ZH].
-// Rows have a padded stride (RSTR = N + pad, RSTR % 32 == L in float2
-// units) so that row loads, spectrum write-back and the column loads each
-// touch every 8-byte LDS bank slot evenly.
+// Stage A, register form: one workgroup per (pose, ligand channel) stages
+// the channel's atoms once, then per occupied x-plane
+//   1. zeroes the packed row pairs of an LDS tile (pair p = rows 2p, 2p+1 in
+//      .x, .y) and spreads the plane's cells into them in the reference's
+//      atom order (one wavefront, atoms selected by ballot),
+//   2. z-transforms the row pairs with register four-step FFTs (the four
+//      wavefronts take 64/L pairs each; first stage pruned when the box fits
+//      in N/2) and writes each packed spectrum back over its row,
+//   3. y-transforms the half-spectrum columns, unpacking the two real rows of
+//      every pair as it loads them (columns 0 and N/2 are real along y and
+//      share one complex line), and writes the plane spectrum to A.
+// Four block barriers per plane; the tile holds only the box's row pairs
+// (dynamic LDS), so every box size fits.  Tile rows have stride
+// RSTR = N + pad with RSTR % 32 == 4 (float2 units): the z-stage row loads of
+// 32 consecutive lanes then touch 32 distinct 8-byte bank slots.
 template <int N>
-struct FwdWaveShape {
-  static constexpr int L = RegShape<N>::L, V = RegShape<N>::V, LPW = RegShape<N>::LPW;
-  static constexpr int RSTR = N + ((L - N % 32) + 32) % 32;
+struct FwdRegShape {
+  static constexpr int RSTR = N + ((4 - N % 32) + 32) % 32;
 };
 
 template <class P>
-__global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __restrict__ slots,
-                                                            const LigandDesc* __restrict__ ligs,
-                                                            const int* __restrict__ chan_atoms,
-                                                            const float* __restrict__ wv,
-                                                            const float* __restrict__ wq,
-                                                            const AtomGrid* __restrict__ ag, int astride,
-                                                            const RotMeta* __restrict__ meta, int cmax, int boxmax,
-                                                            int natom_lds, int npmax, float2* __restrict__ A,
-                                                            const float2* __restrict__ twg) {
+__global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __restrict__ slots,
+                                                           const LigandDesc* __restrict__ ligs,
+                                                           const int* __restrict__ chan_atoms,
+                                                           const float* __restrict__ wv, const float* __restrict__ wq,
+                                                           const AtomGrid* __restrict__ ag, int astride,
+                                                           const RotMeta* __restrict__ meta, int cmax, int boxmax,
+                                                           int natom_lds, int npmax, float2* __restrict__ A,
+                                                           const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
-  using FS = FwdWaveShape<N>;
-  constexpr int L = FS::L, V = FS::V, LPW = FS::LPW, RSTR = FS::RSTR;
-  extern __shared__ double dyn[];
+  using RS_ = RegShape<N>;
+  constexpr int L = RS_::L, V = RS_::V, LPW = RS_::LPW, RSTR = FwdRegShape<N>::RSTR;
+  constexpr bool TW_LDS = V >= 16;  // long lines: twiddles from LDS, registers to the data
+  __shared__ float2 sbuf[4][RS_::SLICE];
+  __shared__ float2 stw[TW_LDS ? L * V : 1];
+  extern __shared__ double dyn[];  // [natom_lds] SpreadAtom, then the tile [npmax][RSTR]
   const int c = blockIdx.x, s = blockIdx.y;
   const RotMeta m = meta[s];
   if (!m.valid) return;
@@ -647,6 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
   if (c >= Ld.nch) return;
   const int nc = Ld.ch_start[c + 1] - Ld.ch_start[c];
   SpreadAtom* sa = reinterpret_cast<SpreadAtom*>(dyn);
+  float2* zt = reinterpret_cast<float2*>(sa + natom_lds);
   {
     const int* ca = chan_atoms + Ld.list_off + Ld.ch_start[c];
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
@@ -663,72 +654,73 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
       sa[i] = t;
     }
   }
-  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int ln = lane % LPW, n2 = lane / LPW;
-  constexpr int SLICE = RegShape<N>::SLICE;
-  float2* tr = reinterpret_cast<float2*>(sa + natom_lds) + wave * SLICE;
-  float2* rows = reinterpret_cast<float2*>(sa + natom_lds) + 4 * SLICE + wave * (npmax * RSTR);
-  float2 twl[V];
-  reg_twiddles<N>(twg, n2, twl);
+  float2* tr = sbuf[wave];
+  float2 twr[TW_LDS ? 1 : V];
+  if constexpr (TW_LDS) {
+    for (int t = threadIdx.x; t < L * V; t += blockDim.x) stw[t] = twg[(t / V) * (t % V)];
+  } else {
+    reg_twiddles<N>(twg, n2, twr);
+  }
+  const float2* twl = TW_LDS ? stw + n2 * V : twr;
   const int bxm = min(m.box[0], boxmax), by = m.box[1], bz = m.box[2];
   const int npair = (by + 1) >> 1;
   const bool zhalf = bz <= N / 2, yhalf = by <= N / 2;
   const int zlim = zhalf ? N / 2 : N;
   float2* op0 = A + (((size_t)s * cmax + c) * boxmax) * (size_t)(N * ZH);
   const int dy = (lane >> 1) & 1, dz = lane & 1;
-  const int ix0 = lane < nc ? sa[lane].ix : -4;
-  for (int x = wave; x < bxm; x += 4) {
-    // 1. zero, then the ordered scatter: atoms touching plane x in ascending
-    // order, the 4 corners of one atom in lanes 0-3 (distinct cells), so each
-    // cell accumulates in the reference's order (FillLigandGrid.cpp:45-66,
-    // 97-117).
-    for (int p = 0; p < npair; ++p)
-      for (int z = lane; z < zlim; z += 64) rows[p * RSTR + z] = make_float2(0.f, 0.f);
-    WaveSync::sync();
-    float* cells = reinterpret_cast<float*>(rows);
-    for (int base = 0; base < nc; base += 64) {
-      // Atoms touching plane x, as a mask over this chunk of 64; set bits
-      // are taken lowest first, which keeps ascending atom order.
-      const int aix = base == 0 ? ix0 : (base + lane < nc ? sa[base + lane].ix : -4);
-      unsigned long long mk = __ballot(x - aix == 0 || x - aix == 1);
-      while (mk) {
-        const int i = base + __ffsll((long long)mk) - 1;
-        mk &= mk - 1;
-        const int ddx = x - sa[i].ix;
-        if (lane < 4) {
-          const int y = sa[i].iy + dy, z = sa[i].iz + dz;
-          float* cell = cells + (((y >> 1) * RSTR + z) << 1) + (y & 1);
-          *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, ddx), sa[i].fy, sa[i].fz, dy, dz);
+  for (int x = 0; x < bxm; ++x) {
+    __syncthreads();  // atoms staged; the previous plane's column loads are done
+    for (int t = threadIdx.x; t < npair * zlim; t += blockDim.x) {
+      const int p = t / zlim;
+      zt[p * RSTR + (t - p * zlim)] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    // 1. ordered scatter by wavefront 0 (FillLigandGrid.cpp:45-66, 97-117).
+    if (wave == 0) {
+      float* cells = reinterpret_cast<float*>(zt);
+      for (int base = 0; base < nc; base += 64) {
+        const int aix = base + lane < nc ? sa[base + lane].ix : -4;
+        unsigned long long mk = __ballot(x - aix == 0 || x - aix == 1);
+        while (mk) {
+          const int i = base + __ffsll((long long)mk) - 1;
+          mk &= mk - 1;
+          const int ddx = x - sa[i].ix;
+          if (lane < 4) {
+            const int y = sa[i].iy + dy, z = sa[i].iz + dz;
+            float* cell = cells + (((y >> 1) * RSTR + z) << 1) + (y & 1);
+            *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, ddx), sa[i].fy, sa[i].fz, dy, dz);
+          }
+          WaveSync::sync();
         }
-        WaveSync::sync();
       }
     }
+    __syncthreads();
     // 2. z transforms of the row pairs, written back in place.
-    for (int p0 = 0; p0 < npair; p0 += LPW) {
+    for (int p0 = wave * LPW; p0 < npair; p0 += 4 * LPW) {
       const int p = p0 + ln;
       const bool ok = p < npair;
       float2 v[V];
       if (zhalf) {
 #pragma unroll
-        for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = ok ? rows[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
+        for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
         reg_fft<N, -1, true>(v, twl, tr, ln, n2);
       } else {
 #pragma unroll
-        for (int n1 = 0; n1 < V; ++n1) v[n1] = ok ? rows[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
+        for (int n1 = 0; n1 < V; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
         reg_fft<N, -1, false>(v, twl, tr, ln, n2);
       }
-      WaveSync::sync();
       if (ok) {
 #pragma unroll
-        for (int q = 0; q < V; ++q) rows[p * RSTR + n2 + L * q] = v[q];
+        for (int q = 0; q < V; ++q) zt[p * RSTR + n2 + L * q] = v[q];
       }
     }
-    WaveSync::sync();
-    // 3. y transforms of the columns; line j = column j, except line 0 =
-    // columns 0 (real part) and N/2 (imaginary part).
-    for (int j0 = 0; j0 < N / 2; j0 += LPW) {
+    __syncthreads();
+    // 3. y transforms; line j = column j, except line 0 = columns 0 (real
+    // part) and N/2 (imaginary part).
+    for (int j0 = wave * LPW; j0 < N / 2; j0 += 4 * LPW) {
       const int j = j0 + ln;
       const bool jok = j < N / 2;
       const int ia = j == 0 ? 0 : j, ib = j == 0 ? N / 2 : N - j;
@@ -737,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
         const int p = y >> 1;
         float2 r = make_float2(0.f, 0.f);
         if (p < npair && jok) {
-          const float2 zk = rows[p * RSTR + ia], zm = rows[p * RSTR + ib];
+          const float2 zk = zt[p * RSTR + ia], zm = zt[p * RSTR + ib];
           if (j == 0)
             r = (y & 1) ? make_float2(zk.y, zm.y) : make_float2(zk.x, zm.x);
           else
@@ -770,17 +762,16 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_wave(const PoseSlot* __re
           for (int q = 0; q < V; ++q) {
             const int ky = n2 + L * q;
             const float2 u = v[q], w = tr[ky == 0 ? 0 : N - ky];
-            op0[a_index<N>(boxmax, x, ky * ZH)] = make_float2(0.5f * (u.x + w.x), 0.5f * (u.y - w.y));
-            op0[a_index<N>(boxmax, x, ky * ZH + N / 2)] = make_float2(0.5f * (u.y + w.y), -0.5f * (u.x - w.x));
+            op0[(size_t)x * (N * ZH) + ky * ZH] = make_float2(0.5f * (u.x + w.x), 0.5f * (u.y - w.y));
+            op0[(size_t)x * (N * ZH) + ky * ZH + N / 2] = make_float2(0.5f * (u.y + w.y), -0.5f * (u.x - w.x));
           }
         }
       }
       if (j > 0 && jok) {
 #pragma unroll
-        for (int q = 0; q < V; ++q) op0[a_index<N>(boxmax, x, (n2 + L * q) * ZH + j)] = v[q];
+        for (int q = 0; q < V; ++q) op0[(size_t)x * (N * ZH) + (n2 + L * q) * ZH + j] = v[q];
       }
     }
-    WaveSync::sync();
   }
 }
 
@@ -844,16 +835,16 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   const float2* twl = TW_LDS ? stw + n2 * V : twr;
   float2* lds = sbuf[wave];
   const int s1 = min(nslots, sg * group + group);
-  // Line blocks (lb_index): the wavefront's lines are block l0 / LPW; lane
-  // offset of row x = n2 within it, row n1 adds L*n1 rows.  Rows past the
-  // descriptor's end (x >= box, x' >= window) read as zero / are dropped.
-  const size_t blk = (size_t)(l0 / LPW) * LPW;  // block start, in rows of LPW lines
-  const int voff0 = lok ? (n2 * LPW + ln) * (int)sizeof(float2) : kVoffNone;
-  constexpr int kRow = L * LPW * (int)sizeof(float2);
-  const int voffA = CDK_A_LB ? voff0 : (lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone);
-  constexpr int kRowA = CDK_A_LB ? kRow : L * NL * (int)sizeof(float2);
-  const int voffT = CDK_T_LB ? voff0 : (lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone);
-  constexpr int kRowT = CDK_T_LB ? kRow : L * NL * (int)sizeof(float2);
+  // Ligand planes A and correlation planes T are plane-major: lane offset of
+  // row x = n2 of the wavefront's lines, row n1 adds L*n1 planes.  The
+  // protein copy is line-blocked (lb_index): block l0 / LPW, rows of LPW
+  // lines.  Rows past a descriptor's end (x >= box, x' >= window) read as
+  // zero / are dropped.
+  const size_t blk = (size_t)(l0 / LPW) * LPW;  // protein block start, in rows of LPW lines
+  const int voffA = lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone;
+  constexpr int kRowA = L * NL * (int)sizeof(float2);
+  const int voffP = lok ? (n2 * LPW + ln) * (int)sizeof(float2) : kVoffNone;
+  constexpr int kRowP = L * LPW * (int)sizeof(float2);
   // Slot-derived values are wave-uniform; readfirstlane keeps them (and the
   // descriptors built from them) in scalar registers.
   auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
@@ -879,10 +870,7 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   };
   auto load = [&](const Item& it, float2* ra, float2* rp) {
     const __amdgpu_buffer_rsrc_t r =
-        CDK_A_LB ? buffer_rsrc(A + (((size_t)it.s * cmax + it.c) * boxmax) * plane + blk * boxmax,
-                               (unsigned)(it.bx * LPW * sizeof(float2)))
-                 : buffer_rsrc(A + (((size_t)it.s * cmax + it.c) * boxmax) * plane + l0,
-                               (unsigned)(it.bx * NL * sizeof(float2)));
+        buffer_rsrc(A + (((size_t)it.s * cmax + it.c) * boxmax) * plane + l0, (unsigned)(it.bx * NL * sizeof(float2)));
     const int pc = uni(ligs[it.lig].pmap[it.c]);
     const __amdgpu_buffer_rsrc_t q =
         buffer_rsrc(Phat + (size_t)pc * N * plane + blk * N, (unsigned)(N * LPW * sizeof(float2)));
@@ -894,7 +882,7 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
       for (int n1 = 0; n1 < V; ++n1) ra[n1] = buffer_load_f2(r, voffA + n1 * kRowA);
     }
 #pragma unroll
-    for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(q, voff0 + m * kRow);
+    for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(q, voffP + m * kRowP);
   };
   float2 acc[V];
 #pragma unroll
@@ -920,10 +908,9 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
       reg_fft<N, 1>(acc, twl, lds, ln, n2);
       const int wx = uni(meta[it.s].win[0]);
       const __amdgpu_buffer_rsrc_t rt =
-          CDK_T_LB ? buffer_rsrc(T + (size_t)it.s * N * plane + blk * N, (unsigned)(max(wx, 0) * LPW * sizeof(float2)))
-                   : buffer_rsrc(T + (size_t)it.s * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
+          buffer_rsrc(T + (size_t)it.s * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
 #pragma unroll
-      for (int m = 0; m < V; ++m) buffer_store_f2(acc[m], rt, voffT + m * kRowT);
+      for (int m = 0; m < V; ++m) buffer_store_f2(acc[m], rt, voffA + m * kRowA);
 #pragma unroll
       for (int i = 0; i < V; ++i) acc[i] = make_float2(0.f, 0.f);
     }
@@ -976,8 +963,8 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin(const RotMeta* __restrict
   const RotMeta m = meta[s];
   if (!m.valid || xp >= m.win[0]) return;
   load_twiddles<N>(tw, twg);
-  const float2* ip = T + (size_t)s * N * (size_t)(N * ZH);
-  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) sA[t] = ip[t_index<N>(xp, t)];
+  const float2* ip = T + ((size_t)s * N + xp) * (size_t)(N * ZH);
+  for (int t = threadIdx.x; t < N * ZH; t += blockDim.x) sA[t] = ip[t];
   __syncthreads();
   float2* r = lds_fft<P, 1, true, ZH>(sA, sB, ZH, ZH, 1, tw);
   float2* u = (r == sA) ? sB : sA;
@@ -1048,16 +1035,16 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin_reg(const RotMeta* __rest
   unsigned long long best = ~0ull;
   auto load_cols = [&](int xp, int pass, float2* v) {
     const int j = (pass * 4 + wave) * LPW + ln;
-    const float2* ip = T + (size_t)s * N * (size_t)(N * ZH);
+    const float2* ip = T + ((size_t)s * N + xp) * (size_t)(N * ZH);
 #pragma unroll
     for (int n1 = 0; n1 < V; ++n1) {
       const int ky = n2 + L * n1;
       float2 r = make_float2(0.f, 0.f);
       if (j == 0) {
-        const float2 a = ip[t_index<N>(xp, ky * ZH)], b = ip[t_index<N>(xp, ky * ZH + N / 2)];
+        const float2 a = ip[ky * ZH], b = ip[ky * ZH + N / 2];
         r = make_float2(a.x - b.y, a.y + b.x);
       } else if (j < NYL) {
-        r = ip[t_index<N>(xp, ky * ZH + j)];
+        r = ip[ky * ZH + j];
       }
       v[n1] = r;
     }
@@ -1290,40 +1277,61 @@ hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* met
   return hipGetLastError();
 }
 
-size_t fwd_wave_lds_bytes(int N, int boxmax, int max_chan_atoms) {
-  int SLICE = 0, RSTR = 0;
+size_t fwd_reg_lds_bytes(int N, int boxmax, int max_chan_atoms) {
+  int RSTR = 0;
   switch (N) {
-    case 32: SLICE = RegShape<32>::SLICE; RSTR = FwdWaveShape<32>::RSTR; break;
-    case 48: SLICE = RegShape<48>::SLICE; RSTR = FwdWaveShape<48>::RSTR; break;
-    case 64: SLICE = RegShape<64>::SLICE; RSTR = FwdWaveShape<64>::RSTR; break;
-    case 96: SLICE = RegShape<96>::SLICE; RSTR = FwdWaveShape<96>::RSTR; break;
-    case 128: SLICE = RegShape<128>::SLICE; RSTR = FwdWaveShape<128>::RSTR; break;
+    case 32: RSTR = FwdRegShape<32>::RSTR; break;
+    case 48: RSTR = FwdRegShape<48>::RSTR; break;
+    case 64: RSTR = FwdRegShape<64>::RSTR; break;
+    case 96: RSTR = FwdRegShape<96>::RSTR; break;
+    case 128: RSTR = FwdRegShape<128>::RSTR; break;
     default: return 0;
   }
   const int npmax = (boxmax + 1) / 2;
-  return (size_t)std::max(1, max_chan_atoms) * sizeof(SpreadAtom) + 4 * (size_t)SLICE * sizeof(float2) +
-         4 * (size_t)npmax * RSTR * sizeof(float2);
+  return (size_t)std::max(1, max_chan_atoms) * sizeof(SpreadAtom) + (size_t)npmax * RSTR * sizeof(float2);
+}
+
+template <int N>
+constexpr size_t fwd_reg_static() {
+  using R = RegShape<N>;
+  return (4 * (size_t)R::SLICE + (R::V >= 16 ? (size_t)R::L * R::V : 1)) * sizeof(float2);
 }
 
 namespace {
-size_t g_fwd_wave_max_lds = 64 * 1024;
+size_t g_max_dyn_lds = 64 * 1024;
 }
 
-hipError_t configure_fwd_wave(size_t max_dyn_lds) {
-  g_fwd_wave_max_lds = max_dyn_lds;
+hipError_t configure_fwd_reg(size_t max_dyn_lds) {
+  g_max_dyn_lds = max_dyn_lds;
   const int sizes[] = {32, 48, 64, 96, 128};
   for (int N : sizes) {
     hipError_t e = hipSuccess;
     CDK_DISPATCH_N(N, {
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_planes_wave<PL>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
+      // the dynamic part plus the static arrays must fit the device
+      const size_t stat = fwd_reg_static<PL::N>();
+      const size_t dyn = max_dyn_lds > stat ? max_dyn_lds - stat : 0;
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_planes_reg<PL>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     });
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
-bool fwd_wave_supported(int N) { return fwd_wave_lds_bytes(N, N, kMaxAtoms) <= g_fwd_wave_max_lds; }
+static size_t fwd_reg_static_lds(int N) {
+  switch (N) {
+    case 32: return fwd_reg_static<32>();
+    case 48: return fwd_reg_static<48>();
+    case 64: return fwd_reg_static<64>();
+    case 96: return fwd_reg_static<96>();
+    case 128: return fwd_reg_static<128>();
+    default: return 0;
+  }
+}
+
+bool fwd_reg_supported(int N) {
+  return fwd_reg_lds_bytes(N, N, kMaxAtoms) + fwd_reg_static_lds(N) <= g_max_dyn_lds;
+}
 
 hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot* slots, int nslots,
                              const LigandDesc* ligs, const int* chan_atoms, const float* wv, const float* wq,
@@ -1331,12 +1339,12 @@ hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot*
                              int max_chan_atoms, float2* A, const float2* tw) {
   if (nslots <= 0) return hipSuccess;
   const int natom = std::max(1, max_chan_atoms);
-  if (variant == kFwdWave) {
-    const size_t dyn = fwd_wave_lds_bytes(N, boxmax, natom);
-    if (dyn == 0 || dyn > g_fwd_wave_max_lds) return hipErrorInvalidValue;
+  if (variant == kFwdReg) {
+    const size_t dyn = fwd_reg_lds_bytes(N, boxmax, natom);
+    if (dyn == 0 || dyn + fwd_reg_static_lds(N) > g_max_dyn_lds) return hipErrorInvalidValue;
     const int npmax = (boxmax + 1) / 2;
     CDK_DISPATCH_N(N, {
-      hipLaunchKernelGGL(k_fwd_planes_wave<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms,
+      hipLaunchKernelGGL(k_fwd_planes_reg<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms,
                          wv, wq, ag, astride, meta, cmax, boxmax, natom, npmax, A, tw);
     });
     return hipGetLastError();

@@ -30,15 +30,15 @@ hipError_t launch_argmin_full(hipStream_t st, const float* E, const RotMeta* met
 
 // Fused, pruned pipeline stages.
 // Forward plane stage variants: one workgroup per (pose, channel) plane by
-// plane with block-wide LDS FFTs, or its four wavefronts on separate planes
-// with register FFTs.  The wavefront form needs fwd_wave_lds_bytes() of
-// dynamic LDS; configure_fwd_wave() sets the device's per-workgroup limit,
-// and fwd_wave_supported(N) says whether every box and atom count of grid
-// size N fits under it (the variant is then usable for any batch).
-constexpr int kFwdBlock = 0, kFwdWave = 1;
-size_t fwd_wave_lds_bytes(int N, int boxmax, int max_chan_atoms);
-hipError_t configure_fwd_wave(size_t max_dyn_lds);
-bool fwd_wave_supported(int N);
+// plane, with block-wide LDS Stockham FFTs or with register four-step FFTs.
+// The register form needs fwd_reg_lds_bytes() of dynamic LDS;
+// configure_fwd_reg() records the device's per-workgroup limit, and
+// fwd_reg_supported(N) says whether every box and atom count of grid size N
+// fits under it (the variant is then usable for any batch).
+constexpr int kFwdBlock = 0, kFwdReg = 1;
+size_t fwd_reg_lds_bytes(int N, int boxmax, int max_chan_atoms);
+hipError_t configure_fwd_reg(size_t max_dyn_lds);
+bool fwd_reg_supported(int N);
 hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot* slots, int nslots,
                              const LigandDesc* ligs, const int* chan_atoms, const float* wv, const float* wq,
                              const AtomGrid* ag, int astride, const RotMeta* meta, int cmax, int boxmax,

@@ -56,8 +56,8 @@ int cdk_context_destroy(cdk_context* ctx);
 /* Options: "ws_budget_bytes", "ref_batch" (B, default 50 as NoMini:208),
  * "fused_chunk" (poses per fused launch set, 0 = from budget), "mid_group"
  * (poses per mid-stage workgroup, 0 = autotuned), "fwd_variant" (forward
- * plane stage: 0 block-wide LDS FFTs, 1 one wavefront per plane with
- * register FFTs, -1 = autotuned once per grid size), "inv_variant" (inverse
+ * plane stage: 0 block-wide LDS Stockham FFTs, 1 register four-step FFTs,
+ * -1 = autotuned once per grid size), "inv_variant" (inverse
  * plane stage + argmin: 0 block-wide LDS FFTs per plane, 1 register FFTs over
  * 4 planes per workgroup, -1 = autotuned once per grid size), "use_graph",
  * "time_stages". */
@@ -82,6 +82,22 @@ int cdk_context_stage_times(cdk_context* ctx, double* ms, int* launches, int res
 int cdk_protein_create(cdk_context* ctx, int num_grids, int xdim, int ydim, int zdim, const float* mid3,
                        const float* len3, float spacing, const float* grid_radii, const float* values,
                        cdk_protein** out);
+/* Builds the potential grids on the device from the protein's atoms and
+ * keeps them resident like cdk_protein_create (the step before the search:
+ * the reference reads grids made elsewhere, ReadGrids.cpp:6-68).  xyz
+ * [natom][3] A, charge e, epsilon kcal/mol, radius A.  Grid channel p <
+ * num_probes: soft-core LJ sum_i sqrt(eps_i) ((R/d)^12 - 2 (R/d)^6), R = r_i +
+ * probe_radii[p], over d <= lj_cutoff, capped at +lj_cap; last channel:
+ * Coulomb sum_i 332.0636 q_i / (4 d^2) over d <= elec_cutoff, clamped to
+ * +-elec_cap; d^2 floored at 1e-4.  Each grid point sums its atoms in
+ * ascending index order in FP64.  Grid point (i,j,k) at mid - len/2 +
+ * (i,j,k) spacing; cubic, edge n as for cdk_protein_create. */
+int cdk_protein_build(cdk_context* ctx, int natom, const double* xyz, const double* charge, const double* epsilon,
+                      const double* radius, int num_probes, const float* probe_radii, int n, float spacing,
+                      const float* mid3, const float* len3, double lj_cutoff, double elec_cutoff, double lj_cap,
+                      double elec_cap, cdk_protein** out);
+/* Copies the resident grids out: float [num_grids][X][Y][Z]. */
+int cdk_protein_grids(cdk_protein* p, float* values);
 int cdk_protein_destroy(cdk_protein* p);
 /* Copies the spectrum of one grid channel out: float [X][Y][Z/2+1][2]. */
 int cdk_protein_spectrum(cdk_protein* p, int channel, float* out);
@@ -243,6 +259,11 @@ int cdk_idx_of_atoms_for_vdw_radius(int natom, const float* atom_radii, int num_
  * 8 soft-core LJ grids (one per probe radius, capped at +3 kcal/mol) and one
  * Coulomb grid (dielectric 4r, capped at +-20) on an n^3 grid of spacing h
  * centred in the box.  values [9][n][n][n]; radii[8]; mid3, len3. */
+/* The atoms behind cdk_synth_protein (same seed, same draws): xyz [natom][3]
+ * uniform in [0, box)^3, charge N(0, 0.3), epsilon U[0.05, 0.25], radius
+ * from the probe set. */
+int cdk_synth_protein_atoms(uint64_t seed, int natom, float box, double* xyz, double* charge, double* epsilon,
+                            double* radius);
 int cdk_synth_protein(uint64_t seed, int natom, float box, int n, float spacing, float* values, float* radii,
                       float* mid3, float* len3);
 /* Random-walk ligand (1.5 A steps, >= 1.2 A between atoms), charges

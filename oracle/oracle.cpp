@@ -20,6 +20,8 @@
 //   orc_search                    NoMini:352-372, 390-409, 441-464 per pose
 //   orc_sparse_energy_f64         the same energy as a per-atom trilinear
 //                                 gather, an independent FP64 formulation
+//   orc_build_grids               the protein potential grids from atoms (the
+//                                 synthetic recipe's definition), brute force
 //
 // Pinning: see tests/test_oracle.py (golden vectors from the compiled
 // reference sources, closed forms, invariants, brute force, planted faults).
@@ -419,3 +421,42 @@ extern "C" double orc_sparse_energy_f64(const float* Pused, int N, float h, int 
   return e;
 }
 
+
+// ---- Protein potential grids from atoms (the grid builder's definition):
+// LJ channel p: sum_i sqrt(eps_i) ((R/d)^12 - 2 (R/d)^6), R = r_i + probe_p,
+// d <= rc_lj, capped at +lj_cap; Coulomb: sum_i 332.0636 q_i / (4 d^2),
+// d <= rc_el, clamped to +-el_cap; d^2 floored at 1e-4.  Brute force over all
+// atoms in ascending index order, FP64.  atoms [natom][6] = x, y, z, q, r, eps.
+extern "C" void orc_build_grids(int natom, const double* atoms, int nprobe, const double* probe, int n,
+                                const double* gmin, double h, double rc_lj, double rc_el, double lj_cap,
+                                double el_cap, float* out) {
+  const size_t G = (size_t)n * n * n;
+  std::vector<double> lj(nprobe);
+  for (int ix = 0; ix < n; ++ix)
+    for (int iy = 0; iy < n; ++iy)
+      for (int iz = 0; iz < n; ++iz) {
+        const double gx = gmin[0] + ix * h, gy = gmin[1] + iy * h, gz = gmin[2] + iz * h;
+        std::fill(lj.begin(), lj.end(), 0.0);
+        double el = 0.0;
+        for (int i = 0; i < natom; ++i) {
+          const double* a = atoms + (size_t)i * 6;
+          const double dx = a[0] - gx, dy = a[1] - gy, dz = a[2] - gz;
+          double d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 > rc_el * rc_el) continue;
+          d2 = std::max(d2, 1e-4);
+          el += 332.0636 * a[3] / (4.0 * d2);
+          if (d2 > rc_lj * rc_lj) continue;
+          const double inv2 = 1.0 / d2;
+          const double se = std::sqrt(a[5]);
+          for (int p = 0; p < nprobe; ++p) {
+            const double R = a[4] + probe[p];
+            const double s2 = R * R * inv2;
+            const double s6 = s2 * s2 * s2;
+            lj[p] += se * (s6 * s6 - 2.0 * s6);
+          }
+        }
+        const size_t g = ((size_t)ix * n + iy) * n + iz;
+        for (int p = 0; p < nprobe; ++p) out[(size_t)p * G + g] = (float)std::min(lj[p], lj_cap);
+        out[(size_t)nprobe * G + g] = (float)std::max(-el_cap, std::min(el, el_cap));
+      }
+}

@@ -255,3 +255,18 @@ class PoseProblem:
         return _orc.orc_sparse_energy_f64(_fp(self.Pused), self.N, C.c_float(self.h), self.natom, _fp(r), _fp(m),
                                           _fp(self.charge), _fp(self.eps), self.num_used, _ip(self.atom_channel),
                                           int(t[0]), int(t[1]), int(t[2]))
+
+
+def build_grids(xyz, charge, epsilon, radius, probe_radii, n, gmin, h, rc_lj=8.0, rc_el=12.0, lj_cap=3.0,
+                el_cap=20.0):
+    """Protein potential grids from atoms, brute force FP64 (orc_build_grids)."""
+    x = np.asarray(xyz, np.float64).reshape(-1, 3)
+    atoms = np.ascontiguousarray(np.concatenate(
+        [x, np.asarray(charge, np.float64)[:, None], np.asarray(radius, np.float64)[:, None],
+         np.asarray(epsilon, np.float64)[:, None]], axis=1))
+    pr = np.ascontiguousarray(np.asarray(probe_radii, np.float32).astype(np.float64))
+    g = np.ascontiguousarray(np.asarray(gmin, np.float64))
+    out = np.empty((pr.shape[0] + 1, n, n, n), np.float32)
+    _orc.orc_build_grids(atoms.shape[0], _dp(atoms), pr.shape[0], _dp(pr), int(n), _dp(g), C.c_double(h),
+                         C.c_double(rc_lj), C.c_double(rc_el), C.c_double(lj_cap), C.c_double(el_cap), _fp(out))
+    return out

@@ -116,3 +116,14 @@ def test_write_poses_files_and_energy_rows(tmp_path):
             xyz = [float(l[30:38]), float(l[38:46]), float(l[46:54])]
             assert np.allclose(xyz, poses[i, a], atol=5e-4)
             assert l[76:78].strip() == elements[a]
+
+
+def test_synthetic_protein_matches_grid_builder_oracle():
+    """The host synthetic recipe (cell-list order) equals the oracle's
+    brute-force grid builder on the same atoms."""
+    n, h = 32, 0.5
+    x, q, e, r = cdk.synth_protein_atoms(5, 300, 16.0)
+    v, radii, mid, ln = cdk.synth_protein(5, 300, 16.0, n, h)
+    gm = np.float64(np.float32(mid[0]) - np.float32(ln[0]) / np.float32(2))
+    g = orc.build_grids(x, q, e, r, radii, n, np.full(3, gm), h)
+    assert np.array_equal(g, v)

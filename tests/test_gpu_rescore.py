@@ -157,3 +157,30 @@ def test_grid_for_openmm_order_and_bounds(ctx, p48):
         gmin = prot.grid_min()
         gmax = gmin + np.float64(np.float32(g["n"] - 1) * np.float32(g["spacing"]))
         assert np.array_equal(b, np.concatenate([gmin * 0.1, gmax * 0.1]))
+
+
+def test_protein_built_on_device_matches_oracle_and_upload(ctx):
+    """cdk_protein_build: grids equal the oracle's brute-force builder bit for
+    bit (same atom order, FP64); the spectra and a search equal those of the
+    same grids uploaded from the host."""
+    say("protein build")
+    n, h, box = 32, 0.5, 16.0
+    x, q, e, r = cdk.synth_protein_atoms(9, 400, box)
+    v, radii, mid, ln = cdk.synth_protein(9, 400, box, n, h)
+    built = cdk.Protein.build(ctx, x, q, e, r, radii, n, h, mid, ln)
+    g = built.grids()
+    gm = (np.float32(mid[0]) - np.float32(ln[0]) / np.float32(2)).astype(np.float64)
+    ref = orc.build_grids(x, q, e, r, radii, n, np.full(3, gm), h)
+    assert np.array_equal(g, ref)
+    up = cdk.Protein(ctx, g, radii, h, mid, ln)
+    for c in (0, 8):
+        assert np.array_equal(built.spectrum(c).view(np.float32), up.spectrum(c).view(np.float32))
+    L = workloads.ligand(31, 12, 1)
+    quats = workloads.quaternion_sets(32, 1, 16)
+    l1 = cdk.Ligand(built, L["charge"], L["epsilon"], L["radius"])
+    l2 = cdk.Ligand(up, L["charge"], L["epsilon"], L["radius"])
+    E1, i1, _ = cdk.search(ctx, built, l1, L["conformers"], quats)
+    E2, i2, _ = cdk.search(ctx, up, l2, L["conformers"], quats)
+    assert np.array_equal(E1.view(np.uint32), E2.view(np.uint32)) and np.array_equal(i1, i2)
+    for o in (l1, l2, built, up):
+        o.close()

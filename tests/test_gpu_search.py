@@ -130,10 +130,9 @@ def test_rank_poses_rebuild(ctx, p48):
 
 @pytest.mark.parametrize("variant", [0, 1])
 def test_plane_stage_forms_vs_oracle(ctx, p48, p64, variant):
-    """Both forms of the forward plane stage (block-wide LDS FFTs, one
-    wavefront per plane with register FFTs) and of the inverse plane stage
-    (block-wide LDS FFTs per plane, register FFTs over 4 planes) against the
-    oracle."""
+    """Both forms of the forward plane stage (block-wide LDS Stockham FFTs,
+    register four-step FFTs) and of the inverse plane stage (block-wide LDS
+    FFTs per plane, register FFTs over 4 planes) against the oracle."""
     ctx.set_option("fwd_variant", variant)
     ctx.set_option("inv_variant", variant)
     try:

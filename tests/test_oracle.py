@@ -283,3 +283,35 @@ def test_window_and_empty_window():
     res = prob.search_pose(x, np.array([1, 0, 0, 0], np.float32))
     ln = np.ptp(x, axis=0).astype(np.float32)
     assert list(res["window"]) == [32 - int(np.float32(l) / np.float32(0.5)) - 2 for l in ln]
+
+
+def test_grid_builder_oracle_closed_forms():
+    """orc_build_grids: one atom reproduces the LJ / Coulomb formulas at every
+    grid point (cutoffs and caps included); two atoms add; a planted charge
+    change is seen."""
+    n, h = 32, 0.5
+    gmin = np.array([-3.0, -2.5, -4.0])
+    probes = np.array([1.2, 1.7, 2.2], np.float32)
+    xyz = np.array([[1.1, 2.3, 0.7]])
+    q, eps, rad = np.array([0.4]), np.array([0.16]), np.array([1.5])
+    g = orc.build_grids(xyz, q, eps, rad, probes, n, gmin, h)
+    ax = gmin[:, None] + np.arange(n)[None, :] * h
+    d2 = ((ax[0][:, None, None] - xyz[0, 0]) ** 2 + (ax[1][None, :, None] - xyz[0, 1]) ** 2
+          + (ax[2][None, None, :] - xyz[0, 2]) ** 2)
+    d2c = np.maximum(d2, 1e-4)
+    el = np.where(d2 <= 144.0, 332.0636 * q[0] / (4.0 * d2c), 0.0)
+    assert np.allclose(g[3], np.clip(el, -20, 20).astype(np.float32), rtol=1e-6, atol=1e-6)
+    for p in range(3):
+        s6 = ((rad[0] + np.float64(probes[p])) ** 2 / d2c) ** 3
+        lj = np.where(d2 <= 64.0, np.sqrt(eps[0]) * (s6 * s6 - 2 * s6), 0.0)
+        assert np.allclose(g[p], np.minimum(lj, 3.0).astype(np.float32), rtol=1e-6, atol=1e-6)
+    assert (g[:3] == 3.0).any() and (g[3] == 20.0).any()  # caps reached near the atom
+    assert (g[3] == 0.0).any() and (g[0] == 0.0).any()    # cutoffs reached far away
+    # a second, distant atom adds its own field (no cap involved far from both)
+    xyz2 = np.array([[1.1, 2.3, 0.7], [9.0, 9.0, 9.0]])
+    g2 = orc.build_grids(xyz2, np.array([0.4, -0.3]), np.array([0.16, 0.09]), np.array([1.5, 1.8]), probes, n,
+                         gmin, h)
+    gb = orc.build_grids(xyz2[1:], np.array([-0.3]), np.array([0.09]), np.array([1.8]), probes, n, gmin, h)
+    far = (np.abs(g) < 1.0) & (np.abs(gb) < 1.0)
+    assert np.allclose(g2[far], (g.astype(np.float64) + gb)[far], rtol=1e-5, atol=1e-6)
+    assert not np.array_equal(orc.build_grids(xyz, q + 0.01, eps, rad, probes, n, gmin, h), g)
