@@ -81,6 +81,26 @@ def reference_bytes(N, C):
     return 8 * C * G + 24 * C * H + 16 * H + 8 * G
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_single_ligand_64.md")
+
+
+def profiled_traffic(kernel, poses_per_launch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the single-ligand workload (per-pose
+    figures, copy-calibrated), scaled to this run's poses per launch; None if
+    the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for ln in f:
+                if ln.startswith("traffic-per-pose:"):
+                    kv = dict(t.split("=") for t in ln.split(":", 1)[1].split())
+                    if kv.get("kernel") == kernel:
+                        return (float(kv["fetch"]) + float(kv["write"])) * poses_per_launch
+    except OSError:
+        pass
+    return None
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
@@ -272,7 +292,7 @@ def line(args, metric, value, total, workload, extra_config, roof, kernels, cpu,
 
 def run_single(args, cdk, workloads):
     t0 = time.time()
-    g = workloads.protein_grids("p64")
+    g = workloads.protein_grids(args.protein)
     w = workloads.config2(nconf=args.conformers, nquat=args.quaternions)
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
@@ -293,6 +313,10 @@ def run_single(args, cdk, workloads):
     box, win = batch.geometry(0)
     n_valid = int(np.isfinite(E).sum())
     kernels, roof = kernel_leg(ctx, cdk, batch, [(box, win, C)], prot.num_grids, N)
+    tr = profiled_traffic(roof["kernel"], n_valid / kernels[roof["kernel"]]["launches"]) if args.protein == "p64" else None
+    if tr is not None:
+        roof["traffic"] = round(tr)
+        roof["traffic_source"] = "profiles/traffic_single_ligand_64.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per pose)"
 
     pc = max(1, min(args.plain_conformers, args.conformers))
     ref = reference_leg(ctx, cdk, prot, lig, L["conformers"][:pc], L["quats"][:pc], idx[:pc])
@@ -305,9 +329,10 @@ def run_single(args, cdk, workloads):
         serving["speedup_vs_reference_engine"] = round(serving["poses_per_s"] / ref["value"], 2)
         log("serving", json.dumps(serving))
 
-    line(args, METRIC, value, total,
-         "single-ligand throughput: 64^3 x 9ch FP32 grids at 0.5 A, 40-atom ligand, "
-         "%d conformers x %d quaternions = %d poses" % (args.conformers, args.quaternions, npose),
+    metric = METRIC if args.protein == "p64" else "poses scored/sec at %d³×9ch FP32" % N
+    line(args, metric, value, total,
+         "single-ligand throughput: %d^3 x 9ch FP32 grids at %g A, 40-atom ligand, "
+         "%d conformers x %d quaternions = %d poses" % (N, g["spacing"], args.conformers, args.quaternions, npose),
          {"grid_edge": N, "channels_used": C, "poses": npose, "valid_poses": n_valid,
           "engine": "fused" + ("" if args.no_graph else "+hipGraph"), "translations_per_s": round(value * N ** 3, 1)},
          roof, kernels, cpu,
@@ -450,6 +475,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["single", "batched", "large"], default="single")
+    ap.add_argument("--protein", choices=["p64", "p96", "p128"], default="p64",
+                    help="grids of the single-ligand workload (the headline is p64)")
     ap.add_argument("--conformers", type=int, default=100)
     ap.add_argument("--quaternions", type=int, default=2000)
     ap.add_argument("--plain-conformers", type=int, default=10)
