@@ -667,7 +667,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
   const float2* twl = TW_LDS ? stw + n2 * V : twr;
   const int bxm = min(m.box[0], boxmax), by = m.box[1], bz = m.box[2];
   const int npair = (by + 1) >> 1;
-  const bool zhalf = bz <= N / 2, yhalf = by <= N / 2;
+  // A pruned first stage exists for 8 and 16 values per lane; other lengths
+  // take the full transform (one code path, fewer live registers).
+  constexpr bool PRUNE = V == 8 || V == 16;
+  const bool zhalf = PRUNE && bz <= N / 2, yhalf = PRUNE && by <= N / 2;
   const int zlim = zhalf ? N / 2 : N;
   float2* op0 = A + (((size_t)s * cmax + c) * boxmax) * (size_t)(N * ZH);
   const int dy = (lane >> 1) & 1, dz = lane & 1;
@@ -703,10 +706,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
       const int p = p0 + ln;
       const bool ok = p < npair;
       float2 v[V];
-      if (zhalf) {
+      if (PRUNE && zhalf) {
 #pragma unroll
         for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
-        reg_fft<N, -1, true>(v, twl, tr, ln, n2);
+        reg_fft<N, -1, PRUNE>(v, twl, tr, ln, n2);
       } else {
 #pragma unroll
         for (int n1 = 0; n1 < V; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
@@ -738,10 +741,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
         }
         return r;
       };
-      if (yhalf) {
+      if (PRUNE && yhalf) {
 #pragma unroll
         for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = col(n2 + L * n1);
-        reg_fft<N, -1, true>(v, twl, tr, ln, n2);
+        reg_fft<N, -1, PRUNE>(v, twl, tr, ln, n2);
       } else {
 #pragma unroll
         for (int n1 = 0; n1 < V; ++n1) v[n1] = col(n2 + L * n1);
@@ -1017,20 +1020,29 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin_reg(const RotMeta* __rest
   using IS = InvRegShape<N>;
   constexpr int L = RS_::L, V = RS_::V, LPW = RS_::LPW, RS = IS::RS, XPB = IS::XPB;
   constexpr int NYL = N / 2;  // y lines: columns 1..N/2-1 and the packed {0, N/2}
+  // Long lines (>= 16 values per lane): twiddles from LDS and no prefetch of
+  // the next plane, so the registers go to the data.
+  constexpr bool LONG = V >= 16;
   __shared__ float2 tile[N * RS];
   __shared__ float2 sbuf[4][RS_::SLICE];
+  __shared__ float2 stw[LONG ? L * V : 1];
   const int s = blockIdx.y;
   const RotMeta m = meta[s];
   const int x0 = blockIdx.x * XPB;
   if (!m.valid || x0 >= m.win[0]) return;
+  if constexpr (LONG) {
+    for (int t = threadIdx.x; t < L * V; t += blockDim.x) stw[t] = twg[(t / V) * (t % V)];
+    __syncthreads();
+  }
   const int x1 = min(x0 + XPB, m.win[0]);
   const int wy = m.win[1], wz = m.win[2];
   const int npw = (wy + 1) >> 1;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int ln = lane % LPW, n2 = lane / LPW;
-  float2 twl[V];
-  reg_twiddles<N>(twg, n2, twl);
+  float2 twr[LONG ? 1 : V];
+  if constexpr (!LONG) reg_twiddles<N>(twg, n2, twr);
+  const float2* twl = LONG ? stw + n2 * V : twr;
   float2* lds = sbuf[wave];
   unsigned long long best = ~0ull;
   auto load_cols = [&](int xp, int pass, float2* v) {
@@ -1050,16 +1062,16 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin_reg(const RotMeta* __rest
     }
   };
   constexpr int YPASS = (NYL + 4 * LPW - 1) / (4 * LPW);
-  float2 nxt[V];
-  load_cols(x0, 0, nxt);
+  float2 nxt[LONG ? 1 : V];
+  if constexpr (!LONG) load_cols(x0, 0, nxt);
   for (int xp = x0; xp < x1; ++xp) {
     // 1-2. y inverse transforms into the tile.
 #pragma unroll
     for (int pass = 0; pass < YPASS; ++pass) {
       float2 v[V];
-      if (pass == 0) {
+      if (!LONG && pass == 0) {
 #pragma unroll
-        for (int i = 0; i < V; ++i) v[i] = nxt[i];
+        for (int i = 0; i < (LONG ? 1 : V); ++i) v[i] = nxt[i];
       } else {
         load_cols(xp, pass, v);
       }
@@ -1081,7 +1093,9 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin_reg(const RotMeta* __rest
       }
     }
     __syncthreads();
-    if (xp + 1 < x1) load_cols(xp + 1, 0, nxt);
+    if constexpr (!LONG) {
+      if (xp + 1 < x1) load_cols(xp + 1, 0, nxt);
+    }
     // 3. z C2R of the row pairs and the window minimum.
     for (int p0 = wave * LPW; p0 < npw; p0 += 4 * LPW) {
       const int p = p0 + ln;
@@ -1388,7 +1402,7 @@ hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const Ligand
   return hipGetLastError();
 }
 
-bool inv_reg_supported(int N) { return N <= 64; }
+bool inv_reg_supported(int N) { return grid_edge_supported(N); }
 
 hipError_t launch_inv_argmin(hipStream_t st, int N, int variant, const RotMeta* meta, int nslots, int wmax,
                              const float2* T, unsigned long long* keys, const float2* tw) {
@@ -1396,10 +1410,8 @@ hipError_t launch_inv_argmin(hipStream_t st, int N, int variant, const RotMeta* 
   if (variant == kInvReg) {
     if (!inv_reg_supported(N)) return hipErrorInvalidValue;
     CDK_DISPATCH_N(N, {
-      if constexpr (PL::N <= 64) {
-        const int gx = (wmax + InvRegShape<PL::N>::XPB - 1) / InvRegShape<PL::N>::XPB;
-        hipLaunchKernelGGL(k_inv_argmin_reg<PL>, dim3(gx, nslots), dim3(kBlock), 0, st, meta, T, keys, tw);
-      }
+      const int gx = (wmax + InvRegShape<PL::N>::XPB - 1) / InvRegShape<PL::N>::XPB;
+      hipLaunchKernelGGL(k_inv_argmin_reg<PL>, dim3(gx, nslots), dim3(kBlock), 0, st, meta, T, keys, tw);
     });
     return hipGetLastError();
   }

@@ -48,7 +48,7 @@ hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const Ligand
                       float2* T, float invG, const float2* tw);
 // Inverse plane stage variants: one workgroup per (pose, window plane) with
 // block-wide LDS FFTs, or one workgroup per (pose, 4 window planes) with
-// register FFTs and one LDS tile (grid sizes up to 64).
+// register FFTs and one LDS tile of N rows.
 constexpr int kInvBlock = 0, kInvReg = 1;
 bool inv_reg_supported(int N);
 hipError_t launch_inv_argmin(hipStream_t st, int N, int variant, const RotMeta* meta, int nslots, int wmax,
