@@ -211,7 +211,8 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
             "algorithmic_bytes_per_launch": round(dom_bytes)}
     kernels["tuned"] = {"fwd_variant": ["block", "register"][ctx.get_option("last_fwd_variant")],
                         "inv_variant": ["block", "register"][ctx.get_option("last_inv_variant")],
-                        "mid_group": group, "max_dyn_lds": ctx.get_option("max_dyn_lds")}
+                        "mid_group": group, "mid_depth": ctx.get_option("last_mid_depth"),
+                        "max_dyn_lds": ctx.get_option("max_dyn_lds")}
     return kernels, roof
 
 

@@ -462,7 +462,7 @@ void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, siz
 }
 
 struct Tuned {
-  int fwd = kFwdBlock, inv = kInvBlock, group = 16;
+  int fwd = kFwdBlock, inv = kInvBlock, group = 16, depth = 1;
 };
 
 // Autotuner: picks the forward and inverse plane forms (once per grid size,
@@ -498,14 +498,18 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     else
       ctx.tuned_fwd[N] = tu.fwd = kFwdBlock;
   }
+  const int max_depth = mid_max_depth(N);
   if (ctx.mid_group > 0) {
     tu.group = ctx.mid_group;
+    tu.depth = ctx.mid_depth > 0 ? std::min(ctx.mid_depth, max_depth) : std::min(2, max_depth);
   } else {
     auto it = ctx.tuned_group.find(gkey);
-    if (it != ctx.tuned_group.end())
-      tu.group = it->second;
-    else
+    if (it != ctx.tuned_group.end()) {
+      tu.group = it->second.first;
+      tu.depth = it->second.second;
+    } else {
       need_group = true;
+    }
   }
   if (!need_fwd && !need_inv && !need_group) return tu;
 
@@ -549,19 +553,33 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       fwd(tu.fwd);
     }
     if (need_group) {
-      float best = 1e30f;
-      for (int g : {4, 8, 16, 32}) {
-        const float ms = timed([&] {
-          hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
-                               prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)prot.G(), tw),
+      auto mid = [&](int g, int depth) {
+        return timed([&] {
+          hip_check(launch_mid(ctx.stream, N, depth, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax,
+                               t.boxmax, prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(),
+                               1.0f / (float)prot.G(), tw),
                     "tune mid");
         });
+      };
+      // Group size at the shallowest ring, then the ring depth at that group.
+      const int d0 = std::min(2, max_depth);
+      float best = 1e30f;
+      for (int g : {4, 8, 16, 32}) {
+        const float ms = mid(g, d0);
         if (ms < best) {
           best = ms;
           tu.group = g;
         }
       }
-      ctx.tuned_group[gkey] = tu.group;
+      tu.depth = d0;
+      for (int d = d0 + 1; d <= max_depth; ++d) {
+        const float ms = mid(tu.group, d);
+        if (ms < best) {
+          best = ms;
+          tu.depth = d;
+        }
+      }
+      ctx.tuned_group[gkey] = {tu.group, tu.depth};
     }
     if (need_inv) {
       auto inv = [&](int variant) {
@@ -612,7 +630,8 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
               "fwd planes");
     timer.end(e);
     e = timer.begin(2);
-    hip_check(launch_mid(ctx.stream, N, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
+    hip_check(launch_mid(ctx.stream, N, tu.depth, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), t.cmax,
+                         t.boxmax,
                          prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
               "mid");
     timer.end(e);
@@ -681,9 +700,10 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
     ctx.last_fwd = tu.fwd;
     ctx.last_inv = tu.inv;
     ctx.last_group = tu.group;
+    ctx.last_depth = tu.depth;
     if (use_graph && !ctx.time_stages) {
       if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.group ||
-          t.graph_fwd != tu.fwd || t.graph_inv != tu.inv) {
+          t.graph_fwd != tu.fwd || t.graph_inv != tu.inv || t.graph_depth != tu.depth) {
         t.drop_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -703,6 +723,7 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
         t.graph_group = tu.group;
         t.graph_fwd = tu.fwd;
         t.graph_inv = tu.inv;
+        t.graph_depth = tu.depth;
       }
       hip_check(hipGraphLaunch(t.graph, ctx.stream), "hipGraphLaunch");
     } else {

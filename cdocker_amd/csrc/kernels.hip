@@ -786,9 +786,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
 // global memory already in the register layout the transform wants, and the
 // forward output layout is the inverse's input layout.  A workgroup is 4
 // wavefronts and walks a group of poses as one sequence of (pose, channel)
-// items; for lines of at most 8 values per lane two register buffers
-// alternate, so the loads of item i+1 are in flight while item i is
-// transformed, with no register copies.  When the
+// items; a ring of DEPTH register buffers (DEPTH tuned, 1 for long lines)
+// keeps the next items' loads in flight while one item is transformed.  When the
 // pose's box is at most N/2 the upper half of every line is zero: those rows
 // are not loaded and the first DFT stage is pruned.  Workgroup w runs on
 // XCD w % 8 and the line groups are split so each XCD touches one eighth of
@@ -799,7 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
 #else
 #define CDK_MID_ATTR
 #endif
-template <class P>
+template <class P, int DEPTH, bool BOXHALF>
 __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __restrict__ slots, const LigandDesc* __restrict__ ligs,
                                                 const RotMeta* __restrict__ meta, int nslots, int group,
                                                 const float2* __restrict__ A, int cmax, int boxmax,
@@ -814,6 +813,8 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   constexpr int PERX = (NLG + 7) / 8;    // line groups per XCD
   // Long lines read their twiddles from LDS (registers go to the data).
   constexpr bool TW_LDS = V >= 16;
+  // BOXHALF: every pose's box fits in N/2, so only V/2 ligand rows exist.
+  constexpr int VA = BOXHALF ? V / 2 : V;
   __shared__ float2 sbuf[4][RS::SLICE];
   __shared__ float2 stw[TW_LDS ? L * V : 1];
   const int w = blockIdx.x;
@@ -877,12 +878,12 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
     const int pc = uni(ligs[it.lig].pmap[it.c]);
     const __amdgpu_buffer_rsrc_t q =
         buffer_rsrc(Phat + (size_t)pc * N * plane + blk * N, (unsigned)(N * LPW * sizeof(float2)));
-    if (it.bx <= N / 2) {
+    if (BOXHALF || it.bx <= N / 2) {
 #pragma unroll
       for (int n1 = 0; n1 < V / 2; ++n1) ra[n1] = buffer_load_f2(r, voffA + n1 * kRowA);
     } else {
 #pragma unroll
-      for (int n1 = 0; n1 < V; ++n1) ra[n1] = buffer_load_f2(r, voffA + n1 * kRowA);
+      for (int n1 = 0; n1 < VA; ++n1) ra[n1] = buffer_load_f2(r, voffA + n1 * kRowA);
     }
 #pragma unroll
     for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(q, voffP + m * kRowP);
@@ -893,13 +894,22 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   // Transform item `it` held in (ra, rp) and accumulate conj(L) P; after the
   // pose's last channel run the inverse and store the window rows.
   auto consume = [&](const Item& it, float2* ra, const float2* rp) {
-    if (it.bx <= N / 2)
-      reg_fft<N, -1, true>(ra, twl, lds, ln, n2);
+    // The transform runs in place on a full buffer; a half buffer (BOXHALF)
+    // widens into a temporary.
+    float2 vt[VA < V ? V : 1];
+    float2* v = ra;
+    if constexpr (VA < V) {
+#pragma unroll
+      for (int i = 0; i < VA; ++i) vt[i] = ra[i];
+      v = vt;
+    }
+    if (BOXHALF || it.bx <= N / 2)
+      reg_fft<N, -1, true>(v, twl, lds, ln, n2);
     else
-      reg_fft<N, -1, false>(ra, twl, lds, ln, n2);
+      reg_fft<N, -1, false>(v, twl, lds, ln, n2);
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      const float2 l = ra[i], p = rp[i];
+      const float2 l = v[i], p = rp[i];
       acc[i].x = fmaf(l.x, p.x, acc[i].x);
       acc[i].x = fmaf(l.y, p.y, acc[i].x);
       acc[i].y = fmaf(l.x, p.y, acc[i].y);
@@ -918,37 +928,33 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
       for (int i = 0; i < V; ++i) acc[i] = make_float2(0.f, 0.f);
     }
   };
-  Item i0;
-  enter(i0, next_valid(sg * group));
-  if (i0.s >= s1) return;
-  if constexpr (V <= 8) {
-    Item i1 = i0;
-    float2 a0[V], p0[V], a1[V], p1[V];
-    load(i0, a0, p0);
-    advance(i1);
-    if (i1.s < s1) load(i1, a1, p1);
-    for (;;) {
-      consume(i0, a0, p0);
-      if (i1.s >= s1) break;
-      Item i2 = i1;
-      advance(i2);
-      if (i2.s < s1) load(i2, a0, p0);
-      consume(i1, a1, p1);
-      if (i2.s >= s1) break;
-      Item i3 = i2;
-      advance(i3);
-      if (i3.s < s1) load(i3, a1, p1);
-      i0 = i2;
-      i1 = i3;
-    }
-  } else {
-    // Long lines: one buffer (two would halve the waves per SIMD).
-    float2 a0[V], p0[V];
-    for (;;) {
-      load(i0, a0, p0);
-      consume(i0, a0, p0);
-      advance(i0);
-      if (i0.s >= s1) break;
+  Item first;
+  enter(first, next_valid(sg * group));
+  if (first.s >= s1) return;
+  // Ring of DEPTH register buffers over the (pose, channel) sequence: the
+  // loads of the next DEPTH-1 items are in flight while one is transformed.
+  // Buffer indices are compile-time (the ring step is unrolled), so nothing
+  // is copied between registers.
+  Item it[DEPTH];
+  float2 ab[DEPTH][VA], pb[DEPTH][V];
+  it[0] = first;
+#pragma unroll
+  for (int d = 1; d < DEPTH; ++d) {
+    it[d] = it[d - 1];
+    advance(it[d]);
+  }
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d)
+    if (it[d].s < s1) load(it[d], ab[d], pb[d]);
+  for (;;) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) {
+      if (it[d].s >= s1) return;  // items are in order: nothing after it is left
+      consume(it[d], ab[d], pb[d]);
+      Item nx = it[(d + DEPTH - 1) % DEPTH];
+      advance(nx);
+      it[d] = nx;
+      if (nx.s < s1) load(nx, ab[d], pb[d]);
     }
   }
 }
@@ -1388,16 +1394,45 @@ int mid_line_groups(int N) {
   return ((NLG + 7) / 8) * 8;
 }
 
-hipError_t launch_mid(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
-                      int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, int npch,
-                      float2* T, float invG, const float2* tw) {
+namespace {
+template <class PL, int D, bool H>
+void mid_launch_one(dim3 grid, hipStream_t st, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
+                    int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, float2* T,
+                    float invG, const float2* tw) {
+  hipLaunchKernelGGL((k_mid<PL, D, H>), grid, dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, A, cmax, boxmax,
+                     Phat, T, invG, tw);
+}
+}  // namespace
+
+int mid_max_depth(int N) { return (N == 32 || N == 64) ? 3 : 1; }
+
+hipError_t launch_mid(hipStream_t st, int N, int depth, const PoseSlot* slots, const LigandDesc* ligs,
+                      const RotMeta* meta, int nslots, int group, const float2* A, int cmax, int boxmax,
+                      const float2* Phat, int npch, float2* T, float invG, const float2* tw) {
   (void)npch;
   if (nslots <= 0) return hipSuccess;
   const unsigned nsg = ceil_div(nslots, group);
-  const unsigned grid = (unsigned)mid_line_groups(N) * nsg;
+  const dim3 grid((unsigned)mid_line_groups(N) * nsg);
+  const bool half = 2 * boxmax <= N;  // every valid pose's box fits in N/2
   CDK_DISPATCH_N(N, {
-    hipLaunchKernelGGL(k_mid<PL>, dim3(grid), dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, A, cmax, boxmax,
-                       Phat, T, invG, tw);
+    if constexpr (RegShape<PL::N>::V <= 8) {
+      if (depth >= 3) {
+        if (half)
+          mid_launch_one<PL, 3, true>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+        else
+          mid_launch_one<PL, 3, false>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+      } else {
+        if (half)
+          mid_launch_one<PL, 2, true>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+        else
+          mid_launch_one<PL, 2, false>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+      }
+    } else {
+      if (half)
+        mid_launch_one<PL, 1, true>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+      else
+        mid_launch_one<PL, 1, false>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+    }
   });
   return hipGetLastError();
 }

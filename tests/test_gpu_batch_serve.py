@@ -69,6 +69,27 @@ def test_chunking_does_not_change_results(ctx, prots):
     assert same(E0, E1) and np.array_equal(i0, i1)
 
 
+def test_mid_schedule_does_not_change_results(ctx, prots):
+    """Mid-stage group size and register ring depth (what the autotuner picks
+    per launch shape) only reorder work: results are bit-identical."""
+    say("mid schedule invariance")
+    g, prot = prots[1]
+    w = workloads.config2(nconf=3, nquat=40)
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+    try:
+        for grp, depth in ((4, 2), (8, 3), (32, 2), (5, 3)):
+            ctx.set_option("mid_group", grp)
+            ctx.set_option("mid_depth", depth)
+            E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+            assert ctx.get_option("last_mid_depth") == depth
+            assert same(E0, E1) and np.array_equal(i0, i1), (grp, depth)
+    finally:
+        ctx.set_option("mid_group", 0)
+        ctx.set_option("mid_depth", 0)
+
+
 def test_server_alone_equals_in_mix(ctx, prots):
     say("server alone vs mix")
     rs = np.random.default_rng(11)
