@@ -324,6 +324,18 @@ def run_single(args, cdk, workloads):
     ref["speedup_fused"] = round(value / ref["value"], 2)
     cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
 
+    # The same protein's grids built on the device from its atoms.
+    natom_p, box_p, _, _ = workloads.PROTEINS[args.protein]
+    xyz_p, q_p, e_p, r_p = cdk.synth_protein_atoms(workloads.PROTEIN_SEED[args.protein], natom_p, box_p)
+    ts = time.perf_counter()
+    built = cdk.Protein.build(ctx, xyz_p, q_p, e_p, r_p, g["radii"], N, g["spacing"], g["mid"], g["length"])
+    build_s = time.perf_counter() - ts
+    diff = float(np.abs(built.grids().astype(np.float64) - g["values"]).max())
+    built.close()
+    grid_builder = {"atoms": natom_p, "grids": "%d^3 x 9" % N, "device_s_incl_spectra": round(build_s, 4),
+                    "max_abs_diff_vs_host_recipe": diff}
+    log("grid builder", grid_builder)
+
     serving = None
     if args.serve_requests > 0:
         serving = serve_leg(ctx, cdk, workloads, args.serve_requests, args.serve_load, args.serve_batch, 7100)
@@ -339,7 +351,7 @@ def run_single(args, cdk, workloads):
          roof, kernels, cpu,
          pipeline_equiv={"bytes_per_pose": reference_bytes(N, C), "GBps": round(value * reference_bytes(N, C) / 1e9, 1),
                          "note": "poses/s x B(N,C) of the unfused reference dataflow"},
-         reference_engine=ref, serving=serving)
+         reference_engine=ref, serving=serving, grid_builder=grid_builder)
     batch.close()
     lig.close()
     prot.close()

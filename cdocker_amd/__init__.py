@@ -13,6 +13,7 @@ from .api import (  # noqa: F401
     QuaternionUniformSampling,
     Rng,
     Server,
+    generate_conformers,
     grid_for_openmm,
     rank_poses,
     refine_poses,

@@ -82,6 +82,8 @@ _SIGS = {
     "cdk_refine_poses": [P, P, I, DP, I, C.c_double, DP],
     "cdk_write_poses": [CP, I, I, I, DP, FP, C.POINTER(CP), C.POINTER(CP), CP],
     "cdk_grid_for_openmm": [P, I, DP, DP],
+    "cdk_generate_conformers": [P, I, DP, I, IP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), I, U64, C.c_double, I,
+                                DP, IP, IP],
     "cdk_batch_create": [P, P, PP],
     "cdk_batch_add": [P, P, I, FP, I, FP, I, IP],
     "cdk_batch_run": [P, I],

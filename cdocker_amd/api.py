@@ -362,6 +362,25 @@ def grid_for_openmm(protein, channel):
     return v, b
 
 
+def generate_conformers(ctx, coords, rotors, moving, exclude, ntrial, seed, clash_dist=1.2, max_keep=None):
+    """Random-torsion conformers on the device; returns (conformers
+    [n][natom][3] float64, trial index of each)."""
+    x = np.ascontiguousarray(np.asarray(coords, np.float64).reshape(-1, 3))
+    na = x.shape[0]
+    r = np.ascontiguousarray(np.asarray(rotors, np.int32).reshape(-1, 4))
+    mv = np.ascontiguousarray(np.asarray(moving, np.uint8).reshape(r.shape[0], na))
+    ex = np.ascontiguousarray(np.asarray(exclude, np.uint8).reshape(na, na))
+    keep = int(ntrial if max_keep is None else max_keep)
+    out = np.empty((max(keep, 1), na, 3), np.float64)
+    ti = np.empty(max(keep, 1), np.int32)
+    n = C.c_int()
+    check(lib.cdk_generate_conformers(ctx.h, na, _dp(x), r.shape[0], _ip(r), mv.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                      ex.ctypes.data_as(C.POINTER(C.c_uint8)), int(ntrial), C.c_uint64(seed),
+                                      float(clash_dist), keep, _dp(out), _ip(ti), C.byref(n)),
+          "cdk_generate_conformers")
+    return out[:n.value], ti[:n.value]
+
+
 class Batch:
     """Many ligands searched together in shared launches."""
 

@@ -411,6 +411,19 @@ int cdk_grid_for_openmm(cdk_protein* p, int channel, double* values, double* bou
   });
 }
 
+int cdk_generate_conformers(cdk_context* ctx, int natom, const double* coords, int nrotor, const int* rotors,
+                            const uint8_t* moving, const uint8_t* exclude, int ntrial, uint64_t seed,
+                            double clash_dist, int max_keep, double* out, int* trial_index, int* n_kept) {
+  return guard([&] {
+    require(ctx && coords && out && n_kept && exclude, "NULL argument");
+    require(natom > 0 && natom <= cdk::kMaxAtoms && nrotor >= 0 && ntrial >= 0 && max_keep >= 0, "bad counts");
+    require(nrotor == 0 || (rotors && moving), "NULL rotor arrays");
+    require(ntrial <= (1 << 20), "at most 2^20 trials per call");
+    *n_kept = cdk::generate_conformers(*ctx->c, natom, coords, nrotor, rotors, moving, exclude, ntrial, seed,
+                                       clash_dist, max_keep, out, trial_index);
+  });
+}
+
 int cdk_batch_create(cdk_context* ctx, cdk_protein* p, cdk_batch** out) {
   return guard([&] {
     require(ctx && p && out, "NULL argument");

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <limits>
 
+#include "conformers.hpp"
 #include "gridbuild.hpp"
 #include "kernels.hpp"
 #include "rescore.hpp"
@@ -861,6 +862,47 @@ void refine_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npos
   hip_check(hipMemcpyAsync(energy, w.energy.p, (size_t)npose * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
             "energy D2H");
   hip_check(hipStreamSynchronize(ctx.stream), "refine sync");
+}
+
+int generate_conformers(Context& ctx, int natom, const double* coords, int nrotor, const int* rotors,
+                        const unsigned char* moving, const unsigned char* exclude, int ntrial, uint64_t seed,
+                        double clash_dist, int max_keep, double* out, int* trial_index) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  if (ntrial <= 0 || max_keep <= 0) return 0;
+  for (int r = 0; r < nrotor; ++r)
+    for (int k = 0; k < 4; ++k)
+      if (rotors[r * 4 + k] < 0 || rotors[r * 4 + k] >= natom) throw Error(kErrInvalid, "rotor atom out of range");
+  std::vector<double> x(coords, coords + (size_t)natom * 3);
+  std::vector<int> rot(rotors, rotors + (size_t)nrotor * 4);
+  std::vector<unsigned char> mv(moving, moving + (size_t)nrotor * natom);
+  std::vector<unsigned char> ex(exclude, exclude + (size_t)natom * natom);
+  DevBuf d_x, d_rot, d_mv, d_ex, d_out, d_ok;
+  upload(ctx, d_x, x);
+  upload(ctx, d_rot, rot);
+  upload(ctx, d_mv, mv);
+  upload(ctx, d_ex, ex);
+  const size_t per = (size_t)natom * 3;
+  d_out.ensure((size_t)ntrial * per * sizeof(double));
+  d_ok.ensure((size_t)ntrial * sizeof(int));
+  hip_check(launch_conformers(ctx.stream, natom, d_x.as<double>(), nrotor, d_rot.as<int>(),
+                              d_mv.as<unsigned char>(), d_ex.as<unsigned char>(), ntrial, seed, clash_dist,
+                              d_out.as<double>(), d_ok.as<int>()),
+            "conformers");
+  std::vector<int> ok(ntrial);
+  std::vector<double> all((size_t)ntrial * per);
+  hip_check(hipMemcpyAsync(ok.data(), d_ok.p, ok.size() * sizeof(int), hipMemcpyDeviceToHost, ctx.stream), "ok D2H");
+  hip_check(hipMemcpyAsync(all.data(), d_out.p, all.size() * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+            "conformers D2H");
+  hip_check(hipStreamSynchronize(ctx.stream), "conformers sync");
+  int kept = 0;
+  for (int t = 0; t < ntrial && kept < max_keep; ++t) {
+    if (!ok[t]) continue;
+    std::memcpy(out + (size_t)kept * per, all.data() + (size_t)t * per, per * sizeof(double));
+    if (trial_index) trial_index[kept] = t;
+    ++kept;
+  }
+  return kept;
 }
 
 }  // namespace cdk

@@ -214,6 +214,14 @@ void rescore_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npo
 void refine_poses(Context& ctx, const Protein& prot, const Ligand& lig, int npose, double* coords, int steps,
                   double penalty, double* energy);
 
+// Random-torsion conformers on the device (conformers.hip): every trial sets
+// the rotors in order to angles from a (seed, trial, rotor) stream; trials
+// passing the steric filter are kept in trial order, at most max_keep.
+// Returns the number kept; out [max_keep][natom][3], trial_index [max_keep].
+int generate_conformers(Context& ctx, int natom, const double* coords, int nrotor, const int* rotors,
+                        const unsigned char* moving, const unsigned char* exclude, int ntrial, uint64_t seed,
+                        double clash_dist, int max_keep, double* out, int* trial_index);
+
 // Retained-pose export (export.cpp).
 std::string pose_file_name(int conformer, int rank);
 std::string energy_row(int conformer, int rank, float energy);

@@ -172,6 +172,19 @@ int cdk_write_poses(const char* dir, int conformer, int n, int natom, const doub
  * values (X*Y*Z doubles) or bounds_nm may be NULL. */
 int cdk_grid_for_openmm(cdk_protein* p, int channel, double* values, double* bounds_nm);
 
+/* ---- conformers -------------------------------------------------------- */
+/* Random-torsion conformers on the device (GeneRandomConformations.cpp:36-47;
+ * the OpenMM minimisation and energy cap of :49-109 are replaced by a steric
+ * filter).  Trial t < ntrial sets each rotor r = (a, b, c, d), in order, to
+ * the IUPAC dihedral theta = -pi + 2 pi u(seed, t, r) by turning the atoms
+ * flagged in moving[r][natom] about b->c; a trial is kept when no atom pair
+ * with exclude[i][j] == 0 is closer than clash_dist.  The first max_keep
+ * kept trials, in trial order, go to out [max_keep][natom][3] (and their
+ * trial numbers to trial_index, may be NULL); *n_kept receives the count. */
+int cdk_generate_conformers(cdk_context* ctx, int natom, const double* coords, int nrotor, const int* rotors,
+                            const uint8_t* moving, const uint8_t* exclude, int ntrial, uint64_t seed,
+                            double clash_dist, int max_keep, double* out, int* trial_index, int* n_kept);
+
 /* ---- batched search: many ligands in one set of launches ------------- */
 int cdk_batch_create(cdk_context* ctx, cdk_protein* p, cdk_batch** out);
 int cdk_batch_add(cdk_batch* b, const cdk_ligand* l, int nconf, const float* conf_coords, int nquat,

@@ -460,3 +460,71 @@ extern "C" void orc_build_grids(int natom, const double* atoms, int nprobe, cons
         out[(size_t)nprobe * G + g] = (float)std::max(-el_cap, std::min(el, el_cap));
       }
 }
+
+// ---- Random-torsion conformers (GeneRandomConformations.cpp:36-47 with a
+// counter-based stream and a steric filter in place of OpenMM): every rotor
+// (a, b, c, d) in order is set to theta = -pi + 2 pi u(seed, trial, rotor);
+// the atoms flagged in moving[r] turn about b->c through c by theta - phi,
+// phi the current (IUPAC) dihedral.  ok[t] = 1 when no non-excluded pair is closer
+// than clash_dist.  FP64, plain loops.
+static double orc_torsion_uniform(uint64_t seed, int trial, int rotor) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)trial * 0x100000001ull + (uint64_t)rotor + 1ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+extern "C" void orc_conformers(int natom, const double* coords, int nrotor, const int* rotors,
+                               const unsigned char* moving, const unsigned char* exclude, int ntrial, uint64_t seed,
+                               double clash_dist, double* out, int* ok) {
+  const double kPi = 3.14159265358979323846;
+  for (int t = 0; t < ntrial; ++t) {
+    double* x = out + (size_t)t * natom * 3;
+    std::memcpy(x, coords, sizeof(double) * natom * 3);
+    for (int r = 0; r < nrotor; ++r) {
+      const int a = rotors[r * 4 + 0], b = rotors[r * 4 + 1], c = rotors[r * 4 + 2], d = rotors[r * 4 + 3];
+      double b1[3], b2[3], b3[3];
+      for (int k = 0; k < 3; ++k) {
+        b1[k] = x[b * 3 + k] - x[a * 3 + k];
+        b2[k] = x[c * 3 + k] - x[b * 3 + k];
+        b3[k] = x[d * 3 + k] - x[c * 3 + k];
+      }
+      const double n1x = b1[1] * b2[2] - b1[2] * b2[1], n1y = b1[2] * b2[0] - b1[0] * b2[2],
+                   n1z = b1[0] * b2[1] - b1[1] * b2[0];
+      const double n2x = b2[1] * b3[2] - b2[2] * b3[1], n2y = b2[2] * b3[0] - b2[0] * b3[2],
+                   n2z = b2[0] * b3[1] - b2[1] * b3[0];
+      const double l2 = std::sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+      // phi = atan2(b2^ . (n1 x n2), n1 . n2), the IUPAC dihedral
+      const double cx_ = n1y * n2z - n1z * n2y, cy_ = n1z * n2x - n1x * n2z, cz_ = n1x * n2y - n1y * n2x;
+      const double yv = (cx_ * b2[0] + cy_ * b2[1] + cz_ * b2[2]) / l2;
+      const double xv = n1x * n2x + n1y * n2y + n1z * n2z;
+      const double phi = std::atan2(yv, xv);
+      const double theta = -kPi + 2.0 * kPi * orc_torsion_uniform(seed, t, r);
+      const double ang = theta - phi;
+      const double ux = b2[0] / l2, uy = b2[1] / l2, uz = b2[2] / l2;
+      const double ca = std::cos(ang), sa = std::sin(ang);
+      const double cx = x[c * 3 + 0], cy = x[c * 3 + 1], cz = x[c * 3 + 2];
+      for (int i = 0; i < natom; ++i) {
+        if (!moving[(size_t)r * natom + i]) continue;
+        const double vx = x[i * 3 + 0] - cx, vy = x[i * 3 + 1] - cy, vz = x[i * 3 + 2] - cz;
+        const double kd = ux * vx + uy * vy + uz * vz;
+        const double kx = uy * vz - uz * vy, ky = uz * vx - ux * vz, kz = ux * vy - uy * vx;
+        x[i * 3 + 0] = cx + vx * ca + kx * sa + ux * kd * (1.0 - ca);
+        x[i * 3 + 1] = cy + vy * ca + ky * sa + uy * kd * (1.0 - ca);
+        x[i * 3 + 2] = cz + vz * ca + kz * sa + uz * kd * (1.0 - ca);
+      }
+    }
+    int bad = 0;
+    for (int i = 0; i < natom && !bad; ++i)
+      for (int j = i + 1; j < natom; ++j) {
+        if (exclude[(size_t)i * natom + j]) continue;
+        const double dx = x[i * 3 + 0] - x[j * 3 + 0], dy = x[i * 3 + 1] - x[j * 3 + 1], dz = x[i * 3 + 2] - x[j * 3 + 2];
+        if (dx * dx + dy * dy + dz * dz < clash_dist * clash_dist) {
+          bad = 1;
+          break;
+        }
+      }
+    ok[t] = bad ? 0 : 1;
+  }
+}

@@ -270,3 +270,30 @@ def build_grids(xyz, charge, epsilon, radius, probe_radii, n, gmin, h, rc_lj=8.0
     _orc.orc_build_grids(atoms.shape[0], _dp(atoms), pr.shape[0], _dp(pr), int(n), _dp(g), C.c_double(h),
                          C.c_double(rc_lj), C.c_double(rc_el), C.c_double(lj_cap), C.c_double(el_cap), _fp(out))
     return out
+
+
+def chain_rotors(natom):
+    """Rotors, moving masks and 1-2/1-3/1-4 exclusions of a linear chain
+    (bond i-(i+1)): rotor (i-1, i, i+1, i+2) turns atoms i+2.. about i->i+1."""
+    rot = np.array([[i - 1, i, i + 1, i + 2] for i in range(1, natom - 2)], np.int32).reshape(-1, 4)
+    mv = np.zeros((rot.shape[0], natom), np.uint8)
+    for r, (a, b, c, d) in enumerate(rot):
+        mv[r, c + 1:] = 1
+    idx = np.arange(natom)
+    ex = (np.abs(idx[:, None] - idx[None, :]) <= 3).astype(np.uint8)
+    return rot, mv, ex
+
+
+def conformers(coords, rotors, moving, exclude, ntrial, seed, clash_dist):
+    """orc_conformers: trial coordinates [ntrial][natom][3] and keep flags."""
+    x = np.ascontiguousarray(np.asarray(coords, np.float64).reshape(-1, 3))
+    na = x.shape[0]
+    r = np.ascontiguousarray(np.asarray(rotors, np.int32).reshape(-1, 4))
+    mv = np.ascontiguousarray(np.asarray(moving, np.uint8).reshape(r.shape[0], na))
+    ex = np.ascontiguousarray(np.asarray(exclude, np.uint8).reshape(na, na))
+    out = np.empty((int(ntrial), na, 3), np.float64)
+    ok = np.empty(int(ntrial), np.int32)
+    _orc.orc_conformers(na, _dp(x), r.shape[0], _ip(r), mv.ctypes.data_as(C.POINTER(C.c_uint8)),
+                        ex.ctypes.data_as(C.POINTER(C.c_uint8)), int(ntrial), C.c_uint64(seed),
+                        C.c_double(clash_dist), _dp(out), _ip(ok))
+    return out, ok.astype(bool)

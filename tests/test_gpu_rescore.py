@@ -184,3 +184,17 @@ def test_protein_built_on_device_matches_oracle_and_upload(ctx):
     assert np.array_equal(E1.view(np.uint32), E2.view(np.uint32)) and np.array_equal(i1, i2)
     for o in (l1, l2, built, up):
         o.close()
+
+
+def test_conformers_on_device_match_oracle(ctx):
+    """Device random-torsion conformers: the kept trials and their order equal
+    the oracle's; coordinates agree to 1e-9 A (atan2/sin/cos of the device
+    and host math libraries may differ in the last bit)."""
+    say("conformers")
+    x, q, e, r = cdk.synth_ligand(21, 30)
+    rot, mv, ex = orc.chain_rotors(30)
+    conf, trial = cdk.generate_conformers(ctx, x, rot, mv, ex, 300, 5, 1.2, max_keep=50)
+    ref, ok = orc.conformers(x.astype(np.float64), rot, mv, ex, 300, 5, 1.2)
+    want = np.flatnonzero(ok)[:50]
+    assert np.array_equal(trial, want)
+    assert np.abs(conf - ref[want]).max() < 1e-9

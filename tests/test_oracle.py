@@ -315,3 +315,49 @@ def test_grid_builder_oracle_closed_forms():
     far = (np.abs(g) < 1.0) & (np.abs(gb) < 1.0)
     assert np.allclose(g2[far], (g.astype(np.float64) + gb)[far], rtol=1e-5, atol=1e-6)
     assert not np.array_equal(orc.build_grids(xyz, q + 0.01, eps, rad, probes, n, gmin, h), g)
+
+
+def _dihedral(p0, p1, p2, p3):
+    b0, b1, b2 = p0 - p1, p2 - p1, p3 - p2
+    b1n = b1 / np.linalg.norm(b1)
+    v = b0 - np.dot(b0, b1n) * b1n
+    w = b2 - np.dot(b2, b1n) * b1n
+    return np.arctan2(np.dot(np.cross(b1n, v), w), np.dot(v, w))
+
+
+def _torsion_u(seed, t, r):
+    m = (1 << 64) - 1
+    z = (seed + 0x9E3779B97F4A7C15 * ((t * 0x100000001 + r + 1) & m)) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0)
+
+
+def test_conformer_oracle_dihedrals_rigidity_and_filter():
+    """orc_conformers: every rotor ends at its drawn IUPAC dihedral (a numpy
+    dihedral, independent of the oracle's formula), bond lengths and angles
+    are unchanged, and the keep flags equal a brute-force minimum-distance
+    check; another seed gives other conformers."""
+    rs = np.random.default_rng(8)
+    n = 18
+    x = np.zeros((n, 3))
+    for i in range(1, n):  # a bent chain, 1.5 A bonds
+        v = rs.normal(size=3)
+        x[i] = x[i - 1] + 1.5 * v / np.linalg.norm(v)
+    rot, mv, ex = orc.chain_rotors(n)
+    out, ok = orc.conformers(x, rot, mv, ex, 40, 11, 1.3)
+    for t in range(40):
+        for r, (a, b, c, d) in enumerate(rot):
+            th = -np.pi + 2 * np.pi * _torsion_u(11, t, r)
+            err = np.angle(np.exp(1j * (_dihedral(*out[t, [a, b, c, d]]) - th)))
+            assert abs(err) < 1e-9
+        bl0 = np.linalg.norm(np.diff(x, axis=0), axis=1)
+        bl = np.linalg.norm(np.diff(out[t], axis=0), axis=1)
+        assert np.abs(bl - bl0).max() < 1e-12
+        d2 = ((out[t][:, None] - out[t][None]) ** 2).sum(-1)
+        iu = np.triu_indices(n, 1)
+        keep = not ((d2[iu] < 1.3 ** 2) & (ex[iu] == 0)).any()
+        assert keep == ok[t]
+    assert 0 < ok.sum() < 40
+    out2, _ = orc.conformers(x, rot, mv, ex, 40, 12, 1.3)
+    assert not np.allclose(out, out2)
