@@ -85,6 +85,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, 
 __device__ __forceinline__ float2 buffer_load_f2(__amdgpu_buffer_rsrc_t r, int voff) {
   return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
 }
+__device__ __forceinline__ float4 buffer_load_f4(__amdgpu_buffer_rsrc_t r, int voff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+}
 __device__ __forceinline__ void buffer_store_f2(float2 v, __amdgpu_buffer_rsrc_t r, int voff) {
   typedef unsigned u2 __attribute__((ext_vector_type(2)));
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, 0, 0);
@@ -496,6 +499,16 @@ __device__ __forceinline__ int lb_index(int R, int x, int line) {
   return ((line / B) * R + x) * B + (line & (B - 1));  // < 2^31 per (slot, channel)
 }
 
+// The protein copy pairs the rows a lane holds next to each other: row
+// x = n2 + L*n1 of a block sits at ((n1/2)*L + n2)*2B + 2*line_in_block +
+// n1 % 2, so one 16-byte load gives a lane rows n1 = 2p and 2p + 1 of its
+// line and one wavefront load covers 1 KB of consecutive bytes.
+template <int N>
+__device__ __forceinline__ int lbp_index(int x, int line) {
+  constexpr int B = RegShape<N>::LPW, L = RegShape<N>::L, V = RegShape<N>::V;
+  const int n2 = x % L, n1 = x / L;
+  return (((line / B) * (V / 2) + n1 / 2) * L + n2) * (2 * B) + (line & (B - 1)) * 2 + (n1 & 1);
+}
 
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_to_line_blocks(const float2* __restrict__ in, float2* __restrict__ out,
@@ -507,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_to_line_blocks(const float2* __restr
   const int c = (int)(t / per);
   const size_t r = t - (size_t)c * per;
   const int x = (int)(r / NL), line = (int)(r % NL);
-  out[(size_t)c * per + (size_t)lb_index<N>(N, x, line)] = in[t];
+  out[(size_t)c * per + (size_t)lbp_index<N>(x, line)] = in[t];
 }
 
 // Stage A: one workgroup per (pose, ligand channel).  The channel's atoms are
@@ -841,14 +854,14 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   const int s1 = min(nslots, sg * group + group);
   // Ligand planes A and correlation planes T are plane-major: lane offset of
   // row x = n2 of the wavefront's lines, row n1 adds L*n1 planes.  The
-  // protein copy is line-blocked (lb_index): block l0 / LPW, rows of LPW
-  // lines.  Rows past a descriptor's end (x >= box, x' >= window) read as
-  // zero / are dropped.
+  // protein copy is line-blocked with paired rows (lbp_index): block
+  // l0 / LPW, 16 bytes = rows n1 = 2p, 2p+1 per lane.  Rows past a
+  // descriptor's end (x >= box, x' >= window) read as zero / are dropped.
   const size_t blk = (size_t)(l0 / LPW) * LPW;  // protein block start, in rows of LPW lines
   const int voffA = lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone;
   constexpr int kRowA = L * NL * (int)sizeof(float2);
-  const int voffP = lok ? (n2 * LPW + ln) * (int)sizeof(float2) : kVoffNone;
-  constexpr int kRowP = L * LPW * (int)sizeof(float2);
+  const int voffP = lok ? (n2 * 2 * LPW + ln * 2) * (int)sizeof(float2) : kVoffNone;
+  constexpr int kPairP = L * 2 * LPW * (int)sizeof(float2);  // rows n1 = 2p, 2p+1 -> pair p
   // Slot-derived values are wave-uniform; readfirstlane keeps them (and the
   // descriptors built from them) in scalar registers.
   auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
@@ -886,7 +899,11 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
       for (int n1 = 0; n1 < VA; ++n1) ra[n1] = buffer_load_f2(r, voffA + n1 * kRowA);
     }
 #pragma unroll
-    for (int m = 0; m < V; ++m) rp[m] = buffer_load_f2(q, voffP + m * kRowP);
+    for (int m = 0; m < V / 2; ++m) {
+      const float4 f = buffer_load_f4(q, voffP + m * kPairP);
+      rp[2 * m] = make_float2(f.x, f.y);
+      rp[2 * m + 1] = make_float2(f.z, f.w);
+    }
   };
   float2 acc[V];
 #pragma unroll
