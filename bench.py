@@ -212,6 +212,7 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     kernels["tuned"] = {"fwd_variant": ["block", "register"][ctx.get_option("last_fwd_variant")],
                         "inv_variant": ["block", "register"][ctx.get_option("last_inv_variant")],
                         "mid_group": group, "mid_depth": ctx.get_option("last_mid_depth"),
+                        "mid_poses": ctx.get_option("last_mid_poses"),
                         "max_dyn_lds": ctx.get_option("max_dyn_lds")}
     return kernels, roof
 

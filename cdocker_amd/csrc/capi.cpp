@@ -158,6 +158,9 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
     } else if (k == "mid_depth") {
       require(v >= 0 && v <= 3, "mid_depth must be in [0, 3]");
       c.mid_depth = (int)v;
+    } else if (k == "mid_poses") {
+      require(v == 0 || v == 1 || v == 2 || v == 4, "mid_poses must be 0, 1, 2 or 4");
+      c.mid_poses = (int)v;
     } else if (k == "fwd_variant") {
       require(v >= -1 && v <= 1, "fwd_variant must be -1 (tuned), 0 (block) or 1 (register)");
       c.fwd_variant = (int)v;
@@ -193,6 +196,8 @@ int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value) {
     else if (k == "last_mid_group") *value = c.last_group;
     else if (k == "mid_depth") *value = c.mid_depth;
     else if (k == "last_mid_depth") *value = c.last_depth;
+    else if (k == "mid_poses") *value = c.mid_poses;
+    else if (k == "last_mid_poses") *value = c.last_poses;
     else if (k == "max_dyn_lds") *value = (int64_t)c.max_dyn_lds;
     else throw cdk::Error(CDK_ERR_INVALID, "unknown option " + k);
   });

@@ -463,13 +463,13 @@ void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, siz
 }
 
 struct Tuned {
-  int fwd = kFwdBlock, inv = kInvBlock, group = 16, depth = 1;
+  int fwd = kFwdBlock, inv = kInvBlock, group = 16, depth = 1, poses = 1;
 };
 
 // Autotuner: picks the forward and inverse plane forms (once per grid size,
-// among the forms that fit every batch of that size) and the mid-stage group
-// size (per launch shape) by timing them on the first launch set of this
-// search.
+// among the forms that fit every batch of that size) and the mid-stage
+// schedule (group size, ring depth, poses per protein load; per launch shape)
+// by timing them on the first launch set of this search.
 Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   const int N = prot.N;
   Tuned tu;
@@ -500,14 +500,17 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       ctx.tuned_fwd[N] = tu.fwd = kFwdBlock;
   }
   const int max_depth = mid_max_depth(N);
+  const int max_poses = mid_max_poses(N);
   if (ctx.mid_group > 0) {
     tu.group = ctx.mid_group;
     tu.depth = ctx.mid_depth > 0 ? std::min(ctx.mid_depth, max_depth) : std::min(2, max_depth);
+    tu.poses = ctx.mid_poses > 0 ? std::min(ctx.mid_poses, max_poses) : 1;
   } else {
     auto it = ctx.tuned_group.find(gkey);
     if (it != ctx.tuned_group.end()) {
-      tu.group = it->second.first;
-      tu.depth = it->second.second;
+      tu.group = it->second[0];
+      tu.depth = it->second[1];
+      tu.poses = it->second[2];
     } else {
       need_group = true;
     }
@@ -554,19 +557,21 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       fwd(tu.fwd);
     }
     if (need_group) {
-      auto mid = [&](int g, int depth) {
+      auto mid = [&](int g, int depth, int np) {
         return timed([&] {
-          hip_check(launch_mid(ctx.stream, N, depth, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax,
+          hip_check(launch_mid(ctx.stream, N, depth, np, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax,
                                t.boxmax, prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(),
                                1.0f / (float)prot.G(), tw),
                     "tune mid");
         });
       };
-      // Group size at the shallowest ring, then the ring depth at that group.
+      // One pose at a time: group size at the shallowest ring, then the ring
+      // depth at that group.  Then runs of 2 and 4 poses per protein load
+      // over the group sizes.
       const int d0 = std::min(2, max_depth);
       float best = 1e30f;
       for (int g : {4, 8, 16, 32}) {
-        const float ms = mid(g, d0);
+        const float ms = mid(g, d0, 1);
         if (ms < best) {
           best = ms;
           tu.group = g;
@@ -574,13 +579,23 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       }
       tu.depth = d0;
       for (int d = d0 + 1; d <= max_depth; ++d) {
-        const float ms = mid(tu.group, d);
+        const float ms = mid(tu.group, d, 1);
         if (ms < best) {
           best = ms;
           tu.depth = d;
         }
       }
-      ctx.tuned_group[gkey] = {tu.group, tu.depth};
+      tu.poses = 1;
+      for (int np = 2; np <= max_poses; np *= 2)
+        for (int g : {4, 8, 16, 32}) {
+          const float ms = mid(g, tu.depth, np);
+          if (ms < best) {
+            best = ms;
+            tu.group = g;
+            tu.poses = np;
+          }
+        }
+      ctx.tuned_group[gkey] = {tu.group, tu.depth, tu.poses};
     }
     if (need_inv) {
       auto inv = [&](int variant) {
@@ -631,7 +646,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
               "fwd planes");
     timer.end(e);
     e = timer.begin(2);
-    hip_check(launch_mid(ctx.stream, N, tu.depth, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), t.cmax,
+    hip_check(launch_mid(ctx.stream, N, tu.depth, tu.poses, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), t.cmax,
                          t.boxmax,
                          prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
               "mid");
@@ -702,9 +717,11 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
     ctx.last_inv = tu.inv;
     ctx.last_group = tu.group;
     ctx.last_depth = tu.depth;
+    ctx.last_poses = tu.poses;
     if (use_graph && !ctx.time_stages) {
       if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.group ||
-          t.graph_fwd != tu.fwd || t.graph_inv != tu.inv || t.graph_depth != tu.depth) {
+          t.graph_fwd != tu.fwd || t.graph_inv != tu.inv || t.graph_depth != tu.depth ||
+          t.graph_poses != tu.poses) {
         t.drop_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -725,6 +742,7 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
         t.graph_fwd = tu.fwd;
         t.graph_inv = tu.inv;
         t.graph_depth = tu.depth;
+        t.graph_poses = tu.poses;
       }
       hip_check(hipGraphLaunch(t.graph, ctx.stream), "hipGraphLaunch");
     } else {

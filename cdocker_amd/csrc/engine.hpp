@@ -138,7 +138,7 @@ struct Prepared {
   hipGraphExec_t graph = nullptr;
   uint64_t graph_gen = 0;
   size_t graph_chunk = 0;
-  int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0;
+  int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0;
   std::vector<PoseResults> results;
   Prepared() = default;
   Prepared(const Prepared&) = delete;
@@ -158,6 +158,7 @@ struct Context {
   int fused_chunk = 0;            // 0 = from budget
   int mid_group = 0;              // poses per mid-stage workgroup; 0 = tuned
   int mid_depth = 0;              // mid-stage register ring depth with a fixed group; 0 = 2
+  int mid_poses = 0;              // poses sharing each protein load in the mid stage with a fixed group; 0 = 1
   int fwd_variant = -1;           // forward plane stage form; -1 = tuned
   int inv_variant = -1;           // inverse plane stage form; -1 = tuned
   bool use_graph = true;
@@ -166,12 +167,13 @@ struct Context {
   // rounding, so each is chosen once per grid size and kept for the
   // context's life (results then do not depend on how poses are batched);
   // the mid group size only changes the schedule and is tuned per launch
-  // shape.
+  // shape, with the ring depth and the poses per protein load.
   size_t max_dyn_lds = 64 * 1024;
   std::map<int, int> tuned_fwd;                      // N -> variant
   std::map<int, int> tuned_inv;                      // N -> variant
-  std::map<std::array<int, 4>, std::pair<int, int>> tuned_group;  // (N, cmax, boxmax, chunk) -> (group, depth)
-  int last_fwd = -1, last_inv = -1, last_group = 0, last_depth = 0;
+  // (N, cmax, boxmax, chunk) -> (group, depth, poses per protein load)
+  std::map<std::array<int, 4>, std::array<int, 3>> tuned_group;
+  int last_fwd = -1, last_inv = -1, last_group = 0, last_depth = 0, last_poses = 0;
   StageTimes stage_times;
   // per-launch-set working arrays; ws_generation counts reallocations so a
   // captured graph that baked in the old addresses is rebuilt

@@ -18,15 +18,31 @@
 
 namespace cdk {
 
+// Every multiply in the transforms is written here, with contraction off and
+// the fused multiply-adds spelled out (fmaf).  The compiler then has no
+// multiply it may fuse on its own, so a transform rounds the same way in
+// every kernel that inlines it, whatever code surrounds it.
 __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 c_scale(float2 a, float s) {
+#pragma clang fp contract(off)
+  return make_float2(a.x * s, a.y * s);
+}
+// b + a*s and b - a*s, one rounding each
+__device__ __forceinline__ float2 c_fma_s(float2 a, float s, float2 b) {
+  return make_float2(fmaf(a.x, s, b.x), fmaf(a.y, s, b.y));
+}
+__device__ __forceinline__ float2 c_fnma_s(float2 a, float s, float2 b) {
+  return make_float2(fmaf(-a.x, s, b.x), fmaf(-a.y, s, b.y));
+}
 __device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+#pragma clang fp contract(off)
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 // conj(a) * b
 __device__ __forceinline__ float2 c_conjmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+#pragma clang fp contract(off)
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.x, b.y, -(a.y * b.x)));
 }
 // multiply by S*i  (S = +1 or -1)
 template <int S>
@@ -53,11 +69,11 @@ template <int S> struct SmallDFT<3, S> {
     float2 a = v[0];
     float2 t = c_add(v[1], v[2]);
     float2 d = c_sub(v[1], v[2]);
-    float2 m = make_float2(a.x - 0.5f * t.x, a.y - 0.5f * t.y);
-    float2 r = c_scale(c_muli<S>(d), kS3);
+    const float2 m = c_fnma_s(t, 0.5f, a);
+    const float2 r = c_muli<S>(d);
     v[0] = c_add(a, t);
-    v[1] = c_add(m, r);
-    v[2] = c_sub(m, r);
+    v[1] = c_fma_s(r, kS3, m);
+    v[2] = c_fnma_s(r, kS3, m);
   }
 };
 
@@ -80,17 +96,18 @@ template <int S> struct SmallDFT<8, S> {
     SmallDFT<4, S>::run(e);
     SmallDFT<4, S>::run(o);
     // w8^1 = (1 + S i)/sqrt2, w8^2 = S i, w8^3 = (-1 + S i)/sqrt2
-    float2 o1 = c_scale(c_add(o[1], c_muli<S>(o[1])), kR);
-    float2 o2 = c_muli<S>(o[2]);
-    float2 o3 = c_scale(c_sub(c_muli<S>(o[3]), o[3]), kR);
+    // o1, o3 below are the products before the 1/sqrt2 scale
+    const float2 o1 = c_add(o[1], c_muli<S>(o[1]));
+    const float2 o2 = c_muli<S>(o[2]);
+    const float2 o3 = c_sub(c_muli<S>(o[3]), o[3]);
     v[0] = c_add(e[0], o[0]);
     v[4] = c_sub(e[0], o[0]);
-    v[1] = c_add(e[1], o1);
-    v[5] = c_sub(e[1], o1);
+    v[1] = c_fma_s(o1, kR, e[1]);
+    v[5] = c_fnma_s(o1, kR, e[1]);
     v[2] = c_add(e[2], o2);
     v[6] = c_sub(e[2], o2);
-    v[3] = c_add(e[3], o3);
-    v[7] = c_sub(e[3], o3);
+    v[3] = c_fma_s(o3, kR, e[3]);
+    v[7] = c_fnma_s(o3, kR, e[3]);
   }
 };
 

@@ -108,17 +108,18 @@ template <int S> struct RegDFTHalf<8, S> {
     const float2 ib0 = c_muli<S>(v[2]), ib1 = c_muli<S>(v[3]);
     const float2 e0 = c_add(v[0], v[2]), e1 = c_add(v[0], ib0), e2 = c_sub(v[0], v[2]), e3 = c_sub(v[0], ib0);
     const float2 o0 = c_add(v[1], v[3]), oa = c_add(v[1], ib1), o2r = c_sub(v[1], v[3]), ob = c_sub(v[1], ib1);
-    const float2 o1 = c_scale(c_add(oa, c_muli<S>(oa)), kR);
+    // o1, o3 are the products before the 1/sqrt2 scale
+    const float2 o1 = c_add(oa, c_muli<S>(oa));
     const float2 o2 = c_muli<S>(o2r);
-    const float2 o3 = c_scale(c_sub(c_muli<S>(ob), ob), kR);
+    const float2 o3 = c_sub(c_muli<S>(ob), ob);
     v[0] = c_add(e0, o0);
     v[4] = c_sub(e0, o0);
-    v[1] = c_add(e1, o1);
-    v[5] = c_sub(e1, o1);
+    v[1] = c_fma_s(o1, kR, e1);
+    v[5] = c_fnma_s(o1, kR, e1);
     v[2] = c_add(e2, o2);
     v[6] = c_sub(e2, o2);
-    v[3] = c_add(e3, o3);
-    v[7] = c_sub(e3, o3);
+    v[3] = c_fma_s(o3, kR, e3);
+    v[7] = c_fnma_s(o3, kR, e3);
   }
 };
 

@@ -934,7 +934,7 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
     }
     if (it.c + 1 == it.nch) {
 #pragma unroll
-      for (int i = 0; i < V; ++i) acc[i] = make_float2(acc[i].x * invG, acc[i].y * invG);
+      for (int i = 0; i < V; ++i) acc[i] = c_scale(acc[i], invG);
       reg_fft<N, 1>(acc, twl, lds, ln, n2);
       const int wx = uni(meta[it.s].win[0]);
       const __amdgpu_buffer_rsrc_t rt =
@@ -972,6 +972,177 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
       advance(nx);
       it[d] = nx;
       if (nx.s < s1) load(nx, ab[d], pb[d]);
+    }
+  }
+}
+
+// Stage B with the protein spectra shared by several poses: the wavefront
+// takes a run of up to NP consecutive valid slots of one ligand (same
+// channels, same protein channel map) and walks channels in the outer loop,
+// so one protein load serves NP poses.  Each pose keeps its own accumulator
+// and sums its channels in the same order with the same operations as
+// k_mid, so the two kernels give bit-identical results.  A ring of two
+// steps (a step = one channel of one run: the protein rows plus NP ligand
+// row sets) keeps the next step's loads in flight during the current
+// step's NP transforms.  Lines of at most 8 values per lane only.
+template <class P, int NP, bool BOXHALF>
+__global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid_np(const PoseSlot* __restrict__ slots,
+                                                   const LigandDesc* __restrict__ ligs,
+                                                   const RotMeta* __restrict__ meta, int nslots, int group,
+                                                   const float2* __restrict__ A, int cmax, int boxmax,
+                                                   const float2* __restrict__ Phat, float2* __restrict__ T,
+                                                   float invG, const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
+  static_assert(V <= 8, "k_mid_np keeps NP accumulators in registers: short lines only");
+  constexpr int NL = N * ZH;
+  constexpr int LPB = 4 * LPW;
+  constexpr int NLG = (NL + LPB - 1) / LPB;
+  constexpr int PERX = (NLG + 7) / 8;
+  constexpr int VA = BOXHALF ? V / 2 : V;
+  __shared__ float2 sbuf[4][RS::SLICE];
+  const int w = blockIdx.x;
+  const int lg = (w & 7) * PERX + ((w >> 3) % PERX);
+  const int sg = (w >> 3) / PERX;
+  if (lg >= NLG) return;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int ln = lane % LPW, n2 = lane / LPW;
+  const int l0 = __builtin_amdgcn_readfirstlane(lg * LPB + wave * LPW);
+  if (l0 >= NL) return;
+  const bool lok = l0 + ln < NL;
+  const size_t plane = (size_t)NL;
+  float2 twr[V];
+  reg_twiddles<N>(twg, n2, twr);
+  float2* lds = sbuf[wave];
+  const int s1 = min(nslots, sg * group + group);
+  const size_t blk = (size_t)(l0 / LPW) * LPW;
+  const int voffA = lok ? (n2 * NL + ln) * (int)sizeof(float2) : kVoffNone;
+  constexpr int kRowA = L * NL * (int)sizeof(float2);
+  const int voffP = lok ? (n2 * 2 * LPW + ln * 2) * (int)sizeof(float2) : kVoffNone;
+  constexpr int kPairP = L * 2 * LPW * (int)sizeof(float2);
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  auto next_valid = [&](int s) {
+    while (s < s1 && !uni(meta[s].valid)) ++s;
+    return s;
+  };
+  // Channel c of a run of np <= NP slots s[0..np) of ligand lig; np == 0 ends
+  // the sequence.  next = the first slot after the run.
+  struct Step {
+    int s[NP], bx[NP];
+    int np, c, nch, lig, next;
+  };
+  auto run_from = [&](Step& st, int s) {
+    st.c = 0;
+    st.np = 0;
+    s = next_valid(s);
+    st.next = s;
+    if (s >= s1) return;
+    st.lig = uni(slots[s].lig);
+    st.nch = uni(ligs[st.lig].nch);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      st.s[p] = s;
+      st.bx[p] = 0;
+      if (st.np == p && s < s1 && uni(slots[s].lig) == st.lig) {
+        st.bx[p] = uni(min(meta[s].box[0], boxmax));
+        st.np = p + 1;
+        s = next_valid(s + 1);
+      }
+    }
+    st.next = s;
+  };
+  auto advance = [&](Step& st) {
+    if (st.np > 0 && ++st.c >= st.nch) run_from(st, st.next);
+  };
+  auto load = [&](const Step& st, float2 (*ra)[VA], float2* rp) {
+    const int pc = uni(ligs[st.lig].pmap[st.c]);
+    const __amdgpu_buffer_rsrc_t q =
+        buffer_rsrc(Phat + (size_t)pc * N * plane + blk * N, (unsigned)(N * LPW * sizeof(float2)));
+#pragma unroll
+    for (int m = 0; m < V / 2; ++m) {
+      const float4 f = buffer_load_f4(q, voffP + m * kPairP);
+      rp[2 * m] = make_float2(f.x, f.y);
+      rp[2 * m + 1] = make_float2(f.z, f.w);
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      if (p < st.np) {
+        const __amdgpu_buffer_rsrc_t r =
+            buffer_rsrc(A + (((size_t)st.s[p] * cmax + st.c) * boxmax) * plane + l0,
+                        (unsigned)(st.bx[p] * NL * sizeof(float2)));
+        if (BOXHALF || st.bx[p] <= N / 2) {
+#pragma unroll
+          for (int n1 = 0; n1 < V / 2; ++n1) ra[p][n1] = buffer_load_f2(r, voffA + n1 * kRowA);
+        } else {
+#pragma unroll
+          for (int n1 = 0; n1 < VA; ++n1) ra[p][n1] = buffer_load_f2(r, voffA + n1 * kRowA);
+        }
+      }
+    }
+  };
+  float2 acc[NP][V];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int i = 0; i < V; ++i) acc[p][i] = make_float2(0.f, 0.f);
+  auto consume = [&](const Step& st, float2 (*ra)[VA], const float2* rp) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      if (p < st.np) {
+        float2 v[V];
+#pragma unroll
+        for (int i = 0; i < VA; ++i) v[i] = ra[p][i];
+        if (BOXHALF || st.bx[p] <= N / 2)
+          reg_fft<N, -1, true>(v, twr, lds, ln, n2);
+        else
+          reg_fft<N, -1, false>(v, twr, lds, ln, n2);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          const float2 l = v[i], q = rp[i];
+          acc[p][i].x = fmaf(l.x, q.x, acc[p][i].x);
+          acc[p][i].x = fmaf(l.y, q.y, acc[p][i].x);
+          acc[p][i].y = fmaf(l.x, q.y, acc[p][i].y);
+          acc[p][i].y = fmaf(-l.y, q.x, acc[p][i].y);
+        }
+      }
+    }
+    if (st.c + 1 == st.nch) {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        if (p < st.np) {
+#pragma unroll
+          for (int i = 0; i < V; ++i) acc[p][i] = c_scale(acc[p][i], invG);
+          reg_fft<N, 1>(acc[p], twr, lds, ln, n2);
+          const int wx = uni(meta[st.s[p]].win[0]);
+          const __amdgpu_buffer_rsrc_t rt =
+              buffer_rsrc(T + (size_t)st.s[p] * N * plane + l0, (unsigned)(max(wx, 0) * NL * sizeof(float2)));
+#pragma unroll
+          for (int m = 0; m < V; ++m) buffer_store_f2(acc[p][m], rt, voffA + m * kRowA);
+        }
+#pragma unroll
+        for (int i = 0; i < V; ++i) acc[p][i] = make_float2(0.f, 0.f);
+      }
+    }
+  };
+  Step st[2];
+  run_from(st[0], sg * group);
+  if (st[0].np == 0) return;
+  st[1] = st[0];
+  advance(st[1]);
+  float2 ab[2][NP][VA], pb[2][V];
+  load(st[0], ab[0], pb[0]);
+  if (st[1].np > 0) load(st[1], ab[1], pb[1]);
+  for (;;) {
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      if (st[d].np == 0) return;  // steps are in order: nothing after it is left
+      consume(st[d], ab[d], pb[d]);
+      Step nx = st[d ^ 1];
+      advance(nx);
+      st[d] = nx;
+      if (nx.np > 0) load(nx, ab[d], pb[d]);
     }
   }
 }
@@ -1419,18 +1590,45 @@ void mid_launch_one(dim3 grid, hipStream_t st, const PoseSlot* slots, const Liga
   hipLaunchKernelGGL((k_mid<PL, D, H>), grid, dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, A, cmax, boxmax,
                      Phat, T, invG, tw);
 }
+template <class PL, int NP, bool H>
+void mid_np_launch_one(dim3 grid, hipStream_t st, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
+                       int nslots, int group, const float2* A, int cmax, int boxmax, const float2* Phat, float2* T,
+                       float invG, const float2* tw) {
+  hipLaunchKernelGGL((k_mid_np<PL, NP, H>), grid, dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, A, cmax,
+                     boxmax, Phat, T, invG, tw);
+}
 }  // namespace
 
 int mid_max_depth(int N) { return (N == 32 || N == 64) ? 3 : 1; }
+int mid_max_poses(int N) { return (N == 32 || N == 64) ? 4 : 1; }
 
-hipError_t launch_mid(hipStream_t st, int N, int depth, const PoseSlot* slots, const LigandDesc* ligs,
+hipError_t launch_mid(hipStream_t st, int N, int depth, int np, const PoseSlot* slots, const LigandDesc* ligs,
                       const RotMeta* meta, int nslots, int group, const float2* A, int cmax, int boxmax,
                       const float2* Phat, int npch, float2* T, float invG, const float2* tw) {
   (void)npch;
   if (nslots <= 0) return hipSuccess;
+  if (group <= 0 || np < 1 || np > mid_max_poses(N) || (np & (np - 1)) != 0) return hipErrorInvalidValue;
   const unsigned nsg = ceil_div(nslots, group);
   const dim3 grid((unsigned)mid_line_groups(N) * nsg);
   const bool half = 2 * boxmax <= N;  // every valid pose's box fits in N/2
+  if (np > 1) {
+    CDK_DISPATCH_N(N, {
+      if constexpr (RegShape<PL::N>::V <= 8) {
+        if (np == 4) {
+          if (half)
+            mid_np_launch_one<PL, 4, true>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+          else
+            mid_np_launch_one<PL, 4, false>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+        } else {
+          if (half)
+            mid_np_launch_one<PL, 2, true>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+          else
+            mid_np_launch_one<PL, 2, false>(grid, st, slots, ligs, meta, nslots, group, A, cmax, boxmax, Phat, T, invG, tw);
+        }
+      }
+    });
+    return hipGetLastError();
+  }
   CDK_DISPATCH_N(N, {
     if constexpr (RegShape<PL::N>::V <= 8) {
       if (depth >= 3) {

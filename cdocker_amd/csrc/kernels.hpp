@@ -43,10 +43,14 @@ hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot*
                              const LigandDesc* ligs, const int* chan_atoms, const float* wv, const float* wq,
                              const AtomGrid* ag, int astride, const RotMeta* meta, int cmax, int boxmax,
                              int max_chan_atoms, float2* A, const float2* tw);
-// Mid stage.  depth = register buffers in the (pose, channel) ring, 2..
-// mid_max_depth(N) (lines of > 8 values per lane use one buffer).
+// Mid stage.  np = 1: one pose per wavefront at a time, depth = register
+// buffers in the (pose, channel) ring, 2..mid_max_depth(N) (lines of > 8
+// values per lane use one buffer).  np = 2 or 4 (<= mid_max_poses(N)): runs
+// of np same-ligand poses share each protein load (depth unused).  Every
+// (depth, np, group) gives bit-identical results.
 int mid_max_depth(int N);
-hipError_t launch_mid(hipStream_t st, int N, int depth, const PoseSlot* slots, const LigandDesc* ligs,
+int mid_max_poses(int N);
+hipError_t launch_mid(hipStream_t st, int N, int depth, int np, const PoseSlot* slots, const LigandDesc* ligs,
                       const RotMeta* meta, int nslots, int group, const float2* A, int cmax, int boxmax,
                       const float2* Phat, int npch, float2* T, float invG, const float2* tw);
 // Inverse plane stage variants: one workgroup per (pose, window plane) with

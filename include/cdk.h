@@ -57,7 +57,9 @@ int cdk_context_destroy(cdk_context* ctx);
  * "fused_chunk" (poses per fused launch set, 0 = from budget), "mid_group"
  * (poses per mid-stage workgroup, 0 = autotuned with the ring depth),
  * "mid_depth" (register buffers in the mid stage's (pose, channel) ring when
- * mid_group is fixed: 2 or 3 at 32^3/64^3, 0 = 2), "fwd_variant" (forward
+ * mid_group is fixed: 2 or 3 at 32^3/64^3, 0 = 2), "mid_poses" (poses of
+ * one ligand that share each protein spectrum load when mid_group is fixed:
+ * 1, or 2 or 4 at 32^3/64^3; 0 = 1), "fwd_variant" (forward
  * plane stage: 0 block-wide LDS Stockham FFTs, 1 register four-step FFTs,
  * -1 = autotuned once per grid size), "inv_variant" (inverse
  * plane stage + argmin: 0 block-wide LDS FFTs per plane, 1 register FFTs over
@@ -65,7 +67,8 @@ int cdk_context_destroy(cdk_context* ctx);
  * "time_stages". */
 int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t value);
 /* Reads an option back; also "last_fwd_variant", "last_inv_variant",
- * "last_mid_group" and "last_mid_depth" (what the autotuner chose for the
+ * "last_mid_group", "last_mid_depth" and "last_mid_poses" (what the
+ * autotuner chose for the
  * last fused search) and "max_dyn_lds" (the
  * device's dynamic LDS limit per workgroup, bytes).  No reference
  * counterpart. */

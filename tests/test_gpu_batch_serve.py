@@ -69,9 +69,20 @@ def test_chunking_does_not_change_results(ctx, prots):
     assert same(E0, E1) and np.array_equal(i0, i1)
 
 
+def _fixed_mid(ctx, grp, depth, poses):
+    ctx.set_option("mid_group", grp)
+    ctx.set_option("mid_depth", depth)
+    ctx.set_option("mid_poses", poses)
+
+
+def _tuned_mid(ctx):
+    _fixed_mid(ctx, 0, 0, 0)
+
+
 def test_mid_schedule_does_not_change_results(ctx, prots):
-    """Mid-stage group size and register ring depth (what the autotuner picks
-    per launch shape) only reorder work: results are bit-identical."""
+    """Mid-stage group size, register ring depth and poses per protein load
+    (what the autotuner picks per launch shape) only reorder work: results
+    are bit-identical."""
     say("mid schedule invariance")
     g, prot = prots[1]
     w = workloads.config2(nconf=3, nquat=40)
@@ -79,15 +90,47 @@ def test_mid_schedule_does_not_change_results(ctx, prots):
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
     try:
-        for grp, depth in ((4, 2), (8, 3), (32, 2), (5, 3)):
-            ctx.set_option("mid_group", grp)
-            ctx.set_option("mid_depth", depth)
+        # group 5 and 3 split pose runs at group edges; 1 pose per group
+        # leaves the run one pose long
+        for grp, depth, poses in ((4, 2, 1), (8, 3, 1), (32, 2, 1), (5, 3, 1), (4, 2, 2), (8, 2, 4), (5, 2, 2),
+                                  (3, 2, 4), (1, 2, 4), (32, 2, 4)):
+            _fixed_mid(ctx, grp, depth, poses)
             E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
             assert ctx.get_option("last_mid_depth") == depth
-            assert same(E0, E1) and np.array_equal(i0, i1), (grp, depth)
+            assert ctx.get_option("last_mid_poses") == poses
+            assert same(E0, E1) and np.array_equal(i0, i1), (grp, depth, poses)
     finally:
-        ctx.set_option("mid_group", 0)
-        ctx.set_option("mid_depth", 0)
+        _tuned_mid(ctx)
+
+
+def test_mid_pose_runs_in_mixed_batches(ctx, prots):
+    """Runs of poses sharing a protein load stop at ligand boundaries and skip
+    poses whose box does not fit: a batch of uneven ligands gives the same
+    bits with 1, 2 or 4 poses per load."""
+    say("mid pose runs, mixed batch")
+    g, prot = prots[1]
+    w = workloads.config3(nlig=5, nquat=7, conf_range=(1, 3), atom_range=(15, 60))
+    ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
+
+    def run():
+        b = cdk.Batch(ctx, prot)
+        for lig, L in zip(ligs, w["ligands"]):
+            b.add(lig, L["conformers"], L["quats"])
+        b.run(cdk.ENGINE_FUSED)
+        out = [b.results(j)[:2] for j in range(len(ligs))]
+        b.close()
+        return out
+
+    try:
+        _fixed_mid(ctx, 16, 2, 1)
+        base = run()
+        for grp, poses in ((16, 2), (16, 4), (6, 4), (2, 2)):
+            _fixed_mid(ctx, grp, 2, poses)
+            got = run()
+            for (E0, i0), (E1, i1) in zip(base, got):
+                assert same(E0, E1) and np.array_equal(i0, i1), (grp, poses)
+    finally:
+        _tuned_mid(ctx)
 
 
 def test_server_alone_equals_in_mix(ctx, prots):
