@@ -47,29 +47,44 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def fused_kernel_bytes(geoms, npch, N, mid_launches):
+def fused_kernel_bytes(geoms, npch, N, mid_launches, columns=False):
     """Algorithmic HBM bytes of each fused kernel over a set of poses.
 
     geoms: [(box, win, nch)] per job, box/win [..., 3] per pose, nch the
     job's channel count.  plane = N*(N/2+1)*8 bytes (one complex
-    half-spectrum plane).
+    half-spectrum plane), row = (N/2+1)*8 bytes (one z half-spectrum row).
+    Plane form:
       forward planes: writes nch*box_x planes per pose (the pruned y/z spectra)
       mid lines:      reads those planes, writes win_x planes (x-inverse done),
                       and reads the protein spectra (npch*H*8) once per launch
       inverse+argmin: reads win_x planes per pose, writes one 8-byte key
+    Column form:
+      forward z:      writes nch*box_x*box_y rows per pose (z spectra)
+      mid kz:         reads those rows, writes win_x*win_y rows (x and y
+                      inverses done), reads the protein spectra once per launch
+      inverse z:      reads win_x*win_y rows per pose, writes one 8-byte key
     """
     plane = N * (N // 2 + 1) * 8
+    row = (N // 2 + 1) * 8
     H8 = N * N * (N // 2 + 1) * 8
     fwd = mid = inv = 0.0
     for box, win, nch in geoms:
-        bx = box[..., 0].astype(np.float64).ravel()
-        wx = win[..., 0].astype(np.float64).ravel()
-        valid = (win.reshape(-1, 3) > 0).all(axis=1)
-        bx, wx = bx[valid], wx[valid]
-        fwd += float((nch * bx * plane).sum())
-        mid += float((nch * bx * plane + wx * plane).sum())
-        inv += float((wx * plane + 8).sum())
+        b = box.reshape(-1, 3).astype(np.float64)
+        w = win.reshape(-1, 3).astype(np.float64)
+        valid = (w > 0).all(axis=1)
+        b, w = b[valid], w[valid]
+        if columns:
+            lig = nch * b[:, 0] * b[:, 1] * row
+            out = w[:, 0] * w[:, 1] * row
+        else:
+            lig = nch * b[:, 0] * plane
+            out = w[:, 0] * plane
+        fwd += float(lig.sum())
+        mid += float((lig + out).sum())
+        inv += float((out + 8).sum())
     mid += npch * H8 * mid_launches
+    if columns:
+        return {"prep": 0.0, "fwd_z": fwd, "mid_kz": mid, "inv_z": inv}
     return {"prep": 0.0, "fwd_planes": fwd, "mid_lines": mid, "inv_argmin": inv}
 
 
@@ -193,8 +208,9 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     ms, launches = ctx.stage_times(reset=True)
     ctx.set_option("time_stages", 0)
     group = ctx.get_option("last_mid_group")
-    names = ["prep", "fwd_planes", "mid_lines", "inv_argmin"]
-    kb = fused_kernel_bytes(geoms, npch, N, int(launches[2]))
+    columns = ctx.get_option("last_pipeline") == 1
+    names = ["prep", "fwd_z", "mid_kz", "inv_z"] if columns else ["prep", "fwd_planes", "mid_lines", "inv_argmin"]
+    kb = fused_kernel_bytes(geoms, npch, N, int(launches[2]), columns)
     kernels = {}
     for i, nm in enumerate(names):
         if launches[i]:
@@ -209,11 +225,15 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
             "algorithmic_bytes_per_launch": round(dom_bytes)}
-    kernels["tuned"] = {"fwd_variant": ["block", "register"][ctx.get_option("last_fwd_variant")],
-                        "inv_variant": ["block", "register"][ctx.get_option("last_inv_variant")],
-                        "mid_group": group, "mid_depth": ctx.get_option("last_mid_depth"),
-                        "mid_poses": ctx.get_option("last_mid_poses"),
-                        "max_dyn_lds": ctx.get_option("max_dyn_lds")}
+    if columns:
+        kernels["tuned"] = {"pipeline": "columns", "mid_group": group, "max_dyn_lds": ctx.get_option("max_dyn_lds")}
+    else:
+        kernels["tuned"] = {"pipeline": "planes",
+                            "fwd_variant": ["block", "register"][ctx.get_option("last_fwd_variant")],
+                            "inv_variant": ["block", "register"][ctx.get_option("last_inv_variant")],
+                            "mid_group": group, "mid_depth": ctx.get_option("last_mid_depth"),
+                            "mid_poses": ctx.get_option("last_mid_poses"),
+                            "max_dyn_lds": ctx.get_option("max_dyn_lds")}
     return kernels, roof
 
 
@@ -299,7 +319,7 @@ def run_single(args, cdk, workloads):
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
-                      inv_variant=args.inv)
+                      inv_variant=args.inv, pipeline=args.pipeline)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     nu, used, _ = lig.channels()
@@ -366,7 +386,7 @@ def run_batched(args, cdk, workloads):
     w = workloads.config3()
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
-                      inv_variant=args.inv)
+                      inv_variant=args.inv, pipeline=args.pipeline)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     N = g["n"]
     ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
@@ -420,7 +440,7 @@ def run_large(args, cdk, workloads):
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
-                      inv_variant=args.inv)
+                      inv_variant=args.inv, pipeline=args.pipeline)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     C = lig.channels()[0] + 1
@@ -499,6 +519,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
     ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
+    ap.add_argument("--pipeline", type=int, default=-1,
+                    help="fused engine form: 0 planes, 1 columns (64^3), -1 autotuned")
     ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--inv", type=int, default=-1, help="inverse plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving leg (0 = skip)")

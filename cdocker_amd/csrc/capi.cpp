@@ -161,6 +161,9 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
     } else if (k == "mid_poses") {
       require(v == 0 || v == 1 || v == 2 || v == 4, "mid_poses must be 0, 1, 2 or 4");
       c.mid_poses = (int)v;
+    } else if (k == "pipeline") {
+      require(v >= -1 && v <= 1, "pipeline must be -1 (tuned), 0 (plane form) or 1 (column form)");
+      c.pipeline = (int)v;
     } else if (k == "fwd_variant") {
       require(v >= -1 && v <= 1, "fwd_variant must be -1 (tuned), 0 (block) or 1 (register)");
       c.fwd_variant = (int)v;
@@ -197,6 +200,8 @@ int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value) {
     else if (k == "mid_depth") *value = c.mid_depth;
     else if (k == "last_mid_depth") *value = c.last_depth;
     else if (k == "mid_poses") *value = c.mid_poses;
+    else if (k == "pipeline") *value = c.pipeline;
+    else if (k == "last_pipeline") *value = c.last_pipe;
     else if (k == "last_mid_poses") *value = c.last_poses;
     else if (k == "max_dyn_lds") *value = (int64_t)c.max_dyn_lds;
     else throw cdk::Error(CDK_ERR_INVALID, "unknown option " + k);
@@ -289,7 +294,7 @@ int cdk_protein_spectrum(cdk_protein* p, int channel, float* out) {
 int cdk_protein_bytes(cdk_protein* p, int64_t* bytes) {
   return guard([&] {
     require(p && bytes, "NULL argument");
-    *bytes = (int64_t)(p->p.grid.cap + p->p.spec.cap + p->p.spec_lb.cap);
+    *bytes = (int64_t)(p->p.grid.cap + p->p.spec.cap + p->p.spec_lb.cap + p->p.spec_kz.cap);
   });
 }
 

@@ -108,8 +108,8 @@ void protein_setup(Protein& p, int nch, int N, float h, const float* mid, const 
   p.grid.ensure((size_t)nch * p.G() * sizeof(float));
 }
 
-// Forward R2C of every channel (NoMini:177-199) plus the line-blocked copy
-// the fused mid stage reads.
+// Forward R2C of every channel (NoMini:177-199) plus the copies the fused
+// mid stages read (line-blocked; kz-plane blocks at grid edge 64).
 void protein_spectra(Protein& p) {
   Context& ctx = *p.ctx;
   const size_t H = p.H();
@@ -119,6 +119,11 @@ void protein_spectra(Protein& p) {
   p.spec_lb.ensure((size_t)p.nch * H * sizeof(float2));
   hip_check(launch_to_line_blocks(ctx.stream, p.N, p.spec.as<float2>(), p.spec_lb.as<float2>(), p.nch),
             "protein spectra line blocks");
+  if (col_supported(p.N)) {
+    p.spec_kz.ensure((size_t)p.nch * H * sizeof(float2));
+    hip_check(launch_to_kz_blocks(ctx.stream, p.N, p.spec.as<float2>(), p.spec_kz.as<float2>(), p.nch),
+              "protein spectra kz blocks");
+  }
   hip_check(hipStreamSynchronize(ctx.stream), "protein R2C sync");
 }
 
@@ -463,20 +468,42 @@ void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, siz
 }
 
 struct Tuned {
-  int fwd = kFwdBlock, inv = kInvBlock, group = 16, depth = 1, poses = 1;
+  int pipe = kPipePlanes;
+  int fwd = kFwdBlock, inv = kInvBlock, group = 16, depth = 1, poses = 1;  // plane form
+  int col_group = 4;                                                       // column form
+  int mid_group() const { return pipe == kPipeColumns ? col_group : group; }
 };
 
-// Autotuner: picks the forward and inverse plane forms (once per grid size,
-// among the forms that fit every batch of that size) and the mid-stage
-// schedule (group size, ring depth, poses per protein load; per launch shape)
-// by timing them on the first launch set of this search.
+// Autotuner, on the first launch set of a search:
+//  - the pipeline form (plane or column form) and, for the plane form, the
+//    forward and inverse plane stage forms, once per grid size: the forms
+//    round differently, so fixing them per (context, N) keeps results
+//    independent of how poses are batched;
+//  - the mid-stage schedule per launch shape (N, channels, box bound, chunk):
+//    plane form group, ring depth and poses per protein load, column form
+//    group.  Schedules only reorder work; results are bit-identical.
+// Choices are made by timing every candidate twice and keeping the faster.
 Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   const int N = prot.N;
   Tuned tu;
   const bool fwd_reg_ok = fwd_reg_supported(N);
   const bool reg_ok = inv_reg_supported(N);
+  const bool col_ok = col_supported(N) && prot.spec_kz.p != nullptr;
   const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)chunk};
-  bool need_fwd = false, need_inv = false, need_group = false;
+  bool need_pipe = false, need_fwd = false, need_inv = false, need_group = false, need_col = false;
+  if (ctx.pipeline >= 0) {
+    tu.pipe = (ctx.pipeline == kPipeColumns && col_ok) ? kPipeColumns : kPipePlanes;
+  } else {
+    auto it = ctx.tuned_pipe.find(N);
+    if (it != ctx.tuned_pipe.end())
+      tu.pipe = it->second;
+    else if (col_ok)
+      need_pipe = true;
+    else
+      ctx.tuned_pipe[N] = tu.pipe = kPipePlanes;
+  }
+  const bool planes = need_pipe || tu.pipe == kPipePlanes;
+  const bool cols = need_pipe || tu.pipe == kPipeColumns;
   if (ctx.inv_variant >= 0) {
     tu.inv = (ctx.inv_variant == kInvReg && reg_ok) ? kInvReg : kInvBlock;
   } else {
@@ -484,7 +511,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     if (it != ctx.tuned_inv.end())
       tu.inv = it->second;
     else if (reg_ok)
-      need_inv = true;
+      need_inv = planes;
     else
       ctx.tuned_inv[N] = tu.inv = kInvBlock;
   }
@@ -495,7 +522,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     if (it != ctx.tuned_fwd.end())
       tu.fwd = it->second;
     else if (fwd_reg_ok)
-      need_fwd = true;
+      need_fwd = planes;
     else
       ctx.tuned_fwd[N] = tu.fwd = kFwdBlock;
   }
@@ -505,25 +532,37 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     tu.group = ctx.mid_group;
     tu.depth = ctx.mid_depth > 0 ? std::min(ctx.mid_depth, max_depth) : std::min(2, max_depth);
     tu.poses = ctx.mid_poses > 0 ? std::min(ctx.mid_poses, max_poses) : 1;
+    tu.col_group = ctx.mid_group;
   } else {
-    auto it = ctx.tuned_group.find(gkey);
-    if (it != ctx.tuned_group.end()) {
-      tu.group = it->second[0];
-      tu.depth = it->second[1];
-      tu.poses = it->second[2];
-    } else {
-      need_group = true;
+    if (planes) {
+      auto it = ctx.tuned_group.find(gkey);
+      if (it != ctx.tuned_group.end()) {
+        tu.group = it->second[0];
+        tu.depth = it->second[1];
+        tu.poses = it->second[2];
+      } else {
+        need_group = true;
+      }
+    }
+    if (cols) {
+      auto it = ctx.tuned_col_group.find(gkey);
+      if (it != ctx.tuned_col_group.end())
+        tu.col_group = it->second;
+      else
+        need_col = true;
     }
   }
-  if (!need_fwd && !need_inv && !need_group) return tu;
+  if (!need_pipe && !need_fwd && !need_inv && !need_group && !need_col) return tu;
 
   const int n = (int)std::min(chunk, t.slots.size());
   const float2* tw = ctx.twiddle(N);
   const PoseSlot* sl = t.d_slots.as<PoseSlot>();
   const LigandDesc* d_ligs = t.d_ligs.as<LigandDesc>();
   RotMeta* d_meta = t.d_meta.as<RotMeta>();
+  unsigned long long* d_keys = t.d_keys.as<unsigned long long>();
+  const float invG = 1.0f / (float)prot.G();
   hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, t.boxmax,
-                        ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta, t.d_keys.as<unsigned long long>()),
+                        ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta, d_keys),
             "tune prep");
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
@@ -547,66 +586,110 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
                                 t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
               "tune fwd planes");
   };
+  auto mid = [&](int g, int depth, int np) {
+    return timed([&] {
+      hip_check(launch_mid(ctx.stream, N, depth, np, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax,
+                           t.boxmax, prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), invG, tw),
+                "tune mid");
+    });
+  };
+  auto inv = [&](int variant) {
+    hip_check(launch_inv_argmin(ctx.stream, N, variant, d_meta, n, N - 2, ctx.d_T.as<float2>(), d_keys, tw),
+              "tune inverse");
+  };
+  float plane_ms = 0.f, col_ms = 0.f;
   try {
-    if (need_fwd) {
-      const float tb = timed([&] { fwd(kFwdBlock); });
-      const float twv = timed([&] { fwd(kFwdReg); });
-      tu.fwd = twv < tb ? kFwdReg : kFwdBlock;
-      ctx.tuned_fwd[N] = tu.fwd;
-    } else {
-      fwd(tu.fwd);
-    }
-    if (need_group) {
-      auto mid = [&](int g, int depth, int np) {
-        return timed([&] {
-          hip_check(launch_mid(ctx.stream, N, depth, np, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax,
-                               t.boxmax, prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(),
-                               1.0f / (float)prot.G(), tw),
-                    "tune mid");
-        });
-      };
-      // One pose at a time: group size at the shallowest ring, then the ring
-      // depth at that group.  Then runs of 2 and 4 poses per protein load
-      // over the group sizes.
-      const int d0 = std::min(2, max_depth);
-      float best = 1e30f;
-      for (int g : {4, 8, 16, 32}) {
-        const float ms = mid(g, d0, 1);
-        if (ms < best) {
-          best = ms;
-          tu.group = g;
-        }
+    if (need_fwd || need_inv || need_group || need_pipe) {
+      if (need_fwd) {
+        const float tb = timed([&] { fwd(kFwdBlock); });
+        const float twv = timed([&] { fwd(kFwdReg); });
+        tu.fwd = twv < tb ? kFwdReg : kFwdBlock;
+        ctx.tuned_fwd[N] = tu.fwd;
+        plane_ms += std::min(tb, twv);
+      } else {
+        plane_ms += timed([&] { fwd(tu.fwd); });
       }
-      tu.depth = d0;
-      for (int d = d0 + 1; d <= max_depth; ++d) {
-        const float ms = mid(tu.group, d, 1);
-        if (ms < best) {
-          best = ms;
-          tu.depth = d;
-        }
-      }
-      tu.poses = 1;
-      for (int np = 2; np <= max_poses; np *= 2)
+      if (need_group) {
+        // One pose at a time: group size at the shallowest ring, then the
+        // ring depth at that group.  Then runs of 2 and 4 poses per protein
+        // load over the group sizes.
+        const int d0 = std::min(2, max_depth);
+        float best = 1e30f;
         for (int g : {4, 8, 16, 32}) {
-          const float ms = mid(g, tu.depth, np);
+          const float ms = mid(g, d0, 1);
           if (ms < best) {
             best = ms;
             tu.group = g;
-            tu.poses = np;
           }
         }
-      ctx.tuned_group[gkey] = {tu.group, tu.depth, tu.poses};
+        tu.depth = d0;
+        for (int d = d0 + 1; d <= max_depth; ++d) {
+          const float ms = mid(tu.group, d, 1);
+          if (ms < best) {
+            best = ms;
+            tu.depth = d;
+          }
+        }
+        tu.poses = 1;
+        for (int np = 2; np <= max_poses; np *= 2)
+          for (int g : {4, 8, 16, 32}) {
+            const float ms = mid(g, tu.depth, np);
+            if (ms < best) {
+              best = ms;
+              tu.group = g;
+              tu.poses = np;
+            }
+          }
+        ctx.tuned_group[gkey] = {tu.group, tu.depth, tu.poses};
+        plane_ms += best;
+      } else {
+        plane_ms += mid(tu.group, tu.depth, tu.poses);
+      }
+      if (need_inv) {
+        const float tb = timed([&] { inv(kInvBlock); });
+        const float tr = timed([&] { inv(kInvReg); });
+        tu.inv = tr < tb ? kInvReg : kInvBlock;
+        ctx.tuned_inv[N] = tu.inv;
+        plane_ms += std::min(tb, tr);
+      } else {
+        plane_ms += timed([&] { inv(tu.inv); });
+      }
     }
-    if (need_inv) {
-      auto inv = [&](int variant) {
-        hip_check(launch_inv_argmin(ctx.stream, N, variant, d_meta, n, N - 2, ctx.d_T.as<float2>(),
-                                    t.d_keys.as<unsigned long long>(), tw),
-                  "tune inverse");
+    if (need_col || need_pipe) {
+      col_ms += timed([&] {
+        hip_check(launch_fwd_z(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+                               t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, t.cmax, t.boxmax,
+                               t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
+                  "tune column forward");
+      });
+      auto midc = [&](int g) {
+        return timed([&] {
+          hip_check(launch_mid_kz(ctx.stream, N, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
+                                  prot.spec_kz.as<float2>(), ctx.d_T.as<float2>(), invG, tw),
+                    "tune column mid");
+        });
       };
-      const float tb = timed([&] { inv(kInvBlock); });
-      const float tr = timed([&] { inv(kInvReg); });
-      tu.inv = tr < tb ? kInvReg : kInvBlock;
-      ctx.tuned_inv[N] = tu.inv;
+      if (need_col) {
+        float best = 1e30f;
+        for (int g : {1, 2, 4, 8}) {
+          const float ms = midc(g);
+          if (ms < best) {
+            best = ms;
+            tu.col_group = g;
+          }
+        }
+        ctx.tuned_col_group[gkey] = tu.col_group;
+        col_ms += best;
+      } else {
+        col_ms += midc(tu.col_group);
+      }
+      col_ms += timed([&] {
+        hip_check(launch_inv_z(ctx.stream, N, d_meta, n, ctx.d_T.as<float2>(), d_keys, tw), "tune column inverse");
+      });
+    }
+    if (need_pipe) {
+      tu.pipe = col_ms < plane_ms ? kPipeColumns : kPipePlanes;
+      ctx.tuned_pipe[N] = tu.pipe;
     }
   } catch (...) {
     (void)hipEventDestroy(e0);
@@ -618,7 +701,8 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   return tu;
 }
 
-// Fused pipeline: four launches per chunk of poses, any mix of ligands.
+// Fused pipeline: four launches per chunk of poses, any mix of ligands, in
+// the plane or the column form.
 void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk, const Tuned& tu,
                    EventTimer& timer) {
   const int N = prot.N;
@@ -639,6 +723,24 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
                           ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
               "prep");
     timer.end(e);
+    if (tu.pipe == kPipeColumns) {
+      e = timer.begin(1);
+      hip_check(launch_fwd_z(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+                             t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, t.cmax, t.boxmax,
+                             t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
+                "column forward");
+      timer.end(e);
+      e = timer.begin(2);
+      hip_check(launch_mid_kz(ctx.stream, N, sl, d_ligs, d_meta + s, n, tu.col_group, ctx.d_A.as<float2>(), t.cmax,
+                              t.boxmax, prot.spec_kz.as<float2>(), ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+                "column mid");
+      timer.end(e);
+      e = timer.begin(3);
+      hip_check(launch_inv_z(ctx.stream, N, d_meta + s, n, ctx.d_T.as<float2>(), d_keys + s, tw),
+                "column inverse + argmin");
+      timer.end(e);
+      continue;
+    }
     e = timer.begin(1);
     hip_check(launch_fwd_planes(ctx.stream, N, tu.fwd, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
                                 t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, t.cmax, t.boxmax,
@@ -715,13 +817,14 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
     const Tuned tu = tune_fused(ctx, prot, t, chunk);
     ctx.last_fwd = tu.fwd;
     ctx.last_inv = tu.inv;
-    ctx.last_group = tu.group;
+    ctx.last_pipe = tu.pipe;
+    ctx.last_group = tu.mid_group();
     ctx.last_depth = tu.depth;
     ctx.last_poses = tu.poses;
     if (use_graph && !ctx.time_stages) {
-      if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.group ||
+      if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.mid_group() ||
           t.graph_fwd != tu.fwd || t.graph_inv != tu.inv || t.graph_depth != tu.depth ||
-          t.graph_poses != tu.poses) {
+          t.graph_poses != tu.poses || t.graph_pipe != tu.pipe) {
         t.drop_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -738,7 +841,8 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
         hip_check(ie, "hipGraphInstantiate");
         t.graph_gen = ctx.ws_generation;
         t.graph_chunk = chunk;
-        t.graph_group = tu.group;
+        t.graph_group = tu.mid_group();
+        t.graph_pipe = tu.pipe;
         t.graph_fwd = tu.fwd;
         t.graph_inv = tu.inv;
         t.graph_depth = tu.depth;

@@ -63,6 +63,7 @@ struct Protein {
   DevBuf grid;               // float [nch][G]
   DevBuf spec;               // float2 [nch][H]
   DevBuf spec_lb;            // the same spectra in the fused engine's line-blocked layout
+  DevBuf spec_kz;            // and in the column form's kz-plane layout (grid edge 64 only)
   size_t G() const { return (size_t)N * N * N; }
   size_t H() const { return (size_t)N * N * (N / 2 + 1); }
   // gridMin = mid - len / 2 evaluated in float, then widened (NoMini:149-151).
@@ -138,7 +139,7 @@ struct Prepared {
   hipGraphExec_t graph = nullptr;
   uint64_t graph_gen = 0;
   size_t graph_chunk = 0;
-  int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0;
+  int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0, graph_pipe = -1;
   std::vector<PoseResults> results;
   Prepared() = default;
   Prepared(const Prepared&) = delete;
@@ -159,6 +160,7 @@ struct Context {
   int mid_group = 0;              // poses per mid-stage workgroup; 0 = tuned
   int mid_depth = 0;              // mid-stage register ring depth with a fixed group; 0 = 2
   int mid_poses = 0;              // poses sharing each protein load in the mid stage with a fixed group; 0 = 1
+  int pipeline = -1;              // fused pipeline form (kPipePlanes, kPipeColumns); -1 = tuned
   int fwd_variant = -1;           // forward plane stage form; -1 = tuned
   int inv_variant = -1;           // inverse plane stage form; -1 = tuned
   bool use_graph = true;
@@ -169,11 +171,14 @@ struct Context {
   // the mid group size only changes the schedule and is tuned per launch
   // shape, with the ring depth and the poses per protein load.
   size_t max_dyn_lds = 64 * 1024;
+  std::map<int, int> tuned_pipe;                     // N -> pipeline form
   std::map<int, int> tuned_fwd;                      // N -> variant
   std::map<int, int> tuned_inv;                      // N -> variant
   // (N, cmax, boxmax, chunk) -> (group, depth, poses per protein load)
   std::map<std::array<int, 4>, std::array<int, 3>> tuned_group;
-  int last_fwd = -1, last_inv = -1, last_group = 0, last_depth = 0, last_poses = 0;
+  std::map<std::array<int, 4>, int> tuned_col_group;  // (N, cmax, boxmax, chunk) -> column form group
+  int last_fwd = -1, last_inv = -1, last_group = 0, last_depth = 0, last_poses = 0,
+      last_pipe = -1;
   StageTimes stage_times;
   // per-launch-set working arrays; ws_generation counts reallocations so a
   // captured graph that baked in the old addresses is rebuilt

@@ -626,6 +626,67 @@ struct FwdRegShape {
   static constexpr int RSTR = N + ((4 - N % 32) + 32) % 32;
 };
 
+// One x-plane of the register forward stages (steps 1 and 2 above): zero the
+// tile's row pairs, spread the plane's cells into them in the reference's
+// atom order (wavefront 0, atoms touching the plane picked by ballot;
+// FillLigandGrid.cpp:45-66, 97-117), z-transform the row pairs in place.
+// Ends behind a block barrier with the packed row spectra in the tile.
+template <int N>
+__device__ __forceinline__ void fwd_plane_rows(int x, const SpreadAtom* __restrict__ sa, int nc, float2* zt,
+                                               int npair, bool zhalf, const float2* twl, float2* tr, int wave,
+                                               int lane) {
+  using RS_ = RegShape<N>;
+  constexpr int L = RS_::L, V = RS_::V, LPW = RS_::LPW, RSTR = FwdRegShape<N>::RSTR;
+  constexpr bool PRUNE = V == 8 || V == 16;
+  const int ln = lane % LPW, n2 = lane / LPW;
+  const int zlim = zhalf ? N / 2 : N;
+  const int dy = (lane >> 1) & 1, dz = lane & 1;
+  __syncthreads();  // atoms staged; the previous plane's reads of the tile are done
+  for (int t = threadIdx.x; t < npair * zlim; t += blockDim.x) {
+    const int p = t / zlim;
+    zt[p * RSTR + (t - p * zlim)] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    float* cells = reinterpret_cast<float*>(zt);
+    for (int base = 0; base < nc; base += 64) {
+      const int aix = base + lane < nc ? sa[base + lane].ix : -4;
+      unsigned long long mk = __ballot(x - aix == 0 || x - aix == 1);
+      while (mk) {
+        const int i = base + __ffsll((long long)mk) - 1;
+        mk &= mk - 1;
+        const int ddx = x - sa[i].ix;
+        if (lane < 4) {
+          const int y = sa[i].iy + dy, z = sa[i].iz + dz;
+          float* cell = cells + (((y >> 1) * RSTR + z) << 1) + (y & 1);
+          *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, ddx), sa[i].fy, sa[i].fz, dy, dz);
+        }
+        WaveSync::sync();
+      }
+    }
+  }
+  __syncthreads();
+  for (int p0 = wave * LPW; p0 < npair; p0 += 4 * LPW) {
+    const int p = p0 + ln;
+    const bool ok = p < npair;
+    float2 v[V];
+    if (PRUNE && zhalf) {
+#pragma unroll
+      for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
+      reg_fft<N, -1, PRUNE>(v, twl, tr, ln, n2);
+    } else {
+#pragma unroll
+      for (int n1 = 0; n1 < V; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
+      reg_fft<N, -1, false>(v, twl, tr, ln, n2);
+    }
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < V; ++q) zt[p * RSTR + n2 + L * q] = v[q];
+    }
+  }
+  __syncthreads();
+}
+
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __restrict__ slots,
                                                            const LigandDesc* __restrict__ ligs,
@@ -684,56 +745,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
   // take the full transform (one code path, fewer live registers).
   constexpr bool PRUNE = V == 8 || V == 16;
   const bool zhalf = PRUNE && bz <= N / 2, yhalf = PRUNE && by <= N / 2;
-  const int zlim = zhalf ? N / 2 : N;
   float2* op0 = A + (((size_t)s * cmax + c) * boxmax) * (size_t)(N * ZH);
-  const int dy = (lane >> 1) & 1, dz = lane & 1;
   for (int x = 0; x < bxm; ++x) {
-    __syncthreads();  // atoms staged; the previous plane's column loads are done
-    for (int t = threadIdx.x; t < npair * zlim; t += blockDim.x) {
-      const int p = t / zlim;
-      zt[p * RSTR + (t - p * zlim)] = make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    // 1. ordered scatter by wavefront 0 (FillLigandGrid.cpp:45-66, 97-117).
-    if (wave == 0) {
-      float* cells = reinterpret_cast<float*>(zt);
-      for (int base = 0; base < nc; base += 64) {
-        const int aix = base + lane < nc ? sa[base + lane].ix : -4;
-        unsigned long long mk = __ballot(x - aix == 0 || x - aix == 1);
-        while (mk) {
-          const int i = base + __ffsll((long long)mk) - 1;
-          mk &= mk - 1;
-          const int ddx = x - sa[i].ix;
-          if (lane < 4) {
-            const int y = sa[i].iy + dy, z = sa[i].iz + dz;
-            float* cell = cells + (((y >> 1) * RSTR + z) << 1) + (y & 1);
-            *cell = accum_corner(*cell, x_factor(sa[i].w, sa[i].fx, ddx), sa[i].fy, sa[i].fz, dy, dz);
-          }
-          WaveSync::sync();
-        }
-      }
-    }
-    __syncthreads();
-    // 2. z transforms of the row pairs, written back in place.
-    for (int p0 = wave * LPW; p0 < npair; p0 += 4 * LPW) {
-      const int p = p0 + ln;
-      const bool ok = p < npair;
-      float2 v[V];
-      if (PRUNE && zhalf) {
-#pragma unroll
-        for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
-        reg_fft<N, -1, PRUNE>(v, twl, tr, ln, n2);
-      } else {
-#pragma unroll
-        for (int n1 = 0; n1 < V; ++n1) v[n1] = ok ? zt[p * RSTR + n2 + L * n1] : make_float2(0.f, 0.f);
-        reg_fft<N, -1, false>(v, twl, tr, ln, n2);
-      }
-      if (ok) {
-#pragma unroll
-        for (int q = 0; q < V; ++q) zt[p * RSTR + n2 + L * q] = v[q];
-      }
-    }
-    __syncthreads();
+    fwd_plane_rows<N>(x, sa, nc, zt, npair, zhalf, twl, tr, wave, lane);
     // 3. y transforms; line j = column j, except line 0 = columns 0 (real
     // part) and N/2 (imaginary part).
     for (int j0 = wave * LPW; j0 < N / 2; j0 += 4 * LPW) {
@@ -1333,6 +1347,385 @@ __global__ __launch_bounds__(kBlock) void k_inv_argmin_reg(const RotMeta* __rest
   block_min_commit(best, keys + s);
 }
 
+// ---------------------------------------------------------------------------
+// Column form of the fused pipeline (grid edge 64: 8 values per lane, two
+// passes of 32 lines per workgroup).  The forward stage stops after the z transform; the mid stage runs,
+// per kz plane, the y and x transforms of every channel through one LDS
+// plane, the product with the protein spectrum, the channel sum, 1/G and the
+// inverse x and y transforms; the last stage runs the z C2R and the argmin.
+// Only the ligand box's (x, y) rows and the window's (x', y') rows cross HBM,
+// about half the bytes of the plane form:
+//   Z [slot][c][kz][n1 < R1][x < boxmax][n2], y = n2 + L*n1 < by, x < bx,
+//     R1 = ceil(boxmax / L)
+//   T2[slot][kz][m < V][x' < N][n2],          y' = n2 + L*m < wy, x' < wx
+//   P2[c][kz][ky / LPW][q < V/2][n2][ky % LPW][2], kx = n2 + L*(2q + pair)
+// Every index is a lane's register layout (line, n2, value n1), so each
+// wavefront load or store covers 512 contiguous bytes (1 KB for P2).
+// ---------------------------------------------------------------------------
+// LDS plane of the column form's mid stage: (x, k) at x*N + (k ^ ((9x) & 15)).
+// Lines along x with values along k, and lines along k with values along x,
+// then both put every 16 consecutive lanes on 16 distinct 8-byte bank slots
+// (checked by enumeration for N = 64; no unswizzled linear layout does).
+// With L = 8 the swizzle of value j of a lane differs from value 0's only in
+// bit 3, by j's parity, so a lane's V addresses are two bases plus
+// compile-time offsets:
+//   lane line x, value j at k = n2 + 8j:  base[j & 1] + 16 * (j >> 1)
+//   lane line k, value j at x = n2 + 8j:  base[j & 1] + 8 * N * j
+struct ColBases {
+  int b0, b1;
+};
+template <int N>
+__device__ __forceinline__ ColBases col_line_x(int x, int n2) {
+  const int m = (9 * x) & 15;
+  return {x * N + (n2 ^ m), x * N + ((n2 + 8) ^ m)};
+}
+template <int N>
+__device__ __forceinline__ ColBases col_line_k(int k, int n2) {
+  const int m = (9 * n2) & 15;
+  return {n2 * N + (k ^ m), n2 * N + (k ^ m ^ 8)};
+}
+template <int N>
+__device__ __forceinline__ int col_at_x(const ColBases& b, int j) {  // lane line x, value j
+  return ((j & 1) ? b.b1 : b.b0) + 16 * (j >> 1);
+}
+template <int N>
+__device__ __forceinline__ int col_at_k(const ColBases& b, int j) {  // lane line k, value j
+  return ((j & 1) ? b.b1 : b.b0) + 8 * N * j;
+}
+
+// Position of spectrum value (kx, ky, kz) in a channel's P2 block.
+template <int N>
+__device__ __forceinline__ int kzb_index(int kx, int ky, int kz) {
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW, NB = N / LPW;
+  const int m = kx / L, n2 = kx % L, kyb = ky / LPW, ln = ky % LPW;
+  return (((kz * NB + kyb) * (V / 2) + m / 2) * L + n2) * (2 * LPW) + ln * 2 + (m & 1);
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_to_kz_blocks(const float2* __restrict__ in, float2* __restrict__ out,
+                                                         int nch) {
+  constexpr int N = P::N, ZH = P::ZH;
+  const size_t per = (size_t)N * N * ZH;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= per * nch) return;
+  const int c = (int)(t / per);
+  const int r = (int)(t - (size_t)c * per);  // [kx][ky][kz] (cuFFT R2C layout)
+  const int kx = r / (N * ZH), ky = (r / ZH) % N, kz = r % ZH;
+  out[(size_t)c * per + (size_t)kzb_index<N>(kx, ky, kz)] = in[t];
+}
+
+// Column form, forward: one workgroup per (pose, ligand channel); per
+// occupied x-plane the register form's spread and z transforms
+// (fwd_plane_rows), then the row pairs unpacked into half-spectrum rows
+// y < by and stored kz-major into Z.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fwd_z(const PoseSlot* __restrict__ slots,
+                                                  const LigandDesc* __restrict__ ligs,
+                                                  const int* __restrict__ chan_atoms, const float* __restrict__ wv,
+                                                  const float* __restrict__ wq, const AtomGrid* __restrict__ ag,
+                                                  int astride, const RotMeta* __restrict__ meta, int cmax,
+                                                  int boxmax, int natom_lds, float2* __restrict__ Z,
+                                                  const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  using RS_ = RegShape<N>;
+  constexpr int L = RS_::L, V = RS_::V, RSTR = FwdRegShape<N>::RSTR;
+  static_assert(V == 8, "column form: 8 values per lane");
+  __shared__ float2 sbuf[4][RS_::SLICE];
+  extern __shared__ double dyn[];  // [natom_lds] SpreadAtom, then the tile [npmax][RSTR]
+  const int c = blockIdx.x, s = blockIdx.y;
+  const RotMeta m = meta[s];
+  if (!m.valid) return;
+  const PoseSlot ps = slots[s];
+  const LigandDesc& Ld = ligs[ps.lig];
+  if (c >= Ld.nch) return;
+  const int nc = Ld.ch_start[c + 1] - Ld.ch_start[c];
+  SpreadAtom* sa = reinterpret_cast<SpreadAtom*>(dyn);
+  float2* zt = reinterpret_cast<float2*>(sa + natom_lds);
+  {
+    const int* ca = chan_atoms + Ld.list_off + Ld.ch_start[c];
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+      const int a = ca[i];
+      const AtomGrid g = ag[(size_t)s * astride + a];
+      SpreadAtom t;
+      t.ix = g.ix;
+      t.iy = g.iy;
+      t.iz = g.iz;
+      t.w = channel_weight(Ld, c, a, wv, wq);
+      t.fx = g.fx;
+      t.fy = g.fy;
+      t.fz = g.fz;
+      sa[i] = t;
+    }
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int n2l = lane / RS_::LPW;
+  float2 twr[V];
+  reg_twiddles<N>(twg, n2l, twr);
+  const int bxm = min(m.box[0], boxmax), by = m.box[1], bz = m.box[2];
+  const int npair = (by + 1) >> 1;
+  const bool zhalf = bz <= N / 2;
+  const int r1 = (boxmax + L - 1) / L;
+  float2* zb = Z + ((size_t)s * cmax + c) * ((size_t)ZH * r1 * boxmax * L);
+  for (int x = 0; x < bxm; ++x) {
+    fwd_plane_rows<N>(x, sa, nc, zt, npair, zhalf, twr, sbuf[wave], wave, lane);
+    for (int t = threadIdx.x; t < ZH * by; t += blockDim.x) {
+      const int kz = t / by, y = t - (t / by) * by;
+      const int p = y >> 1;
+      const float2 zk = zt[p * RSTR + kz], zm = zt[p * RSTR + (kz == 0 ? 0 : N - kz)];
+      const float2 v = (y & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
+                               : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      zb[((size_t)(kz * r1 + y / L) * boxmax + x) * L + (y % L)] = v;
+    }
+  }
+}
+
+// Column form, mid: one workgroup per (kz, group of poses).  For each pose
+// and channel: y transforms of the box's x lines into the LDS plane [x][ky],
+// x transforms of all ky lines out of it, product with conj and the protein
+// spectrum, channel sum; then 1/G, inverse x transforms of the ky lines,
+// back through the plane, inverse y transforms of the window's x' lines,
+// store of the window rows.  Lane lines: 4 wavefronts x 64/L lines per
+// pass, N / (4 * 64/L) passes.  When the box fits in N/2 along a transform's
+// input axis the upper half is neither loaded nor transformed in the first
+// DFT stage.  Workgroup w runs on XCD w % 8 and takes kz = w % 8 + 8 j, so
+// each XCD's L2 holds the protein spectra of every eighth kz plane only.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ slots,
+                                                   const LigandDesc* __restrict__ ligs,
+                                                   const RotMeta* __restrict__ meta, int nslots, int group,
+                                                   const float2* __restrict__ Z, int cmax, int boxmax,
+                                                   const float2* __restrict__ P2, float2* __restrict__ T2, float invG,
+                                                   const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
+  static_assert(V == 8 && L == 8, "column form: 8 lanes x 8 values per line");
+  constexpr int LPP = 4 * LPW;     // lines per pass
+  static_assert(N % LPP == 0, "whole passes of lines");
+  constexpr int NPASS = N / LPP;
+  constexpr int KZX = (ZH + 7) / 8;
+  __shared__ float2 sbuf[4][RS::SLICE];
+  __shared__ float2 pl[N * N];
+  __shared__ float2 stw[L * V];
+  const int w = blockIdx.x;
+  const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
+  const int sg = (w >> 3) / KZX;
+  if (kz >= ZH) return;
+  // twiddles from LDS: the registers go to the accumulators
+  for (int t = threadIdx.x; t < L * V; t += blockDim.x) stw[t] = twg[(t / V) * (t % V)];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  float2* tr = sbuf[wave];
+  const float2* twr = stw + (lane / LPW) * V;
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  const int r1 = (boxmax + L - 1) / L;
+  const size_t zslab = (size_t)ZH * r1 * boxmax * L;  // float2 per (pose, channel)
+  const int s0 = sg * group, s1 = min(nslots, s0 + group);
+  for (int s = s0; s < s1; ++s) {
+    if (!uni(meta[s].valid)) continue;
+    const int lig = uni(slots[s].lig);
+    const int nch = uni(ligs[lig].nch);
+    const int bx = uni(meta[s].box[0]), by = uni(meta[s].box[1]);
+    const int wx = uni(meta[s].win[0]), wy = uni(meta[s].win[1]);
+    const bool xhalf = bx <= N / 2, yhalf = by <= N / 2;
+    float2 acc[NPASS][V];
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p)
+#pragma unroll
+      for (int i = 0; i < V; ++i) acc[p][i] = make_float2(0.f, 0.f);
+    const int zrow = boxmax * L * (int)sizeof(float2);  // bytes per row group n1
+    for (int c = 0; c < nch; ++c) {
+      // Lane coordinates pass through an empty asm each iteration: the
+      // address arithmetic below is then redone here instead of being
+      // hoisted out of the loops into dozens of live registers.
+      int ln = lane % LPW, n2 = lane / LPW;
+      asm volatile("" : "+v"(ln), "+v"(n2));
+      // the kz block of Z; lanes outside the box read zero through the
+      // descriptor (offset past its end), with no memory traffic
+      const __amdgpu_buffer_rsrc_t zr =
+          buffer_rsrc(Z + ((size_t)s * cmax + c) * zslab + (size_t)kz * r1 * boxmax * L, (unsigned)(r1 * zrow));
+      __syncthreads();  // the plane's previous reads are done
+#pragma unroll
+      for (int p = 0; p < NPASS; ++p) {
+        const int x0 = p * LPP + wave * LPW;
+        if (x0 < bx) {
+          const int x = x0 + ln;
+          const int voff = x < bx ? (x * L + n2) * (int)sizeof(float2) : kVoffNone;
+          auto zload = [&](int n1) {
+            return buffer_load_f2(zr, n2 + L * n1 < by ? voff + n1 * zrow : kVoffNone);
+          };
+          float2 v[V];
+          if (yhalf) {
+#pragma unroll
+            for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = zload(n1);
+            reg_fft<N, -1, true>(v, twr, tr, ln, n2);
+          } else {
+#pragma unroll
+            for (int n1 = 0; n1 < V; ++n1) v[n1] = zload(n1);
+            reg_fft<N, -1, false>(v, twr, tr, ln, n2);
+          }
+          const ColBases wb = col_line_x<N>(x, n2);
+#pragma unroll
+          for (int q = 0; q < V; ++q) pl[col_at_x<N>(wb, q)] = v[q];
+        }
+      }
+      __syncthreads();
+      const int pc = uni(ligs[lig].pmap[c]);
+#pragma unroll
+      for (int p = 0; p < NPASS; ++p) {
+        const int ky0 = p * LPP + wave * LPW, ky = ky0 + ln;
+        const float2* pb = P2 + ((size_t)pc * ZH + kz) * (size_t)(N * N) +
+                           (size_t)(ky0 / LPW) * (V / 2) * L * 2 * LPW + n2 * 2 * LPW + ln * 2;
+        float2 pr[V];
+#pragma unroll
+        for (int q = 0; q < V / 2; ++q) {
+          const float4 f = *reinterpret_cast<const float4*>(pb + q * L * 2 * LPW);
+          pr[2 * q] = make_float2(f.x, f.y);
+          pr[2 * q + 1] = make_float2(f.z, f.w);
+        }
+        float2 v[V];
+        const ColBases rb = col_line_k<N>(ky, n2);
+        // rows x >= bx of the plane hold stale values: read, then select
+        auto pread = [&](int n1) {
+          const float2 t = pl[col_at_k<N>(rb, n1)];
+          const bool in = n2 + L * n1 < bx;
+          return make_float2(in ? t.x : 0.f, in ? t.y : 0.f);
+        };
+        if (xhalf) {
+#pragma unroll
+          for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = pread(n1);
+          reg_fft<N, -1, true>(v, twr, tr, ln, n2);
+        } else {
+#pragma unroll
+          for (int n1 = 0; n1 < V; ++n1) v[n1] = pread(n1);
+          reg_fft<N, -1, false>(v, twr, tr, ln, n2);
+        }
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          const float2 l = v[i], q = pr[i];
+          acc[p][i].x = fmaf(l.x, q.x, acc[p][i].x);
+          acc[p][i].x = fmaf(l.y, q.y, acc[p][i].x);
+          acc[p][i].y = fmaf(l.x, q.y, acc[p][i].y);
+          acc[p][i].y = fmaf(-l.y, q.x, acc[p][i].y);
+        }
+        // keep the passes apart: overlapping them costs registers, not time
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // inverse x transforms of the ky lines: value m = x' = n2 + L*m
+    int ln = lane % LPW, n2 = lane / LPW;
+    asm volatile("" : "+v"(ln), "+v"(n2));
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) acc[p][i] = c_scale(acc[p][i], invG);
+      reg_fft<N, 1>(acc[p], twr, tr, ln, n2);
+    }
+    __syncthreads();  // the plane's forward reads are done
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+      const ColBases wb = col_line_k<N>(p * LPP + wave * LPW + ln, n2);
+#pragma unroll
+      for (int q = 0; q < V; ++q) pl[col_at_k<N>(wb, q)] = acc[p][q];
+    }
+    __syncthreads();
+    float2* tb = T2 + ((size_t)s * ZH + kz) * (size_t)(N * N);
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+      const int x0 = p * LPP + wave * LPW;
+      if (x0 < wx) {
+        const int xp = x0 + ln;
+        float2 v[V];
+        const ColBases rb = col_line_x<N>(xp, n2);
+#pragma unroll
+        for (int n1 = 0; n1 < V; ++n1) v[n1] = pl[col_at_x<N>(rb, n1)];
+        reg_fft<N, 1>(v, twr, tr, ln, n2);
+        if (xp < wx) {
+#pragma unroll
+          for (int q = 0; q < V; ++q)
+            if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
+        }
+      }
+    }
+  }
+}
+
+// Column form, last stage: one workgroup per (pose, block of 64/L * 4 row
+// pairs).  Row pair (x', y0 = L*m + 2h, y0 + 1) is one complex line A + iB
+// of the z C2R (two-for-one, imaginary parts of the kz = 0 and N/2 bins
+// dropped as a C2R transform does); the window minimum of the two real rows
+// goes to the pose's key.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ meta, const float2* __restrict__ T2,
+                                                  unsigned long long* __restrict__ keys,
+                                                  const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
+  static_assert(V == 8, "column form: 8 values per lane");
+  constexpr int H = L / 2;        // row pairs per L rows
+  constexpr int LPB = 4 * LPW;    // row pairs per workgroup
+  __shared__ float2 sbuf[4][RS::SLICE];
+  const int s = blockIdx.y;
+  const RotMeta m = meta[s];
+  if (!m.valid) return;
+  const int wx = m.win[0], wy = m.win[1], wz = m.win[2];
+  const int M = (wy + L - 1) / L;
+  const int nl = M * wx * H;
+  if ((int)blockIdx.x * LPB >= nl) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ln = lane % LPW, n2 = lane / LPW;
+  float2 twr[V];
+  reg_twiddles<N>(twg, n2, twr);
+  const int l = blockIdx.x * LPB + wave * LPW + ln;
+  const bool ok = l < nl;
+  const int mm = l / (wx * H), r = l - mm * (wx * H);
+  const int xp = r / H, h = r - (r / H) * H;
+  const int y0 = L * mm + 2 * h;
+  const bool aok = ok && y0 < wy, bok = ok && y0 + 1 < wy;
+  const float2* tb = T2 + (size_t)s * ZH * (N * N) + (size_t)(mm * N + xp) * L + 2 * h;
+  float2 z[V];
+#pragma unroll
+  for (int n1 = 0; n1 < V; ++n1) {
+    const int k = n2 + L * n1;
+    const int kk = k <= N / 2 ? k : N - k;
+    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ok) f = *reinterpret_cast<const float4*>(tb + (size_t)kk * (N * N));
+    float2 A = aok ? make_float2(f.x, f.y) : make_float2(0.f, 0.f);
+    float2 B = bok ? make_float2(f.z, f.w) : make_float2(0.f, 0.f);
+    if (k == 0 || k == N / 2) {
+      A.y = 0.f;
+      B.y = 0.f;
+    } else if (k > N / 2) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z[n1] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  reg_fft<N, 1>(z, twr, sbuf[wave], ln, n2);
+  unsigned long long best = ~0ull;
+  if (ok) {
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const int zz = n2 + L * q;
+      if (zz < wz) {
+        const unsigned lin0 = (unsigned)((xp * N + y0) * N + zz);
+        if (aok) {
+          const unsigned long long k0 = energy_key(z[q].x, lin0);
+          best = k0 < best ? k0 : best;
+        }
+        if (bok) {
+          const unsigned long long k1 = energy_key(z[q].y, lin0 + (unsigned)N);
+          best = k1 < best ? k1 : best;
+        }
+      }
+    }
+  }
+  block_min_commit(best, keys + s);
+}
+
 // Rotate.cpp:4-27 for every (quaternion, atom): out[q][a][3].
 __global__ __launch_bounds__(kBlock) void k_rotate(int nq, int natom, const float* __restrict__ quats,
                                                    const float* __restrict__ coords, float* __restrict__ out) {
@@ -1520,6 +1913,11 @@ hipError_t configure_fwd_reg(size_t max_dyn_lds) {
       const size_t dyn = max_dyn_lds > stat ? max_dyn_lds - stat : 0;
       e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_planes_reg<PL>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+      if constexpr (PL::N == 64) {
+        if (e == hipSuccess)
+          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_z<PL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+      }
     });
     if (e != hipSuccess) return e;
   }
@@ -1539,6 +1937,67 @@ static size_t fwd_reg_static_lds(int N) {
 
 bool fwd_reg_supported(int N) {
   return fwd_reg_lds_bytes(N, N, kMaxAtoms) + fwd_reg_static_lds(N) <= g_max_dyn_lds;
+}
+
+bool col_supported(int N) { return N == 64 && fwd_reg_supported(N); }
+
+hipError_t launch_fwd_z(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                        const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
+                        const RotMeta* meta, int cmax, int boxmax, int max_chan_atoms, float2* Z, const float2* tw) {
+  if (nslots <= 0) return hipSuccess;
+  if (!col_supported(N)) return hipErrorInvalidValue;
+  const int natom = std::max(1, max_chan_atoms);
+  const size_t dyn = fwd_reg_lds_bytes(N, boxmax, natom);
+  if (dyn == 0 || dyn + fwd_reg_static_lds(N) > g_max_dyn_lds) return hipErrorInvalidValue;
+  CDK_DISPATCH_N(N, {
+    if constexpr (PL::N == 64)
+      hipLaunchKernelGGL(k_fwd_z<PL>, dim3(cmax, nslots), dim3(kBlock), dyn, st, slots, ligs, chan_atoms, wv, wq, ag,
+                         astride, meta, cmax, boxmax, natom, Z, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
+                         int nslots, int group, const float2* Z, int cmax, int boxmax, const float2* P2, float2* T2,
+                         float invG, const float2* tw) {
+  if (nslots <= 0) return hipSuccess;
+  if (!col_supported(N) || group <= 0) return hipErrorInvalidValue;
+  const int kzx = (N / 2 + 1 + 7) / 8;
+  const dim3 grid((unsigned)(8 * kzx) * ceil_div(nslots, group));
+  CDK_DISPATCH_N(N, {
+    if constexpr (PL::N == 64)
+      hipLaunchKernelGGL(k_mid_kz<PL>, grid, dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, Z, cmax, boxmax,
+                         P2, T2, invG, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, const float2* T2,
+                        unsigned long long* keys, const float2* tw) {
+  if (nslots <= 0) return hipSuccess;
+  if (!col_supported(N)) return hipErrorInvalidValue;
+  CDK_DISPATCH_N(N, {
+    if constexpr (PL::N == 64) {
+      using RS = RegShape<PL::N>;
+      // windows are at most N - 2 wide (boxes are at least 3 cells)
+      const int w = PL::N - 2;
+      const int pairs = ((w + RS::L - 1) / RS::L) * w * (RS::L / 2);
+      const unsigned gx = ceil_div(pairs, 4 * RS::LPW);
+      hipLaunchKernelGGL(k_inv_z<PL>, dim3(gx, nslots), dim3(kBlock), 0, st, meta, T2, keys, tw);
+    }
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_to_kz_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch) {
+  const size_t n = (size_t)nch * N * N * (N / 2 + 1);
+  if (n == 0) return hipSuccess;
+  if (N != 64) return hipErrorInvalidValue;
+  CDK_DISPATCH_N(N, {
+    if constexpr (PL::N == 64)
+      hipLaunchKernelGGL(k_to_kz_blocks<PL>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, in, out, nch);
+  });
+  return hipGetLastError();
 }
 
 hipError_t launch_fwd_planes(hipStream_t st, int N, int variant, const PoseSlot* slots, int nslots,

@@ -53,6 +53,22 @@ int mid_max_poses(int N);
 hipError_t launch_mid(hipStream_t st, int N, int depth, int np, const PoseSlot* slots, const LigandDesc* ligs,
                       const RotMeta* meta, int nslots, int group, const float2* A, int cmax, int boxmax,
                       const float2* Phat, int npch, float2* T, float invG, const float2* tw);
+// Column form of the fused pipeline (grid edge 64): the forward stage
+// stops after the z transform (Z, kz-major rows of the ligand box), the mid
+// stage runs the y and x transforms per kz plane through LDS and writes the
+// window rows (T2), the last stage runs the z C2R and the argmin.  P2 is the
+// protein spectra in the mid stage's register layout (launch_to_kz_blocks).
+constexpr int kPipePlanes = 0, kPipeColumns = 1;
+bool col_supported(int N);
+hipError_t launch_fwd_z(hipStream_t st, int N, const PoseSlot* slots, int nslots, const LigandDesc* ligs,
+                        const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
+                        const RotMeta* meta, int cmax, int boxmax, int max_chan_atoms, float2* Z, const float2* tw);
+hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs, const RotMeta* meta,
+                         int nslots, int group, const float2* Z, int cmax, int boxmax, const float2* P2, float2* T2,
+                         float invG, const float2* tw);
+hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, const float2* T2,
+                        unsigned long long* keys, const float2* tw);
+hipError_t launch_to_kz_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
 // Inverse plane stage variants: one workgroup per (pose, window plane) with
 // block-wide LDS FFTs, or one workgroup per (pose, 4 window planes) with
 // register FFTs and one LDS tile of N rows.

@@ -59,7 +59,11 @@ int cdk_context_destroy(cdk_context* ctx);
  * "mid_depth" (register buffers in the mid stage's (pose, channel) ring when
  * mid_group is fixed: 2 or 3 at 32^3/64^3, 0 = 2), "mid_poses" (poses of
  * one ligand that share each protein spectrum load when mid_group is fixed:
- * 1, or 2 or 4 at 32^3/64^3; 0 = 1), "fwd_variant" (forward
+ * 1, or 2 or 4 at 32^3/64^3; 0 = 1), "pipeline" (fused engine form: 0 plane
+ * form, whose forward stage writes ligand planes after the z and y
+ * transforms; 1 column form at 64^3, whose forward stage stops after the z
+ * transform and whose mid stage runs the y and x transforms per kz plane;
+ * -1 = autotuned once per grid size), "fwd_variant" (forward
  * plane stage: 0 block-wide LDS Stockham FFTs, 1 register four-step FFTs,
  * -1 = autotuned once per grid size), "inv_variant" (inverse
  * plane stage + argmin: 0 block-wide LDS FFTs per plane, 1 register FFTs over
@@ -67,8 +71,8 @@ int cdk_context_destroy(cdk_context* ctx);
  * "time_stages". */
 int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t value);
 /* Reads an option back; also "last_fwd_variant", "last_inv_variant",
- * "last_mid_group", "last_mid_depth" and "last_mid_poses" (what the
- * autotuner chose for the
+ * "last_pipeline", "last_mid_group", "last_mid_depth" and "last_mid_poses"
+ * (what the autotuner chose for the
  * last fused search) and "max_dyn_lds" (the
  * device's dynamic LDS limit per workgroup, bytes).  No reference
  * counterpart. */

@@ -70,6 +70,7 @@ def test_chunking_does_not_change_results(ctx, prots):
 
 
 def _fixed_mid(ctx, grp, depth, poses):
+    ctx.set_option("pipeline", 0)  # the schedules below are the plane form's
     ctx.set_option("mid_group", grp)
     ctx.set_option("mid_depth", depth)
     ctx.set_option("mid_poses", poses)
@@ -77,6 +78,7 @@ def _fixed_mid(ctx, grp, depth, poses):
 
 def _tuned_mid(ctx):
     _fixed_mid(ctx, 0, 0, 0)
+    ctx.set_option("pipeline", -1)
 
 
 def test_mid_schedule_does_not_change_results(ctx, prots):
@@ -88,8 +90,9 @@ def test_mid_schedule_does_not_change_results(ctx, prots):
     w = workloads.config2(nconf=3, nquat=40)
     L = w["ligands"][0]
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
-    E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
     try:
+        ctx.set_option("pipeline", 0)
+        E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
         # group 5 and 3 split pose runs at group edges; 1 pose per group
         # leaves the run one pose long
         for grp, depth, poses in ((4, 2, 1), (8, 3, 1), (32, 2, 1), (5, 3, 1), (4, 2, 2), (8, 2, 4), (5, 2, 2),

@@ -133,6 +133,7 @@ def test_plane_stage_forms_vs_oracle(ctx, p48, p64, variant):
     """Both forms of the forward plane stage (block-wide LDS Stockham FFTs,
     register four-step FFTs) and of the inverse plane stage (block-wide LDS
     FFTs per plane, register FFTs over 4 planes) against the oracle."""
+    ctx.set_option("pipeline", 0)
     ctx.set_option("fwd_variant", variant)
     ctx.set_option("inv_variant", variant)
     try:
@@ -149,8 +150,48 @@ def test_plane_stage_forms_vs_oracle(ctx, p48, p64, variant):
             say("forward form", variant, stats)
             assert stats.get("exact", 0) >= nq - 2
     finally:
+        ctx.set_option("pipeline", -1)
         ctx.set_option("fwd_variant", -1)
         ctx.set_option("inv_variant", -1)
+
+
+def test_column_form_vs_oracle(ctx, p64):
+    """Column form at 64^3 (forward z transform only; y and x transforms per
+    kz plane in the mid stage; z C2R last) against the oracle, for ligands
+    whose boxes fit in N/2 and ligands whose boxes do not; and the same bits
+    for every mid-stage group size."""
+    g, prot = p64
+    ctx.set_option("pipeline", 1)
+    try:
+        worst = 0
+        for seed, na, nq in ((901, 15, 8), (902, 40, 8), (903, 90, 6)):
+            L = workloads.ligand(seed, na, 1)
+            quats = workloads.quaternion_sets(seed + 1, 1, nq)
+            lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+            b = cdk.Batch(ctx, prot)
+            b.add(lig, L["conformers"], quats)
+            b.run(cdk.ENGINE_FUSED)
+            box, _ = b.geometry(0)
+            b.close()
+            worst = max(worst, int(box.max()))
+            say("column form ligand", na, "atoms, largest box", int(box.max()))
+            E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats)
+            assert ctx.get_option("last_pipeline") == 1
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+            stats = check_search(prob, L["conformers"], quats, E, idx, mc)
+            say("column form", na, stats)
+            assert stats.get("exact", 0) >= nq - 1
+            for grp in (1, 3, 8):
+                ctx.set_option("mid_group", grp)
+                E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"], quats)
+                assert ctx.get_option("last_mid_group") == grp
+                assert np.array_equal(E1.view(np.uint32), E.view(np.uint32)) and np.array_equal(i1, idx), grp
+            ctx.set_option("mid_group", 0)
+            lig.close()
+        assert worst > 32  # the unpruned transforms ran too
+    finally:
+        ctx.set_option("mid_group", 0)
+        ctx.set_option("pipeline", -1)
 
 
 def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
@@ -162,10 +203,13 @@ def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
     E0, i0, _ = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
     v = ctx.get_option("last_fwd_variant")
     vi = ctx.get_option("last_inv_variant")
-    assert v in (0, 1) and vi in (0, 1) and ctx.get_option("last_mid_group") in (4, 8, 16, 32)
+    assert v in (0, 1) and vi in (0, 1) and ctx.get_option("last_mid_group") in (1, 2, 4, 8, 16, 32)
+    pipe = ctx.get_option("last_pipeline")
+    assert pipe in (0, 1)
     # a different batch shape keeps the forward form of this grid size
     E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"][:1], L["quats"][:1, :7])
     assert ctx.get_option("last_fwd_variant") == v and ctx.get_option("last_inv_variant") == vi
+    assert ctx.get_option("last_pipeline") == pipe
     assert np.array_equal(E1.view(np.uint32), E0[:1, :7].view(np.uint32)) and np.array_equal(i1, i0[:1, :7])
     ctx.set_option("fwd_variant", v)
     ctx.set_option("inv_variant", vi)
