@@ -1489,15 +1489,20 @@ __global__ __launch_bounds__(kBlock) void k_fwd_z(const PoseSlot* __restrict__ s
 // store of the window rows.  Lane lines: 4 wavefronts x 64/L lines per
 // pass, N / (4 * 64/L) passes.  When the box fits in N/2 along a transform's
 // input axis the upper half is neither loaded nor transformed in the first
-// DFT stage.  Workgroup w runs on XCD w % 8 and takes kz = w % 8 + 8 j, so
-// each XCD's L2 holds the protein spectra of every eighth kz plane only.
-template <class P>
+// DFT stage.  HALF: the kernel takes only the poses whose box fits in N/2
+// along x, so the plane needs only N/2 rows (x < N/2 forward; the inverse
+// passes it x' rows N/2 at a time) and four workgroups fit a CU; the full
+// form takes the others (skip_small) or all.  Both round identically, so
+// the form a pose goes through does not change its result.  Workgroup w
+// runs on XCD w % 8 and takes kz = w % 8 + 8 j, so each XCD's L2 holds the
+// protein spectra of every eighth kz plane only.
+template <class P, bool HALF>
 __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ slots,
                                                    const LigandDesc* __restrict__ ligs,
                                                    const RotMeta* __restrict__ meta, int nslots, int group,
                                                    const float2* __restrict__ Z, int cmax, int boxmax,
                                                    const float2* __restrict__ P2, float2* __restrict__ T2, float invG,
-                                                   const float2* __restrict__ twg) {
+                                                   const float2* __restrict__ twg, int skip_small) {
   constexpr int N = P::N, ZH = P::ZH;
   using RS = RegShape<N>;
   constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
@@ -1507,7 +1512,8 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
   constexpr int NPASS = N / LPP;
   constexpr int KZX = (ZH + 7) / 8;
   __shared__ float2 sbuf[4][RS::SLICE];
-  __shared__ float2 pl[N * N];
+  constexpr int PROWS = HALF ? N / 2 : N;  // plane rows
+  __shared__ float2 pl[PROWS * N];
   __shared__ float2 stw[L * V];
   const int w = blockIdx.x;
   const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
@@ -1528,8 +1534,9 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
     const int lig = uni(slots[s].lig);
     const int nch = uni(ligs[lig].nch);
     const int bx = uni(meta[s].box[0]), by = uni(meta[s].box[1]);
+    if (HALF ? bx > N / 2 : (skip_small && bx <= N / 2)) continue;  // the other form's pose
     const int wx = uni(meta[s].win[0]), wy = uni(meta[s].win[1]);
-    const bool xhalf = bx <= N / 2, yhalf = by <= N / 2;
+    const bool xhalf = HALF || bx <= N / 2, yhalf = by <= N / 2;
     float2 acc[NPASS][V];
 #pragma unroll
     for (int p = 0; p < NPASS; ++p)
@@ -1623,29 +1630,36 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
       for (int i = 0; i < V; ++i) acc[p][i] = c_scale(acc[p][i], invG);
       reg_fft<N, 1>(acc[p], twr, tr, ln, n2);
     }
-    __syncthreads();  // the plane's forward reads are done
-#pragma unroll
-    for (int p = 0; p < NPASS; ++p) {
-      const ColBases wb = col_line_k<N>(p * LPP + wave * LPW + ln, n2);
-#pragma unroll
-      for (int q = 0; q < V; ++q) pl[col_at_k<N>(wb, q)] = acc[p][q];
-    }
-    __syncthreads();
+    // Back through the plane, N / ROUNDS rows x' at a time, then inverse y
+    // transforms of the window's x' lines.
+    constexpr int ROUNDS = N / PROWS;
+    constexpr int QR = V / ROUNDS;  // row groups (values) per round
     float2* tb = T2 + ((size_t)s * ZH + kz) * (size_t)(N * N);
 #pragma unroll
-    for (int p = 0; p < NPASS; ++p) {
-      const int x0 = p * LPP + wave * LPW;
-      if (x0 < wx) {
-        const int xp = x0 + ln;
-        float2 v[V];
-        const ColBases rb = col_line_x<N>(xp, n2);
+    for (int r = 0; r < ROUNDS; ++r) {
+      __syncthreads();  // the plane's previous reads are done
 #pragma unroll
-        for (int n1 = 0; n1 < V; ++n1) v[n1] = pl[col_at_x<N>(rb, n1)];
-        reg_fft<N, 1>(v, twr, tr, ln, n2);
-        if (xp < wx) {
+      for (int p = 0; p < NPASS; ++p) {
+        const ColBases wb = col_line_k<N>(p * LPP + wave * LPW + ln, n2);
 #pragma unroll
-          for (int q = 0; q < V; ++q)
-            if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
+        for (int q = 0; q < QR; ++q) pl[col_at_k<N>(wb, q)] = acc[p][r * QR + q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < NPASS; ++p) {
+        const int x0 = p * LPP + wave * LPW;
+        if (x0 >= r * PROWS && x0 < (r + 1) * PROWS && x0 < wx) {
+          const int xp = x0 + ln;
+          float2 v[V];
+          const ColBases rb = col_line_x<N>(xp - r * PROWS, n2);
+#pragma unroll
+          for (int n1 = 0; n1 < V; ++n1) v[n1] = pl[col_at_x<N>(rb, n1)];
+          reg_fft<N, 1>(v, twr, tr, ln, n2);
+          if (xp < wx) {
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+              if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
+          }
         }
       }
     }
@@ -1965,9 +1979,15 @@ hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const Lig
   const int kzx = (N / 2 + 1 + 7) / 8;
   const dim3 grid((unsigned)(8 * kzx) * ceil_div(nslots, group));
   CDK_DISPATCH_N(N, {
-    if constexpr (PL::N == 64)
-      hipLaunchKernelGGL(k_mid_kz<PL>, grid, dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, Z, cmax, boxmax,
-                         P2, T2, invG, tw);
+    if constexpr (PL::N == 64) {
+      // poses whose box fits in N/2 along x go through the half-plane form,
+      // the others (if the launch can hold any) through the full form
+      hipLaunchKernelGGL((k_mid_kz<PL, true>), grid, dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, Z, cmax,
+                         boxmax, P2, T2, invG, tw, 1);
+      if (2 * boxmax > N)
+        hipLaunchKernelGGL((k_mid_kz<PL, false>), grid, dim3(kBlock), 0, st, slots, ligs, meta, nslots, group, Z,
+                           cmax, boxmax, P2, T2, invG, tw, 1);
+    }
   });
   return hipGetLastError();
 }
