@@ -639,12 +639,20 @@ __device__ __forceinline__ void fwd_plane_rows(int x, const SpreadAtom* __restri
   constexpr int L = RS_::L, V = RS_::V, LPW = RS_::LPW, RSTR = FwdRegShape<N>::RSTR;
   constexpr bool PRUNE = V == 8 || V == 16;
   const int ln = lane % LPW, n2 = lane / LPW;
-  const int zlim = zhalf ? N / 2 : N;
+  const int zsh = zhalf ? 31 - __builtin_clz(N / 2) : 31 - __builtin_clz(N);  // log2 of the row length zeroed
   const int dy = (lane >> 1) & 1, dz = lane & 1;
+  static_assert((N & (N - 1)) == 0 || !PRUNE, "pruned lengths are powers of two");
   __syncthreads();  // atoms staged; the previous plane's reads of the tile are done
-  for (int t = threadIdx.x; t < npair * zlim; t += blockDim.x) {
-    const int p = t / zlim;
-    zt[p * RSTR + (t - p * zlim)] = make_float2(0.f, 0.f);
+  if ((N & (N - 1)) == 0) {
+    for (int t = threadIdx.x; t < (npair << zsh); t += blockDim.x) {
+      const int p = t >> zsh;
+      zt[p * RSTR + (t - (p << zsh))] = make_float2(0.f, 0.f);
+    }
+  } else {
+    for (int t = threadIdx.x; t < npair * N; t += blockDim.x) {
+      const int p = t / N;
+      zt[p * RSTR + (t - p * N)] = make_float2(0.f, 0.f);
+    }
   }
   __syncthreads();
   if (wave == 0) {
@@ -1430,7 +1438,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_z(const PoseSlot* __restrict__ s
   constexpr int N = P::N, ZH = P::ZH;
   using RS_ = RegShape<N>;
   constexpr int L = RS_::L, V = RS_::V, RSTR = FwdRegShape<N>::RSTR;
-  static_assert(V == 8, "column form: 8 values per lane");
+  static_assert(V == 8 && L == 8, "column form: 8 lanes x 8 values per line");
   __shared__ float2 sbuf[4][RS_::SLICE];
   extern __shared__ double dyn[];  // [natom_lds] SpreadAtom, then the tile [npmax][RSTR]
   const int c = blockIdx.x, s = blockIdx.y;
@@ -1470,13 +1478,17 @@ __global__ __launch_bounds__(kBlock) void k_fwd_z(const PoseSlot* __restrict__ s
   float2* zb = Z + ((size_t)s * cmax + c) * ((size_t)ZH * r1 * boxmax * L);
   for (int x = 0; x < bxm; ++x) {
     fwd_plane_rows<N>(x, sa, nc, zt, npair, zhalf, twr, sbuf[wave], wave, lane);
-    for (int t = threadIdx.x; t < ZH * by; t += blockDim.x) {
-      const int kz = t / by, y = t - (t / by) * by;
+    // thread t takes rows y = t % 32 (+ 32) of bins kz = t / 32 (+ 8 j):
+    // power-of-two index math only
+    for (int y = threadIdx.x & 31; y < by; y += 32) {
       const int p = y >> 1;
-      const float2 zk = zt[p * RSTR + kz], zm = zt[p * RSTR + (kz == 0 ? 0 : N - kz)];
-      const float2 v = (y & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
-                               : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-      zb[((size_t)(kz * r1 + y / L) * boxmax + x) * L + (y % L)] = v;
+      float2* dst = zb + ((size_t)(y >> 3) * boxmax + x) * L + (y & 7);
+      for (int kz = threadIdx.x >> 5; kz < ZH; kz += kBlock / 32) {
+        const float2 zk = zt[p * RSTR + kz], zm = zt[p * RSTR + (kz == 0 ? 0 : N - kz)];
+        const float2 v = (y & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
+                                 : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        dst[(size_t)kz * r1 * boxmax * L] = v;
+      }
     }
   }
 }
@@ -1543,36 +1555,47 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
 #pragma unroll
       for (int i = 0; i < V; ++i) acc[p][i] = make_float2(0.f, 0.f);
     const int zrow = boxmax * L * (int)sizeof(float2);  // bytes per row group n1
+    constexpr int NPY = HALF ? 1 : NPASS;                // passes of x lines (x < bx)
+    // Z rows of channel c for the lines of pass p: the kz block of Z; lanes
+    // outside the box read zero through the descriptor (offset past its
+    // end), with no memory traffic.
+    auto zfetch = [&](int c, int p, float2* dst) {
+      int ln = lane % LPW, n2 = lane / LPW;
+      asm volatile("" : "+v"(ln), "+v"(n2));
+      const __amdgpu_buffer_rsrc_t zr =
+          buffer_rsrc(Z + ((size_t)s * cmax + c) * zslab + (size_t)kz * r1 * boxmax * L, (unsigned)(r1 * zrow));
+      const int x = p * LPP + wave * LPW + ln;
+      const int voff = x < bx ? (x * L + n2) * (int)sizeof(float2) : kVoffNone;
+#pragma unroll
+      for (int n1 = 0; n1 < V; ++n1)
+        dst[n1] = (n1 < V / 2 || !yhalf) ? buffer_load_f2(zr, n2 + L * n1 < by ? voff + n1 * zrow : kVoffNone)
+                                         : make_float2(0.f, 0.f);
+    };
+    // The next channel's rows are in flight while this channel is transformed.
+    float2 zp[NPY][V];
+#pragma unroll
+    for (int p = 0; p < NPY; ++p)
+      if (p * LPP + wave * LPW < bx) zfetch(0, p, zp[p]);
     for (int c = 0; c < nch; ++c) {
       // Lane coordinates pass through an empty asm each iteration: the
       // address arithmetic below is then redone here instead of being
       // hoisted out of the loops into dozens of live registers.
       int ln = lane % LPW, n2 = lane / LPW;
       asm volatile("" : "+v"(ln), "+v"(n2));
-      // the kz block of Z; lanes outside the box read zero through the
-      // descriptor (offset past its end), with no memory traffic
-      const __amdgpu_buffer_rsrc_t zr =
-          buffer_rsrc(Z + ((size_t)s * cmax + c) * zslab + (size_t)kz * r1 * boxmax * L, (unsigned)(r1 * zrow));
       __syncthreads();  // the plane's previous reads are done
 #pragma unroll
-      for (int p = 0; p < NPASS; ++p) {
+      for (int p = 0; p < NPY; ++p) {
         const int x0 = p * LPP + wave * LPW;
         if (x0 < bx) {
           const int x = x0 + ln;
-          const int voff = x < bx ? (x * L + n2) * (int)sizeof(float2) : kVoffNone;
-          auto zload = [&](int n1) {
-            return buffer_load_f2(zr, n2 + L * n1 < by ? voff + n1 * zrow : kVoffNone);
-          };
           float2 v[V];
-          if (yhalf) {
 #pragma unroll
-            for (int n1 = 0; n1 < V / 2; ++n1) v[n1] = zload(n1);
+          for (int n1 = 0; n1 < V; ++n1) v[n1] = zp[p][n1];
+          if (c + 1 < nch) zfetch(c + 1, p, zp[p]);
+          if (yhalf)
             reg_fft<N, -1, true>(v, twr, tr, ln, n2);
-          } else {
-#pragma unroll
-            for (int n1 = 0; n1 < V; ++n1) v[n1] = zload(n1);
+          else
             reg_fft<N, -1, false>(v, twr, tr, ln, n2);
-          }
           const ColBases wb = col_line_x<N>(x, n2);
 #pragma unroll
           for (int q = 0; q < V; ++q) pl[col_at_x<N>(wb, q)] = v[q];
