@@ -41,6 +41,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "poses scored/sec at 64³×9ch FP32; p50/p99 request latency in serve loop"
 PEAK_HBM_GBPS = 8000.0
+PEAK_VALU_FP32_TFLOPS = 78.6  # MI355X FP32 vector ALUs, unpacked: 256 CUs x 64 lanes x 2 flops x 2.4 GHz
 
 
 def log(*a):
@@ -86,6 +87,31 @@ def fused_kernel_bytes(geoms, npch, N, mid_launches, columns=False):
     if columns:
         return {"prep": 0.0, "fwd_z": fwd, "mid_kz": mid, "inv_z": inv}
     return {"prep": 0.0, "fwd_planes": fwd, "mid_lines": mid, "inv_argmin": inv}
+
+
+def mid_flops(geoms, N, columns):
+    """FFT-convention FP32 flops of the mid kernel over a set of poses: a
+    complex line transform of n points counts 5 n log2 n, a complex
+    multiply-accumulate 8, a complex scale 2.
+      column form, per pose and kz plane: y lines x < box_x and all N x
+        lines per channel, N*N products per channel, then N inverse x lines,
+        win_x inverse y lines and N*N scales;
+      plane form, per pose: N*(N/2+1) x lines and their products per channel,
+        then the inverse x lines and scales."""
+    F = 5.0 * N * np.log2(N)
+    ZH = N // 2 + 1
+    total = 0.0
+    for box, win, nch in geoms:
+        b = box.reshape(-1, 3).astype(np.float64)
+        w = win.reshape(-1, 3).astype(np.float64)
+        valid = (w > 0).all(axis=1)
+        b, w = b[valid], w[valid]
+        if columns:
+            per = ZH * (nch * ((b[:, 0] + N) * F + 8 * N * N) + (N + w[:, 0]) * F + 2 * N * N)
+        else:
+            per = np.full(len(b), N * ZH * (nch * (F + 8 * N) + F + 2 * N))
+        total += float(per.sum())
+    return total
 
 
 def reference_bytes(N, C):
@@ -225,6 +251,13 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
             "algorithmic_bytes_per_launch": round(dom_bytes)}
+    if dom in ("mid_kz", "mid_lines"):
+        # The transforms run on the vector ALUs in FP32 without packed
+        # instructions: 78.6 TFLOP/s is that ceiling (half the listed
+        # 157.3 TFLOP/s FP32 matrix-core peak).
+        tf = mid_flops(geoms, N, columns) / kernels[dom]["launches"] / dom_avg_s / 1e12
+        roof["valu"] = {"achieved": round(tf, 2), "peak": PEAK_VALU_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tf / PEAK_VALU_FP32_TFLOPS, 4)}
     if columns:
         kernels["tuned"] = {"pipeline": "columns", "mid_group": group, "max_dyn_lds": ctx.get_option("max_dyn_lds")}
     else:
