@@ -41,6 +41,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "poses scored/sec at 64³×9ch FP32; p50/p99 request latency in serve loop"
 PEAK_HBM_GBPS = 8000.0
+PEAK_MFMA_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix cores
 PEAK_VALU_FP32_TFLOPS = 78.6  # MI355X FP32 vector ALUs, unpacked: 256 CUs x 64 lanes x 2 flops x 2.4 GHz
 
 
@@ -48,7 +49,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def fused_kernel_bytes(geoms, npch, N, mid_launches, columns=False):
+def fused_kernel_bytes(geoms, npch, N, mid_launches, columns=False, atoms=False):
     """Algorithmic HBM bytes of each fused kernel over a set of poses.
 
     geoms: [(box, win, nch)] per job, box/win [..., 3] per pose, nch the
@@ -59,6 +60,9 @@ def fused_kernel_bytes(geoms, npch, N, mid_launches, columns=False):
       mid lines:      reads those planes, writes win_x planes (x-inverse done),
                       and reads the protein spectra (npch*H*8) once per launch
       inverse+argmin: reads win_x planes per pose, writes one 8-byte key
+    Atom form: no forward stage; the mid stage reads the atoms (nothing
+      compared with the rest), writes win_x*win_y rows and reads the protein
+      spectra once per launch; inverse z as the column form.
     Column form:
       forward z:      writes nch*box_x*box_y rows per pose (z spectra)
       mid kz:         reads those rows, writes win_x*win_y rows (x and y
@@ -69,12 +73,15 @@ def fused_kernel_bytes(geoms, npch, N, mid_launches, columns=False):
     row = (N // 2 + 1) * 8
     H8 = N * N * (N // 2 + 1) * 8
     fwd = mid = inv = 0.0
-    for box, win, nch in geoms:
+    for box, win, nch, *_ in geoms:
         b = box.reshape(-1, 3).astype(np.float64)
         w = win.reshape(-1, 3).astype(np.float64)
         valid = (w > 0).all(axis=1)
         b, w = b[valid], w[valid]
-        if columns:
+        if atoms:
+            lig = np.zeros(len(b))
+            out = w[:, 0] * w[:, 1] * row
+        elif columns:
             lig = nch * b[:, 0] * b[:, 1] * row
             out = w[:, 0] * w[:, 1] * row
         else:
@@ -84,6 +91,8 @@ def fused_kernel_bytes(geoms, npch, N, mid_launches, columns=False):
         mid += float((lig + out).sum())
         inv += float((out + 8).sum())
     mid += npch * H8 * mid_launches
+    if atoms:
+        return {"prep": 0.0, "mid_atoms": mid, "inv_z": inv}
     if columns:
         return {"prep": 0.0, "fwd_z": fwd, "mid_kz": mid, "inv_z": inv}
     return {"prep": 0.0, "fwd_planes": fwd, "mid_lines": mid, "inv_argmin": inv}
@@ -101,7 +110,7 @@ def mid_flops(geoms, N, columns):
     F = 5.0 * N * np.log2(N)
     ZH = N // 2 + 1
     total = 0.0
-    for box, win, nch in geoms:
+    for box, win, nch, *_ in geoms:
         b = box.reshape(-1, 3).astype(np.float64)
         w = win.reshape(-1, 3).astype(np.float64)
         valid = (w > 0).all(axis=1)
@@ -111,6 +120,22 @@ def mid_flops(geoms, N, columns):
         else:
             per = np.full(len(b), N * ZH * (nch * (F + 8 * N) + F + 2 * N))
         total += float(per.sum())
+    return total
+
+
+def atom_mfma_flops(geoms, N):
+    """Flops of the atom form's matrix-core products over a set of poses: per
+    pose, kz plane and channel atom, a (N x 2) by (2 x 2N) real product
+    (N kx rows; real and imaginary k slots; 2N columns for the real and
+    imaginary parts of the N ky values), i.e. 8 N^2 flops.  nent (the fourth
+    item of a geometry) counts the ligand's channel atoms: every atom is in
+    one vdW channel and in the elec channel."""
+    ZH = N // 2 + 1
+    total = 0.0
+    for box, win, nch, nent in geoms:
+        w = win.reshape(-1, 3)
+        nvalid = int((w > 0).all(axis=1).sum())
+        total += float(nvalid) * ZH * nent * 8.0 * N * N
     return total
 
 
@@ -234,16 +259,22 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     ms, launches = ctx.stage_times(reset=True)
     ctx.set_option("time_stages", 0)
     group = ctx.get_option("last_mid_group")
-    columns = ctx.get_option("last_pipeline") == 1
-    names = ["prep", "fwd_z", "mid_kz", "inv_z"] if columns else ["prep", "fwd_planes", "mid_lines", "inv_argmin"]
-    kb = fused_kernel_bytes(geoms, npch, N, int(launches[2]), columns)
+    pipe = ctx.get_option("last_pipeline")
+    columns, atoms = pipe == 1, pipe == 2
+    if atoms:
+        names = ["prep", None, "mid_atoms", "inv_z"]
+    elif columns:
+        names = ["prep", "fwd_z", "mid_kz", "inv_z"]
+    else:
+        names = ["prep", "fwd_planes", "mid_lines", "inv_argmin"]
+    kb = fused_kernel_bytes(geoms, npch, N, int(launches[2]), columns, atoms)
     kernels = {}
     for i, nm in enumerate(names):
-        if launches[i]:
+        if nm and launches[i]:
             kernels[nm] = {"ms_total": round(float(ms[i]), 3), "launches": int(launches[i]),
                            "avg_ms": round(float(ms[i] / launches[i]), 4),
                            "GBps": round(kb[nm] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None}
-    dom = max(names, key=lambda k: kernels.get(k, {}).get("ms_total", 0))
+    dom = max([nm for nm in names if nm], key=lambda k: kernels.get(k, {}).get("ms_total", 0))
     dom_avg_s = kernels[dom]["ms_total"] * 1e-3 / kernels[dom]["launches"]
     dom_bytes = kb[dom] / kernels[dom]["launches"]
     achieved = dom_bytes / dom_avg_s / 1e9
@@ -251,6 +282,15 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
             "algorithmic_bytes_per_launch": round(dom_bytes)}
+    if dom == "mid_atoms":
+        # the ligand spectra are FP32 matrix-core products: the algorithmic
+        # flops of those products against the FP32 matrix-core peak
+        tf = atom_mfma_flops(geoms, N) / kernels[dom]["launches"] / dom_avg_s / 1e12
+        hbm = dict(roof)
+        roof = {"bound": "mfma", "kernel": dom, "achieved": round(tf, 2), "peak": PEAK_MFMA_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tf / PEAK_MFMA_FP32_TFLOPS, 4), "traffic": None,
+                "avg_launch_ms": hbm["avg_launch_ms"], "algorithmic_bytes_per_launch": hbm["algorithmic_bytes_per_launch"],
+                "hbm": {"achieved": hbm["achieved"], "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": hbm["frac"]}}
     if dom in ("mid_kz", "mid_lines"):
         # The transforms run on the vector ALUs in FP32 without packed
         # instructions: 78.6 TFLOP/s is that ceiling (half the listed
@@ -258,8 +298,9 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
         tf = mid_flops(geoms, N, columns) / kernels[dom]["launches"] / dom_avg_s / 1e12
         roof["valu"] = {"achieved": round(tf, 2), "peak": PEAK_VALU_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(tf / PEAK_VALU_FP32_TFLOPS, 4)}
-    if columns:
-        kernels["tuned"] = {"pipeline": "columns", "mid_group": group, "max_dyn_lds": ctx.get_option("max_dyn_lds")}
+    if columns or atoms:
+        kernels["tuned"] = {"pipeline": "atoms" if atoms else "columns", "mid_group": group,
+                            "max_dyn_lds": ctx.get_option("max_dyn_lds")}
     else:
         kernels["tuned"] = {"pipeline": "planes",
                             "fwd_variant": ["block", "register"][ctx.get_option("last_fwd_variant")],
@@ -367,7 +408,7 @@ def run_single(args, cdk, workloads):
     E, idx, mc = batch.results(0)
     box, win = batch.geometry(0)
     n_valid = int(np.isfinite(E).sum())
-    kernels, roof = kernel_leg(ctx, cdk, batch, [(box, win, C)], prot.num_grids, N)
+    kernels, roof = kernel_leg(ctx, cdk, batch, [(box, win, C, 2 * lig.natom)], prot.num_grids, N)
     tr = profiled_traffic(roof["kernel"], n_valid / kernels[roof["kernel"]]["launches"]) if args.protein == "p64" else None
     if tr is not None:
         roof["traffic"] = round(tr)
@@ -433,7 +474,7 @@ def run_batched(args, cdk, workloads):
     n_valid = 0
     for j, lig in enumerate(ligs):
         box, win = batch.geometry(j)
-        geoms.append((box, win, lig.channels()[0] + 1))
+        geoms.append((box, win, lig.channels()[0] + 1, 2 * lig.natom))
         n_valid += int(np.isfinite(batch.results(j)[0]).sum())
     kernels, roof = kernel_leg(ctx, cdk, batch, geoms, prot.num_grids, N)
 
@@ -486,7 +527,7 @@ def run_large(args, cdk, workloads):
     value = npose * args.steps / total
     E, idx, mc = batch.results(0)
     box, win = batch.geometry(0)
-    kernels, roof = kernel_leg(ctx, cdk, batch, [(box, win, C)], prot.num_grids, N)
+    kernels, roof = kernel_leg(ctx, cdk, batch, [(box, win, C, 2 * lig.natom)], prot.num_grids, N)
 
     # Top-10 poses per conformer (sort_index + pose rebuild, NoMini:466-487).
     ts = time.perf_counter()

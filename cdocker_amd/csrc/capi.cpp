@@ -162,7 +162,7 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
       require(v == 0 || v == 1 || v == 2 || v == 4, "mid_poses must be 0, 1, 2 or 4");
       c.mid_poses = (int)v;
     } else if (k == "pipeline") {
-      require(v >= -1 && v <= 1, "pipeline must be -1 (tuned), 0 (plane form) or 1 (column form)");
+      require(v >= -1 && v <= 2, "pipeline must be -1 (tuned), 0 (plane form), 1 (column form) or 2 (atom form)");
       c.pipeline = (int)v;
     } else if (k == "fwd_variant") {
       require(v >= -1 && v <= 1, "fwd_variant must be -1 (tuned), 0 (block) or 1 (register)");
@@ -294,7 +294,7 @@ int cdk_protein_spectrum(cdk_protein* p, int channel, float* out) {
 int cdk_protein_bytes(cdk_protein* p, int64_t* bytes) {
   return guard([&] {
     require(p && bytes, "NULL argument");
-    *bytes = (int64_t)(p->p.grid.cap + p->p.spec.cap + p->p.spec_lb.cap + p->p.spec_kz.cap);
+    *bytes = (int64_t)(p->p.grid.cap + p->p.spec.cap + p->p.spec_lb.cap + p->p.spec_kz.cap + p->p.spec_mf.cap);
   });
 }
 

@@ -109,7 +109,8 @@ void protein_setup(Protein& p, int nch, int N, float h, const float* mid, const 
 }
 
 // Forward R2C of every channel (NoMini:177-199) plus the copies the fused
-// mid stages read (line-blocked; kz-plane blocks at grid edge 64).
+// mid stages read (line-blocked; kz-plane and matrix-core blocks at grid
+// edge 64).
 void protein_spectra(Protein& p) {
   Context& ctx = *p.ctx;
   const size_t H = p.H();
@@ -123,6 +124,11 @@ void protein_spectra(Protein& p) {
     p.spec_kz.ensure((size_t)p.nch * H * sizeof(float2));
     hip_check(launch_to_kz_blocks(ctx.stream, p.N, p.spec.as<float2>(), p.spec_kz.as<float2>(), p.nch),
               "protein spectra kz blocks");
+  }
+  if (atoms_supported(p.N)) {
+    p.spec_mf.ensure((size_t)p.nch * H * sizeof(float2));
+    hip_check(launch_to_mfma_blocks(ctx.stream, p.N, p.spec.as<float2>(), p.spec_mf.as<float2>(), p.nch),
+              "protein spectra matrix-core blocks");
   }
   hip_check(hipStreamSynchronize(ctx.stream), "protein R2C sync");
 }
@@ -471,7 +477,8 @@ struct Tuned {
   int pipe = kPipePlanes;
   int fwd = kFwdBlock, inv = kInvBlock, group = 16, depth = 1, poses = 1;  // plane form
   int col_group = 4;                                                       // column form
-  int mid_group() const { return pipe == kPipeColumns ? col_group : group; }
+  int atom_group = 4;                                                      // atom form
+  int mid_group() const { return pipe == kPipeColumns ? col_group : pipe == kPipeAtoms ? atom_group : group; }
 };
 
 // Autotuner, on the first launch set of a search:
@@ -489,21 +496,26 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   const bool fwd_reg_ok = fwd_reg_supported(N);
   const bool reg_ok = inv_reg_supported(N);
   const bool col_ok = col_supported(N) && prot.spec_kz.p != nullptr;
+  const bool atoms_ok = atoms_supported(N) && prot.spec_mf.p != nullptr;
   const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)chunk};
-  bool need_pipe = false, need_fwd = false, need_inv = false, need_group = false, need_col = false;
+  bool need_pipe = false, need_fwd = false, need_inv = false, need_group = false, need_col = false,
+       need_atom = false;
   if (ctx.pipeline >= 0) {
-    tu.pipe = (ctx.pipeline == kPipeColumns && col_ok) ? kPipeColumns : kPipePlanes;
+    tu.pipe = (ctx.pipeline == kPipeColumns && col_ok)  ? kPipeColumns
+              : (ctx.pipeline == kPipeAtoms && atoms_ok) ? kPipeAtoms
+                                                         : kPipePlanes;
   } else {
     auto it = ctx.tuned_pipe.find(N);
     if (it != ctx.tuned_pipe.end())
       tu.pipe = it->second;
-    else if (col_ok)
+    else if (col_ok || atoms_ok)
       need_pipe = true;
     else
       ctx.tuned_pipe[N] = tu.pipe = kPipePlanes;
   }
   const bool planes = need_pipe || tu.pipe == kPipePlanes;
-  const bool cols = need_pipe || tu.pipe == kPipeColumns;
+  const bool cols = (need_pipe && col_ok) || tu.pipe == kPipeColumns;
+  const bool atoms = (need_pipe && atoms_ok) || tu.pipe == kPipeAtoms;
   if (ctx.inv_variant >= 0) {
     tu.inv = (ctx.inv_variant == kInvReg && reg_ok) ? kInvReg : kInvBlock;
   } else {
@@ -533,6 +545,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     tu.depth = ctx.mid_depth > 0 ? std::min(ctx.mid_depth, max_depth) : std::min(2, max_depth);
     tu.poses = ctx.mid_poses > 0 ? std::min(ctx.mid_poses, max_poses) : 1;
     tu.col_group = ctx.mid_group;
+    tu.atom_group = ctx.mid_group;
   } else {
     if (planes) {
       auto it = ctx.tuned_group.find(gkey);
@@ -551,8 +564,15 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       else
         need_col = true;
     }
+    if (atoms) {
+      auto it = ctx.tuned_atom_group.find(gkey);
+      if (it != ctx.tuned_atom_group.end())
+        tu.atom_group = it->second;
+      else
+        need_atom = true;
+    }
   }
-  if (!need_pipe && !need_fwd && !need_inv && !need_group && !need_col) return tu;
+  if (!need_pipe && !need_fwd && !need_inv && !need_group && !need_col && !need_atom) return tu;
 
   const int n = (int)std::min(chunk, t.slots.size());
   const float2* tw = ctx.twiddle(N);
@@ -597,9 +617,9 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     hip_check(launch_inv_argmin(ctx.stream, N, variant, d_meta, n, N - 2, ctx.d_T.as<float2>(), d_keys, tw),
               "tune inverse");
   };
-  float plane_ms = 0.f, col_ms = 0.f;
+  float plane_ms = 0.f, col_ms = 0.f, atom_ms = 0.f;
   try {
-    if (need_fwd || need_inv || need_group || need_pipe) {
+    if (need_fwd || need_inv || need_group || (need_pipe && planes)) {
       if (need_fwd) {
         const float tb = timed([&] { fwd(kFwdBlock); });
         const float twv = timed([&] { fwd(kFwdReg); });
@@ -655,7 +675,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
         plane_ms += timed([&] { inv(tu.inv); });
       }
     }
-    if (need_col || need_pipe) {
+    if (need_col || (need_pipe && cols)) {
       col_ms += timed([&] {
         hip_check(launch_fwd_z(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
                                t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, t.cmax, t.boxmax,
@@ -687,8 +707,44 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
         hip_check(launch_inv_z(ctx.stream, N, d_meta, n, ctx.d_T.as<float2>(), d_keys, tw), "tune column inverse");
       });
     }
+    if (need_atom || (need_pipe && atoms)) {
+      auto mida = [&](int g) {
+        return timed([&] {
+          hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+                                     t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, n, g,
+                                     prot.spec_mf.as<float2>(), ctx.d_T.as<float2>(), invG, tw),
+                    "tune atom mid");
+        });
+      };
+      if (need_atom) {
+        float best = 1e30f;
+        for (int g : {1, 2, 4, 8}) {
+          const float ms = mida(g);
+          if (ms < best) {
+            best = ms;
+            tu.atom_group = g;
+          }
+        }
+        ctx.tuned_atom_group[gkey] = tu.atom_group;
+        atom_ms += best;
+      } else {
+        atom_ms += mida(tu.atom_group);
+      }
+      atom_ms += timed([&] {
+        hip_check(launch_inv_z(ctx.stream, N, d_meta, n, ctx.d_T.as<float2>(), d_keys, tw), "tune atom inverse");
+      });
+    }
     if (need_pipe) {
-      tu.pipe = col_ms < plane_ms ? kPipeColumns : kPipePlanes;
+      tu.pipe = kPipePlanes;
+      float best = plane_ms;
+      if (cols && col_ms < best) {
+        best = col_ms;
+        tu.pipe = kPipeColumns;
+      }
+      if (atoms && atom_ms < best) {
+        best = atom_ms;
+        tu.pipe = kPipeAtoms;
+      }
       ctx.tuned_pipe[N] = tu.pipe;
     }
   } catch (...) {
@@ -723,6 +779,19 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
                           ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
               "prep");
     timer.end(e);
+    if (tu.pipe == kPipeAtoms) {
+      e = timer.begin(2);
+      hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+                                 t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, n, tu.atom_group,
+                                 prot.spec_mf.as<float2>(), ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+                "atom mid");
+      timer.end(e);
+      e = timer.begin(3);
+      hip_check(launch_inv_z(ctx.stream, N, d_meta + s, n, ctx.d_T.as<float2>(), d_keys + s, tw),
+                "atom inverse + argmin");
+      timer.end(e);
+      continue;
+    }
     if (tu.pipe == kPipeColumns) {
       e = timer.begin(1);
       hip_check(launch_fwd_z(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),

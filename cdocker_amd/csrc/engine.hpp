@@ -64,6 +64,7 @@ struct Protein {
   DevBuf spec;               // float2 [nch][H]
   DevBuf spec_lb;            // the same spectra in the fused engine's line-blocked layout
   DevBuf spec_kz;            // and in the column form's kz-plane layout (grid edge 64 only)
+  DevBuf spec_mf;            // and in the atom form's matrix-core layout (grid edge 64 only)
   size_t G() const { return (size_t)N * N * N; }
   size_t H() const { return (size_t)N * N * (N / 2 + 1); }
   // gridMin = mid - len / 2 evaluated in float, then widened (NoMini:149-151).
@@ -160,7 +161,7 @@ struct Context {
   int mid_group = 0;              // poses per mid-stage workgroup; 0 = tuned
   int mid_depth = 0;              // mid-stage register ring depth with a fixed group; 0 = 2
   int mid_poses = 0;              // poses sharing each protein load in the mid stage with a fixed group; 0 = 1
-  int pipeline = -1;              // fused pipeline form (kPipePlanes, kPipeColumns); -1 = tuned
+  int pipeline = -1;              // fused pipeline form (kPipePlanes, kPipeColumns, kPipeAtoms); -1 = tuned
   int fwd_variant = -1;           // forward plane stage form; -1 = tuned
   int inv_variant = -1;           // inverse plane stage form; -1 = tuned
   bool use_graph = true;
@@ -176,7 +177,8 @@ struct Context {
   std::map<int, int> tuned_inv;                      // N -> variant
   // (N, cmax, boxmax, chunk) -> (group, depth, poses per protein load)
   std::map<std::array<int, 4>, std::array<int, 3>> tuned_group;
-  std::map<std::array<int, 4>, int> tuned_col_group;  // (N, cmax, boxmax, chunk) -> column form group
+  std::map<std::array<int, 4>, int> tuned_col_group;   // (N, cmax, boxmax, chunk) -> column form group
+  std::map<std::array<int, 4>, int> tuned_atom_group;  // (N, cmax, boxmax, chunk) -> atom form group
   int last_fwd = -1, last_inv = -1, last_group = 0, last_depth = 0, last_poses = 0,
       last_pipe = -1;
   StageTimes stage_times;

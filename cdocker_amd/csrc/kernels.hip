@@ -1493,6 +1493,91 @@ __global__ __launch_bounds__(kBlock) void k_fwd_z(const PoseSlot* __restrict__ s
   }
 }
 
+// Inverse part of the column-form mid stages, for one pose and kz plane:
+// acc holds the product spectrum on the lane's ky lines (pass p line
+// p*LPP + wave*LPW + ln, value m at kx = n2 + L*m).  1/G, inverse x
+// transforms, back through the plane PROWS rows x' at a time, inverse y
+// transforms of the window's x' lines, store of the window rows into tb
+// ([m][x'][n2], y' = n2 + L*m).  Starts with a block barrier before its first
+// plane write; all threads of the workgroup call it.  TR_IN_PLANE: the
+// transforms' scratch tr lies inside the plane, so every transform waits
+// until no wavefront reads the plane any more.
+template <int N, int PROWS, bool TR_IN_PLANE = false>
+__device__ __forceinline__ void col_inverse(float2 (&acc)[N / (4 * RegShape<N>::LPW)][RegShape<N>::V], float2* pl,
+                                            float2* tr, const float2* twr, int lane, int wave, float2* tb, int wx,
+                                            int wy, float invG) {
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
+  constexpr int LPP = 4 * LPW;
+  constexpr int NPASS = N / LPP;
+  // inverse x transforms of the ky lines: value m = x' = n2 + L*m
+  int ln = lane % LPW, n2 = lane / LPW;
+  asm volatile("" : "+v"(ln), "+v"(n2));
+  if constexpr (TR_IN_PLANE) __syncthreads();  // the plane's reads are done
+#pragma unroll
+  for (int p = 0; p < NPASS; ++p) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) acc[p][i] = c_scale(acc[p][i], invG);
+    reg_fft<N, 1>(acc[p], twr, tr, ln, n2);
+  }
+  // Back through the plane, N / ROUNDS rows x' at a time, then inverse y
+  // transforms of the window's x' lines.
+  constexpr int ROUNDS = N / PROWS;
+  constexpr int QR = V / ROUNDS;  // row groups (values) per round
+#pragma unroll
+  for (int r = 0; r < ROUNDS; ++r) {
+    __syncthreads();  // the plane's previous reads are done
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+      const ColBases wb = col_line_k<N>(p * LPP + wave * LPW + ln, n2);
+#pragma unroll
+      for (int q = 0; q < QR; ++q) pl[col_at_k<N>(wb, q)] = acc[p][r * QR + q];
+    }
+    __syncthreads();
+    auto in_round = [&](int p) {
+      const int x0 = p * LPP + wave * LPW;
+      return x0 >= r * PROWS && x0 < (r + 1) * PROWS && x0 < wx;
+    };
+    auto store = [&](int p, const float2* v) {
+      const int xp = p * LPP + wave * LPW + ln;
+      if (xp < wx) {
+#pragma unroll
+        for (int q = 0; q < V; ++q)
+          if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
+      }
+    };
+    if constexpr (TR_IN_PLANE) {
+      // read every line first, then transform behind a barrier
+      float2 v[NPASS][V];
+#pragma unroll
+      for (int p = 0; p < NPASS; ++p)
+        if (in_round(p)) {
+          const ColBases rb = col_line_x<N>(p * LPP + wave * LPW + ln - r * PROWS, n2);
+#pragma unroll
+          for (int n1 = 0; n1 < V; ++n1) v[p][n1] = pl[col_at_x<N>(rb, n1)];
+        }
+      __syncthreads();  // the plane's reads are done
+#pragma unroll
+      for (int p = 0; p < NPASS; ++p)
+        if (in_round(p)) {
+          reg_fft<N, 1>(v[p], twr, tr, ln, n2);
+          store(p, v[p]);
+        }
+    } else {
+#pragma unroll
+      for (int p = 0; p < NPASS; ++p)
+        if (in_round(p)) {
+          float2 v[V];
+          const ColBases rb = col_line_x<N>(p * LPP + wave * LPW + ln - r * PROWS, n2);
+#pragma unroll
+          for (int n1 = 0; n1 < V; ++n1) v[n1] = pl[col_at_x<N>(rb, n1)];
+          reg_fft<N, 1>(v, twr, tr, ln, n2);
+          store(p, v);
+        }
+    }
+  }
+}
+
 // Column form, mid: one workgroup per (kz, group of poses).  For each pose
 // and channel: y transforms of the box's x lines into the LDS plane [x][ky],
 // x transforms of all ky lines out of it, product with conj and the protein
@@ -1644,48 +1729,238 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    // inverse x transforms of the ky lines: value m = x' = n2 + L*m
-    int ln = lane % LPW, n2 = lane / LPW;
-    asm volatile("" : "+v"(ln), "+v"(n2));
+    col_inverse<N, PROWS>(acc, pl, tr, twr, lane, wave, T2 + ((size_t)s * ZH + kz) * (size_t)(N * N), wx, wy, invG);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Atom form of the fused pipeline (grid edge 64).  Trilinear spreading is
+// separable, so the DFT of a channel's ligand grid is, per kz plane, a sum of
+// outer products over the channel's atoms:
+//   L_c(kx, ky, kz) = sum_a w_a X_a(kx) Y_a(ky) Z_a(kz),
+//   X_a(k) = (1 - fx) W^(k ix) + fx W^(k (ix + 1)),  W = exp(-2 pi i / N),
+// (Y, Z likewise; w = sqrt(eps) or the charge, FillLigandGrid.cpp:45, :97).
+// conj(L_c) on a kz plane is then a complex matrix product (64 kx) x (atoms)
+// x (64 ky), run on the matrix cores: v_mfma_f32_32x32x2f32 with k = (atom,
+// real/imaginary part), one instruction per atom for the real part of a
+// 32 x 32 block and one for the imaginary part.  The ligand grids, their
+// forward transforms and the forward stage's HBM traffic disappear; the
+// product with the protein spectrum, the channel sum and the inverse x and y
+// transforms follow as in the column form, and k_inv_z finishes.
+//
+// MFMA 32x32x2 f32 lane layout (checked on the device): lane l supplies
+// A[l % 32][l / 32] and B[l / 32][l % 32]; result register r of lane l is
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// P3[c][kz][block b = 2 (kx / 32) + ky / 32][r / 2][l][2]: the protein
+// spectra in that result layout (one 16-byte load gives a lane registers
+// 2q and 2q + 1; a wavefront load reads 1 KB).
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ int mfma_index(int kx, int ky, int kz) {
+  const int row = kx & 31, col = ky & 31;
+  const int l = col + 32 * ((row >> 2) & 1), r = 4 * (row >> 3) + (row & 3);
+  const int b = 2 * (kx >> 5) + (ky >> 5);
+  return (((kz * 4 + b) * 8 + (r >> 1)) * 64 + l) * 2 + (r & 1);
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_to_mfma_blocks(const float2* __restrict__ in, float2* __restrict__ out,
+                                                           int nch) {
+  constexpr int N = P::N, ZH = P::ZH;
+  const size_t per = (size_t)N * N * ZH;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= per * nch) return;
+  const int c = (int)(t / per);
+  const int r = (int)(t - (size_t)c * per);  // [kx][ky][kz] (cuFFT R2C layout)
+  const int kx = r / (N * ZH), ky = (r / ZH) % N, kz = r % ZH;
+  out[(size_t)c * per + (size_t)mfma_index<N>(kx, ky, kz)] = in[t];
+}
+
+typedef float mfma_acc16 __attribute__((ext_vector_type(16)));
+#ifndef CDK_ATOM_GROUP
+#define CDK_ATOM_GROUP 4
+#endif
+
+// Atom form, mid: one workgroup per (kz, group of poses); wavefront w owns
+// the 32 x 32 (kx, ky) block (w / 2, w % 2).  Per pose the ligand's channel
+// atoms (channel order) are staged in LDS, all of them when they fit, else
+// one channel at a time: wZ = the weight times Z_a(kz), fx wZ, ix, iy, fy.  Per
+// channel every wavefront walks the channel's atoms in order,
+// builds its A and B operands from the twiddle table (the next atom's
+// lookups in flight while the current one is multiplied) and accumulates
+// conj(L_c) on the matrix cores; at each channel end it multiplies by the
+// protein spectrum into the pose's sum.
+// The sum goes through the LDS plane into the ky-line register layout and
+// col_inverse finishes.  Per pose the arithmetic order is fixed (atoms in
+// channel order, one accumulation chain per channel), so results do not
+// depend on the group size or on what else shares the launch.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict__ slots,
+                                                      const LigandDesc* __restrict__ ligs,
+                                                      const int* __restrict__ chan_atoms,
+                                                      const float* __restrict__ wv, const float* __restrict__ wq,
+                                                      const AtomGrid* __restrict__ ag, int astride,
+                                                      const RotMeta* __restrict__ meta, int nslots, int group,
+                                                      const float2* __restrict__ P3, float2* __restrict__ T2,
+                                                      float invG, const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
+  static_assert(N == 64 && V == 8 && L == 8, "atom form: 64 points, 8 lanes x 8 values per line");
+  constexpr int LPP = 4 * LPW;
+  constexpr int NPASS = N / LPP;
+  constexpr int KZX = (ZH + 7) / 8;
+  constexpr int ENT = 512;  // channel atoms staged at a time
+  constexpr int AU = CDK_ATOM_GROUP;  // atoms per group in the matrix-core loop
+  // The staged atoms, the plane's contents and the inverse transforms'
+  // scratch are never live together: all three share the plane's memory.
+  constexpr int SCR = 4 * RS::SLICE * (int)sizeof(float2);
+  constexpr int STG = ENT * (int)(2 * sizeof(float4));
+  static_assert(SCR <= N * N * (int)sizeof(float2) && STG <= N * N * (int)sizeof(float2), "fits the plane");
+  __shared__ __align__(16) float2 pl[N * N];
+  unsigned char* scratch = reinterpret_cast<unsigned char*>(pl);
+  __shared__ float2 stw[N];       // W^m
+  __shared__ float2 stwr[L * V];  // register FFT twiddles
+  float4* ef = reinterpret_cast<float4*>(scratch);                      // (w Z(kz), fx w Z(kz))
+  int4* ei = reinterpret_cast<int4*>(scratch + ENT * sizeof(float4));  // (8 ix, 8 iy, fy bits, 0)
+  const int w = blockIdx.x;
+  const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
+  const int sg = (w >> 3) / KZX;
+  if (kz >= ZH) return;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
+  for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int part = lane >> 5;                      // k slot: real (0) or imaginary (1) part
+  const int kxA = 32 * (wave >> 1) + (lane & 31);  // this lane's A row
+  const int kyB = 32 * (wave & 1) + (lane & 31);   // this lane's B column
+  float2* tr = reinterpret_cast<float2*>(scratch) + wave * RS::SLICE;
+  const float2* twr = stwr + (lane / LPW) * V;
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  __syncthreads();
+  // X_a(k) = W^(k ix) (1 - fx (1 - W^k)): per lane, D = 1 - W^k
+  const float2 dx = make_float2(1.f - stw[kxA].x, -stw[kxA].y), dy = make_float2(1.f - stw[kyB].x, -stw[kyB].y);
+  const char* stwb = reinterpret_cast<const char*>(stw);
+  const int s0 = sg * group, s1 = min(nslots, s0 + group);
+  for (int s = s0; s < s1; ++s) {
+    if (!uni(meta[s].valid)) continue;
+    const int lig = uni(slots[s].lig);
+    const LigandDesc& Ld = ligs[lig];
+    const int nch = uni(Ld.nch);
+    const int wx = uni(meta[s].win[0]), wy = uni(meta[s].win[1]);
+    const int list_off = uni(Ld.list_off), w_off = uni(Ld.w_off);
+    const int ne = uni(Ld.ch_start[nch]), e_elec = uni(Ld.ch_start[nch - 1]);
+    float2 E[16];
 #pragma unroll
-    for (int p = 0; p < NPASS; ++p) {
-#pragma unroll
-      for (int i = 0; i < V; ++i) acc[p][i] = c_scale(acc[p][i], invG);
-      reg_fft<N, 1>(acc[p], twr, tr, ln, n2);
-    }
-    // Back through the plane, N / ROUNDS rows x' at a time, then inverse y
-    // transforms of the window's x' lines.
-    constexpr int ROUNDS = N / PROWS;
-    constexpr int QR = V / ROUNDS;  // row groups (values) per round
-    float2* tb = T2 + ((size_t)s * ZH + kz) * (size_t)(N * N);
-#pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-      __syncthreads();  // the plane's previous reads are done
-#pragma unroll
-      for (int p = 0; p < NPASS; ++p) {
-        const ColBases wb = col_line_k<N>(p * LPP + wave * LPW + ln, n2);
-#pragma unroll
-        for (int q = 0; q < QR; ++q) pl[col_at_k<N>(wb, q)] = acc[p][r * QR + q];
+    for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
+    // Stage channel atoms [e_lo, e_hi) into LDS slots 0.. (the whole list when
+    // it fits, else one channel at a time: a channel has at most kMaxAtoms).
+    int st0 = 0, st1 = 0;
+    auto stage = [&](int e_lo, int e_hi) {
+      __syncthreads();  // the previous pose's scratch and staged atoms are done with
+      for (int k = tid; k < e_hi - e_lo; k += blockDim.x) {
+        const int e = e_lo + k;
+        const int a = chan_atoms[list_off + e];
+        const AtomGrid g = ag[(size_t)s * astride + a];
+        // elec channel: the charge; vdW channels: sqrt(eps) (FillLigandGrid.cpp:45, :97)
+        const float wa = e >= e_elec ? wq[w_off + a] : wv[w_off + a];
+        const float2 z0 = stw[(kz * g.iz) & (N - 1)], z1 = stw[(kz * (g.iz + 1)) & (N - 1)];
+        const float2 zf = c_fma_s(z1, (float)g.fz, c_scale(z0, (float)(1.0 - g.fz)));
+        const float2 wz = make_float2(wa * zf.x, wa * zf.y);
+        const float fx = (float)g.fx;
+        ef[k] = make_float4(wz.x, wz.y, fx * wz.x, fx * wz.y);
+        ei[k] = make_int4(8 * g.ix, 8 * g.iy, __float_as_int((float)g.fy), 0);
       }
       __syncthreads();
+      st0 = e_lo;
+      st1 = e_hi;
+    };
+    for (int c = 0; c < nch; ++c) {
+      const int e0 = uni(Ld.ch_start[c]), e1 = uni(Ld.ch_start[c + 1]);
+      if (e1 > st1) stage(e0, ne <= ENT ? ne : e1);
+      mfma_acc16 cr, ci;
 #pragma unroll
-      for (int p = 0; p < NPASS; ++p) {
-        const int x0 = p * LPP + wave * LPW;
-        if (x0 >= r * PROWS && x0 < (r + 1) * PROWS && x0 < wx) {
-          const int xp = x0 + ln;
-          float2 v[V];
-          const ColBases rb = col_line_x<N>(xp - r * PROWS, n2);
+      for (int r = 0; r < 16; ++r) {
+        cr[r] = 0.f;
+        ci[r] = 0.f;
+      }
+      // Atoms in groups of AU: a group's entries and table values are all
+      // in flight together, then its operands are built and multiplied.
+      // A = conj(w Z X) = conj(W^(kx ix) (wZ - fx wZ D)) (one component per
+      // lane: k slot `part`), B = conj(Y) = conj(W^(ky iy) (1 - fy D)).
+      auto atom = [&](int k, float& av, float& br, float& bi) {
+        const float4 f = ef[k];
+        const int4 ii = ei[k];
+        const float2 tx = *reinterpret_cast<const float2*>(stwb + (__umul24(kxA, ii.x) & (8 * N - 8)));
+        const float2 ty = *reinterpret_cast<const float2*>(stwb + (__umul24(kyB, ii.y) & (8 * N - 8)));
+        const float fy = __int_as_float(ii.z);
+        const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));   // wZ - fx wZ D
+        const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
+        const float p1 = part ? -gy : gx, p2 = part ? -gx : -gy;  // (A.x) or (-A.y) of tx g
+        av = fmaf(tx.x, p1, tx.y * p2);
+        const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;    // 1 - fy D
+        const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
+        // conj(Y) = (yx, -yy): real part Br Ar - Bi Ai, imaginary Bi Ar + Br Ai
+        br = part ? yy : yx;
+        bi = part ? yx : -yy;
+      };
+      int k = e0 - st0;
+      const int kend = e1 - st0;
+      for (; k + AU <= kend; k += AU) {
+        float av[AU], br[AU], bi[AU];
 #pragma unroll
-          for (int n1 = 0; n1 < V; ++n1) v[n1] = pl[col_at_x<N>(rb, n1)];
-          reg_fft<N, 1>(v, twr, tr, ln, n2);
-          if (xp < wx) {
+        for (int u = 0; u < AU; ++u) atom(k + u, av[u], br[u], bi[u]);
 #pragma unroll
-            for (int q = 0; q < V; ++q)
-              if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
-          }
+        for (int u = 0; u < AU; ++u) {
+          cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], br[u], cr, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bi[u], ci, 0, 0, 0);
+        }
+      }
+      for (; k < kend; ++k) {
+        float av, br, bi;
+        atom(k, av, br, bi);
+        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av, br, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bi, ci, 0, 0, 0);
+      }
+      // sum += conj(L_c) P_c
+      const int pc = uni(Ld.pmap[c]);
+      const float2* pb = P3 + (((size_t)pc * ZH + kz) * 4 + wave) * (8 * 128) + lane * 2;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float4 pq = *reinterpret_cast<const float4*>(pb + q * 128);  // registers 2q, 2q + 1
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = 2 * q + h;
+          const float px = h ? pq.z : pq.x, py = h ? pq.w : pq.y;
+          E[r].x = fmaf(cr[r], px, E[r].x);
+          E[r].x = fmaf(-ci[r], py, E[r].x);
+          E[r].y = fmaf(cr[r], py, E[r].y);
+          E[r].y = fmaf(ci[r], px, E[r].y);
         }
       }
     }
+    // the sum through the plane into the ky-line register layout
+    __syncthreads();  // the staged atoms' reads are done
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int kx = 32 * (wave >> 1) + 8 * (r >> 2) + 4 * part + (r & 3);
+      pl[kx * N + (kyB ^ ((9 * kx) & 15))] = E[r];
+    }
+    __syncthreads();
+    float2 acc[NPASS][V];
+    {
+      int ln = lane % LPW, n2 = lane / LPW;
+      asm volatile("" : "+v"(ln), "+v"(n2));
+#pragma unroll
+      for (int p = 0; p < NPASS; ++p) {
+        const ColBases rb = col_line_k<N>(p * LPP + wave * LPW + ln, n2);
+#pragma unroll
+        for (int j = 0; j < V; ++j) acc[p][j] = pl[col_at_k<N>(rb, j)];
+      }
+    }
+    col_inverse<N, N, true>(acc, pl, tr, twr, lane, wave, T2 + ((size_t)s * ZH + kz) * (size_t)(N * N), wx, wy,
+                            invG);
   }
 }
 
@@ -2028,6 +2303,35 @@ hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, 
       const unsigned gx = ceil_div(pairs, 4 * RS::LPW);
       hipLaunchKernelGGL(k_inv_z<PL>, dim3(gx, nslots), dim3(kBlock), 0, st, meta, T2, keys, tw);
     }
+  });
+  return hipGetLastError();
+}
+
+bool atoms_supported(int N) { return N == 64; }
+
+hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
+                            const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
+                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* T2, float invG,
+                            const float2* tw) {
+  if (nslots <= 0) return hipSuccess;
+  if (!atoms_supported(N) || group <= 0) return hipErrorInvalidValue;
+  const int kzx = (N / 2 + 1 + 7) / 8;
+  const dim3 grid((unsigned)(8 * kzx) * ceil_div(nslots, group));
+  CDK_DISPATCH_N(N, {
+    if constexpr (PL::N == 64)
+      hipLaunchKernelGGL(k_mid_atoms<PL>, grid, dim3(kBlock), 0, st, slots, ligs, chan_atoms, wv, wq, ag, astride,
+                         meta, nslots, group, P3, T2, invG, tw);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_to_mfma_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch) {
+  const size_t n = (size_t)nch * N * N * (N / 2 + 1);
+  if (n == 0) return hipSuccess;
+  if (!atoms_supported(N)) return hipErrorInvalidValue;
+  CDK_DISPATCH_N(N, {
+    if constexpr (PL::N == 64)
+      hipLaunchKernelGGL(k_to_mfma_blocks<PL>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, in, out, nch);
   });
   return hipGetLastError();
 }

@@ -69,6 +69,18 @@ hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const Lig
 hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, const float2* T2,
                         unsigned long long* keys, const float2* tw);
 hipError_t launch_to_kz_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
+// Atom form (grid edge 64): the mid stage builds each channel's ligand
+// spectrum on a kz plane as a sum of per-atom outer products on the matrix
+// cores (no ligand grids, no forward stage); then as the column form, with
+// k_inv_z last.  P3 is the protein spectra in the matrix-core result layout
+// (launch_to_mfma_blocks).
+constexpr int kPipeAtoms = 2;
+bool atoms_supported(int N);
+hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
+                            const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
+                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* T2, float invG,
+                            const float2* tw);
+hipError_t launch_to_mfma_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
 // Inverse plane stage variants: one workgroup per (pose, window plane) with
 // block-wide LDS FFTs, or one workgroup per (pose, 4 window planes) with
 // register FFTs and one LDS tile of N rows.

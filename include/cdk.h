@@ -63,7 +63,10 @@ int cdk_context_destroy(cdk_context* ctx);
  * form, whose forward stage writes ligand planes after the z and y
  * transforms; 1 column form at 64^3, whose forward stage stops after the z
  * transform and whose mid stage runs the y and x transforms per kz plane;
- * -1 = autotuned once per grid size), "fwd_variant" (forward
+ * 2 atom form at 64^3, whose mid stage builds each channel's ligand
+ * spectrum per kz plane from the atoms as a sum of outer products on the
+ * matrix cores, with no forward stage; -1 = autotuned once per grid size),
+ * "fwd_variant" (forward
  * plane stage: 0 block-wide LDS Stockham FFTs, 1 register four-step FFTs,
  * -1 = autotuned once per grid size), "inv_variant" (inverse
  * plane stage + argmin: 0 block-wide LDS FFTs per plane, 1 register FFTs over

@@ -155,13 +155,15 @@ def test_plane_stage_forms_vs_oracle(ctx, p48, p64, variant):
         ctx.set_option("inv_variant", -1)
 
 
-def test_column_form_vs_oracle(ctx, p64):
+@pytest.mark.parametrize("pipeline", [1, 2])
+def test_column_and_atom_forms_vs_oracle(ctx, p64, pipeline):
     """Column form at 64^3 (forward z transform only; y and x transforms per
-    kz plane in the mid stage; z C2R last) against the oracle, for ligands
-    whose boxes fit in N/2 and ligands whose boxes do not; and the same bits
-    for every mid-stage group size."""
+    kz plane in the mid stage; z C2R last) and atom form (ligand spectra per
+    kz plane from the atoms on the matrix cores; then as the column form)
+    against the oracle, for ligands whose boxes fit in N/2 and ligands whose
+    boxes do not; and the same bits for every mid-stage group size."""
     g, prot = p64
-    ctx.set_option("pipeline", 1)
+    ctx.set_option("pipeline", pipeline)
     try:
         worst = 0
         for seed, na, nq in ((901, 15, 8), (902, 40, 8), (903, 90, 6)):
@@ -174,12 +176,12 @@ def test_column_form_vs_oracle(ctx, p64):
             box, _ = b.geometry(0)
             b.close()
             worst = max(worst, int(box.max()))
-            say("column form ligand", na, "atoms, largest box", int(box.max()))
+            say("pipeline", pipeline, "ligand", na, "atoms, largest box", int(box.max()))
             E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats)
-            assert ctx.get_option("last_pipeline") == 1
+            assert ctx.get_option("last_pipeline") == pipeline
             prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
             stats = check_search(prob, L["conformers"], quats, E, idx, mc)
-            say("column form", na, stats)
+            say("pipeline", pipeline, na, stats)
             assert stats.get("exact", 0) >= nq - 1
             for grp in (1, 3, 8):
                 ctx.set_option("mid_group", grp)
@@ -205,7 +207,7 @@ def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
     vi = ctx.get_option("last_inv_variant")
     assert v in (0, 1) and vi in (0, 1) and ctx.get_option("last_mid_group") in (1, 2, 4, 8, 16, 32)
     pipe = ctx.get_option("last_pipeline")
-    assert pipe in (0, 1)
+    assert pipe in (0, 1, 2)
     # a different batch shape keeps the forward form of this grid size
     E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"][:1], L["quats"][:1, :7])
     assert ctx.get_option("last_fwd_variant") == v and ctx.get_option("last_inv_variant") == vi
