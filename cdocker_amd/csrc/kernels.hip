@@ -1751,16 +1751,17 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
 // MFMA 32x32x2 f32 lane layout (checked on the device): lane l supplies
 // A[l % 32][l / 32] and B[l / 32][l % 32]; result register r of lane l is
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
-// P3[c][kz][block b = 2 (kx / 32) + ky / 32][r / 2][l][2]: the protein
+// P3[c][kz][block b = (N/32) (kx / 32) + ky / 32][r / 2][l][2]: the protein
 // spectra in that result layout (one 16-byte load gives a lane registers
 // 2q and 2q + 1; a wavefront load reads 1 KB).
 // ---------------------------------------------------------------------------
 template <int N>
 __device__ __forceinline__ int mfma_index(int kx, int ky, int kz) {
+  constexpr int NB = N / 32;  // 32 x 32 blocks per row of the plane
   const int row = kx & 31, col = ky & 31;
   const int l = col + 32 * ((row >> 2) & 1), r = 4 * (row >> 3) + (row & 3);
-  const int b = 2 * (kx >> 5) + (ky >> 5);
-  return (((kz * 4 + b) * 8 + (r >> 1)) * 64 + l) * 2 + (r & 1);
+  const int b = NB * (kx >> 5) + (ky >> 5);
+  return (((kz * NB * NB + b) * 8 + (r >> 1)) * 64 + l) * 2 + (r & 1);
 }
 
 template <class P>
@@ -1964,6 +1965,217 @@ __global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict
   }
 }
 
+// Atom form at grid edge 96: the same matrix-core ligand spectra over nine
+// 32 x 32 blocks (one per wavefront, 576 threads), the table index reduced
+// mod 96 exactly by a multiply and shift ((t * 43691) >> 22 = t / 96 for
+// t <= 95 * 95).  The inverse uses 96-point register transforms (4 lanes x
+// 24 values per line, 16 lines per wavefront, 6 wavefronts carry the 96
+// lines) through a plane of row stride 97 in dynamic LDS: with 4 lanes per
+// line an odd stride puts every 16 consecutive lanes on distinct 8-byte bank
+// slots in both directions.  The staged atoms and the transforms' scratch
+// (three wavefront slices, transforms in two rounds) live in the plane.
+template <int N>
+__device__ __forceinline__ unsigned tw_mod96_bytes(unsigned k, unsigned i) {
+  const unsigned t = __umul24(k, i);
+  const unsigned q = __umul24(t, 43691u) >> 22;
+  return (t - 96u * q) * 8u;
+}
+
+template <class P>
+__global__ __launch_bounds__(576) void k_mid_atoms96(const PoseSlot* __restrict__ slots,
+                                                     const LigandDesc* __restrict__ ligs,
+                                                     const int* __restrict__ chan_atoms,
+                                                     const float* __restrict__ wv, const float* __restrict__ wq,
+                                                     const AtomGrid* __restrict__ ag, int astride,
+                                                     const RotMeta* __restrict__ meta, int nslots, int group,
+                                                     const float2* __restrict__ P3, float2* __restrict__ T2,
+                                                     float invG, const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  static_assert(N == 96, "the 96-point atom form");
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
+  constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks = wavefronts
+  constexpr int S = N + 1;                  // plane row stride
+  constexpr int NLW = N / LPW;              // wavefronts carrying transform lines
+  constexpr int RW = 3;                     // wavefronts transforming at a time
+  constexpr int KZX = (ZH + 7) / 8;
+  constexpr int ENT = 512;
+  constexpr int AU = CDK_ATOM_GROUP;
+  static_assert(RW * RS::SLICE <= N * S && ENT * 32 <= N * S * 8, "scratch fits the plane");
+  extern __shared__ float2 pl[];  // [N * S]
+  __shared__ float2 stw[N];
+  __shared__ float2 stwr[L * V];
+  float4* ef = reinterpret_cast<float4*>(pl);
+  int4* ei = reinterpret_cast<int4*>(reinterpret_cast<unsigned char*>(pl) + ENT * sizeof(float4));
+  const int w = blockIdx.x;
+  const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
+  const int sg = (w >> 3) / KZX;
+  if (kz >= ZH) return;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
+  for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int part = lane >> 5;
+  const int kxA = 32 * (wave / NB) + (lane & 31);
+  const int kyB = 32 * (wave % NB) + (lane & 31);
+  const float2* twr = stwr + (lane / LPW) * V;
+  const char* stwb = reinterpret_cast<const char*>(stw);
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  __syncthreads();
+  const float2 dx = make_float2(1.f - stw[kxA].x, -stw[kxA].y), dy = make_float2(1.f - stw[kyB].x, -stw[kyB].y);
+  const int s0 = sg * group, s1 = min(nslots, s0 + group);
+  for (int s = s0; s < s1; ++s) {
+    if (!uni(meta[s].valid)) continue;
+    const int lig = uni(slots[s].lig);
+    const LigandDesc& Ld = ligs[lig];
+    const int nch = uni(Ld.nch);
+    const int wx = uni(meta[s].win[0]), wy = uni(meta[s].win[1]);
+    const int list_off = uni(Ld.list_off), w_off = uni(Ld.w_off);
+    const int ne = uni(Ld.ch_start[nch]), e_elec = uni(Ld.ch_start[nch - 1]);
+    float2 E[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
+    int st0 = 0, st1 = 0;
+    auto stage = [&](int e_lo, int e_hi) {
+      __syncthreads();  // the previous pose's plane and staged atoms are done with
+      for (int k = tid; k < e_hi - e_lo; k += blockDim.x) {
+        const int e = e_lo + k;
+        const int a = chan_atoms[list_off + e];
+        const AtomGrid g = ag[(size_t)s * astride + a];
+        // elec channel: the charge; vdW channels: sqrt(eps) (FillLigandGrid.cpp:45, :97)
+        const float wa = e >= e_elec ? wq[w_off + a] : wv[w_off + a];
+        const float2 z0 = stw[(kz * g.iz) % N], z1 = stw[(kz * (g.iz + 1)) % N];
+        const float2 zf = c_fma_s(z1, (float)g.fz, c_scale(z0, (float)(1.0 - g.fz)));
+        const float2 wz = make_float2(wa * zf.x, wa * zf.y);
+        const float fx = (float)g.fx;
+        ef[k] = make_float4(wz.x, wz.y, fx * wz.x, fx * wz.y);
+        ei[k] = make_int4(g.ix, g.iy, __float_as_int((float)g.fy), 0);
+      }
+      __syncthreads();
+      st0 = e_lo;
+      st1 = e_hi;
+    };
+    for (int c = 0; c < nch; ++c) {
+      const int e0 = uni(Ld.ch_start[c]), e1 = uni(Ld.ch_start[c + 1]);
+      if (e1 > st1) stage(e0, ne <= ENT ? ne : e1);
+      mfma_acc16 cr, ci;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        cr[r] = 0.f;
+        ci[r] = 0.f;
+      }
+      // operands as k_mid_atoms: A = conj(w Z X), B = conj(Y), X_a(k) =
+      // W^(k ix) (1 - fx D), one component per lane slot `part`
+      auto atom = [&](int k, float& av, float& br, float& bi) {
+        const float4 f = ef[k];
+        const int4 ii = ei[k];
+        const float2 tx = *reinterpret_cast<const float2*>(stwb + tw_mod96_bytes<N>(kxA, ii.x));
+        const float2 ty = *reinterpret_cast<const float2*>(stwb + tw_mod96_bytes<N>(kyB, ii.y));
+        const float fy = __int_as_float(ii.z);
+        const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));
+        const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
+        const float p1 = part ? -gy : gx, p2 = part ? -gx : -gy;
+        av = fmaf(tx.x, p1, tx.y * p2);
+        const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;
+        const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
+        br = part ? yy : yx;
+        bi = part ? yx : -yy;
+      };
+      int k = e0 - st0;
+      const int kend = e1 - st0;
+      for (; k + AU <= kend; k += AU) {
+        float av[AU], br[AU], bi[AU];
+#pragma unroll
+        for (int u = 0; u < AU; ++u) atom(k + u, av[u], br[u], bi[u]);
+#pragma unroll
+        for (int u = 0; u < AU; ++u) {
+          cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], br[u], cr, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bi[u], ci, 0, 0, 0);
+        }
+      }
+      for (; k < kend; ++k) {
+        float av, br, bi;
+        atom(k, av, br, bi);
+        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av, br, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bi, ci, 0, 0, 0);
+      }
+      const int pc = uni(Ld.pmap[c]);
+      const float2* pb = P3 + (((size_t)pc * ZH + kz) * NW + wave) * (8 * 128) + lane * 2;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float4 pq = *reinterpret_cast<const float4*>(pb + q * 128);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = 2 * q + h;
+          const float px = h ? pq.z : pq.x, py = h ? pq.w : pq.y;
+          E[r].x = fmaf(cr[r], px, E[r].x);
+          E[r].x = fmaf(-ci[r], py, E[r].x);
+          E[r].y = fmaf(cr[r], py, E[r].y);
+          E[r].y = fmaf(ci[r], px, E[r].y);
+        }
+      }
+    }
+    // the sum through the plane [kx][ky] into ky lines (value j at kx = n2 + 4j)
+    __syncthreads();  // the staged atoms' reads are done
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int kx = 32 * (wave / NB) + 8 * (r >> 2) + 4 * part + (r & 3);
+      pl[kx * S + kyB] = E[r];
+    }
+    __syncthreads();
+    int ln = lane % LPW, n2 = lane / LPW;
+    asm volatile("" : "+v"(ln), "+v"(n2));
+    const bool lines = wave < NLW;
+    float2 v[V];
+    if (lines) {
+      const int ky = wave * LPW + ln;
+#pragma unroll
+      for (int j = 0; j < V; ++j) v[j] = pl[(n2 + L * j) * S + ky];
+    }
+    __syncthreads();  // the plane is free: the transforms' scratch
+#pragma unroll
+    for (int rr = 0; rr < NLW / RW; ++rr) {
+      if (lines && wave / RW == rr) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[i] = c_scale(v[i], invG);
+        reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
+      }
+      __syncthreads();
+    }
+    // x' rows back into the plane [x'][ky], then lines along x' (value j at
+    // y' = n2 + 4j)
+    if (lines) {
+      const int ky = wave * LPW + ln;
+#pragma unroll
+      for (int m = 0; m < V; ++m) pl[(n2 + L * m) * S + ky] = v[m];
+    }
+    __syncthreads();
+    const int xp = wave * LPW + ln;
+    const bool xl = lines && wave * LPW < wx;
+    if (xl) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) v[j] = pl[xp * S + n2 + L * j];
+    }
+    __syncthreads();  // the plane is free: the transforms' scratch
+    float2* tb = T2 + ((size_t)s * ZH + kz) * (size_t)(N * N);
+#pragma unroll
+    for (int rr = 0; rr < NLW / RW; ++rr) {
+      if (xl && wave / RW == rr) {
+        reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
+        if (xp < wx) {
+#pragma unroll
+          for (int q = 0; q < V; ++q)
+            if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+size_t mid_atoms96_lds_bytes() { return (size_t)96 * 97 * sizeof(float2); }
+
 // Column form, last stage: one workgroup per (pose, block of 64/L * 4 row
 // pairs).  Row pair (x', y0 = L*m + 2h, y0 + 1) is one complex line A + iB
 // of the z C2R (two-for-one, imaginary parts of the kz = 0 and N/2 bins
@@ -1976,7 +2188,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
   constexpr int N = P::N, ZH = P::ZH;
   using RS = RegShape<N>;
   constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
-  static_assert(V == 8, "column form: 8 values per lane");
+  static_assert(N == 64 || N == 96, "column and atom forms: grid edges 64 and 96");
   constexpr int H = L / 2;        // row pairs per L rows
   constexpr int LPB = 4 * LPW;    // row pairs per workgroup
   __shared__ float2 sbuf[4][RS::SLICE];
@@ -2230,6 +2442,11 @@ hipError_t configure_fwd_reg(size_t max_dyn_lds) {
           e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_z<PL>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
       }
+      if constexpr (PL::N == 96) {
+        if (e == hipSuccess && mid_atoms96_lds_bytes() + 2 * 96 * sizeof(float2) <= max_dyn_lds)
+          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mid_atoms96<PL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)mid_atoms96_lds_bytes());
+      }
     });
     if (e != hipSuccess) return e;
   }
@@ -2293,9 +2510,9 @@ hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const Lig
 hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, const float2* T2,
                         unsigned long long* keys, const float2* tw) {
   if (nslots <= 0) return hipSuccess;
-  if (!col_supported(N)) return hipErrorInvalidValue;
+  if (!col_supported(N) && !atoms_supported(N)) return hipErrorInvalidValue;
   CDK_DISPATCH_N(N, {
-    if constexpr (PL::N == 64) {
+    if constexpr (PL::N == 64 || PL::N == 96) {
       using RS = RegShape<PL::N>;
       // windows are at most N - 2 wide (boxes are at least 3 cells)
       const int w = PL::N - 2;
@@ -2307,7 +2524,9 @@ hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, 
   return hipGetLastError();
 }
 
-bool atoms_supported(int N) { return N == 64; }
+bool atoms_supported(int N) {
+  return N == 64 || (N == 96 && mid_atoms96_lds_bytes() + 2 * 96 * sizeof(float2) <= g_max_dyn_lds);
+}
 
 hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
                             const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
@@ -2321,6 +2540,9 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
     if constexpr (PL::N == 64)
       hipLaunchKernelGGL(k_mid_atoms<PL>, grid, dim3(kBlock), 0, st, slots, ligs, chan_atoms, wv, wq, ag, astride,
                          meta, nslots, group, P3, T2, invG, tw);
+    if constexpr (PL::N == 96)
+      hipLaunchKernelGGL(k_mid_atoms96<PL>, grid, dim3(576), mid_atoms96_lds_bytes(), st, slots, ligs, chan_atoms, wv,
+                         wq, ag, astride, meta, nslots, group, P3, T2, invG, tw);
   });
   return hipGetLastError();
 }
@@ -2330,7 +2552,7 @@ hipError_t launch_to_mfma_blocks(hipStream_t st, int N, const float2* in, float2
   if (n == 0) return hipSuccess;
   if (!atoms_supported(N)) return hipErrorInvalidValue;
   CDK_DISPATCH_N(N, {
-    if constexpr (PL::N == 64)
+    if constexpr (PL::N == 64 || PL::N == 96)
       hipLaunchKernelGGL(k_to_mfma_blocks<PL>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, in, out, nch);
   });
   return hipGetLastError();

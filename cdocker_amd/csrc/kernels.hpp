@@ -69,7 +69,7 @@ hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const Lig
 hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, const float2* T2,
                         unsigned long long* keys, const float2* tw);
 hipError_t launch_to_kz_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
-// Atom form (grid edge 64): the mid stage builds each channel's ligand
+// Atom form (grid edges 64 and 96): the mid stage builds each channel's ligand
 // spectrum on a kz plane as a sum of per-atom outer products on the matrix
 // cores (no ligand grids, no forward stage); then as the column form, with
 // k_inv_z last.  P3 is the protein spectra in the matrix-core result layout

@@ -221,6 +221,37 @@ def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
     assert np.array_equal(E2.view(np.uint32), E0.view(np.uint32)) and np.array_equal(i2, i0)
 
 
+def test_atom_form_96_vs_oracle(ctx):
+    """Atom form at 96^3 (nine 32 x 32 matrix-core blocks per kz plane, the
+    twiddle index reduced mod 96, 96-point inverse transforms) against the
+    oracle, and the same bits for every mid-stage group size."""
+    g = workloads.protein_grids("p96")
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    ctx.set_option("pipeline", 2)
+    try:
+        for seed, na, nq in ((911, 18, 5), (912, 45, 4)):
+            L = workloads.ligand(seed, na, 1)
+            quats = workloads.quaternion_sets(seed + 1, 1, nq)
+            lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+            say("atom form 96 ligand", na)
+            E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats)
+            assert ctx.get_option("last_pipeline") == 2
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+            stats = check_search(prob, L["conformers"], quats, E, idx, mc)
+            say("atom form 96", na, stats)
+            assert stats.get("exact", 0) >= nq - 1
+            for grp in (1, 3):
+                ctx.set_option("mid_group", grp)
+                E1, i1, _ = cdk.search(ctx, prot, lig, L["conformers"], quats)
+                assert np.array_equal(E1.view(np.uint32), E.view(np.uint32)) and np.array_equal(i1, idx), grp
+            ctx.set_option("mid_group", 0)
+            lig.close()
+    finally:
+        ctx.set_option("mid_group", 0)
+        ctx.set_option("pipeline", -1)
+        prot.close()
+
+
 @pytest.mark.parametrize("name", ["p96", "p128"])
 def test_large_grids_vs_oracle(ctx, name):
     """96^3 and 128^3 grids, both engines, a few poses each."""
