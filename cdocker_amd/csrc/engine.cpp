@@ -498,6 +498,10 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   const bool col_ok = col_supported(N) && prot.spec_kz.p != nullptr;
   const bool atoms_ok = atoms_supported(N) && prot.spec_mf.p != nullptr;
   const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)chunk};
+  // The column and atom forms' group choice is kept per (N, channels): their
+  // best group hardly moves with the box bound or the launch size, and a
+  // server sees a new (box bound, chunk) pair on many launch sets.
+  const std::array<int, 4> ckey = {N, t.cmax, 0, 0};
   bool need_pipe = false, need_fwd = false, need_inv = false, need_group = false, need_col = false,
        need_atom = false;
   if (ctx.pipeline >= 0) {
@@ -558,14 +562,14 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
       }
     }
     if (cols) {
-      auto it = ctx.tuned_col_group.find(gkey);
+      auto it = ctx.tuned_col_group.find(ckey);
       if (it != ctx.tuned_col_group.end())
         tu.col_group = it->second;
       else
         need_col = true;
     }
     if (atoms) {
-      auto it = ctx.tuned_atom_group.find(gkey);
+      auto it = ctx.tuned_atom_group.find(ckey);
       if (it != ctx.tuned_atom_group.end())
         tu.atom_group = it->second;
       else
@@ -698,7 +702,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
             tu.col_group = g;
           }
         }
-        ctx.tuned_col_group[gkey] = tu.col_group;
+        ctx.tuned_col_group[ckey] = tu.col_group;
         col_ms += best;
       } else {
         col_ms += midc(tu.col_group);
@@ -725,7 +729,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
             tu.atom_group = g;
           }
         }
-        ctx.tuned_atom_group[gkey] = tu.atom_group;
+        ctx.tuned_atom_group[ckey] = tu.atom_group;
         atom_ms += best;
       } else {
         atom_ms += mida(tu.atom_group);

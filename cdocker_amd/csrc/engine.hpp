@@ -177,8 +177,8 @@ struct Context {
   std::map<int, int> tuned_inv;                      // N -> variant
   // (N, cmax, boxmax, chunk) -> (group, depth, poses per protein load)
   std::map<std::array<int, 4>, std::array<int, 3>> tuned_group;
-  std::map<std::array<int, 4>, int> tuned_col_group;   // (N, cmax, boxmax, chunk) -> column form group
-  std::map<std::array<int, 4>, int> tuned_atom_group;  // (N, cmax, boxmax, chunk) -> atom form group
+  std::map<std::array<int, 4>, int> tuned_col_group;   // (N, cmax, 0, 0) -> column form group
+  std::map<std::array<int, 4>, int> tuned_atom_group;  // (N, cmax, 0, 0) -> atom form group
   int last_fwd = -1, last_inv = -1, last_group = 0, last_depth = 0, last_poses = 0,
       last_pipe = -1;
   StageTimes stage_times;
