@@ -221,6 +221,38 @@ def test_autotuned_choice_is_sticky_and_reproducible(ctx, p64):
     assert np.array_equal(E2.view(np.uint32), E0.view(np.uint32)) and np.array_equal(i2, i0)
 
 
+def test_atom_form_large_ligand(ctx, p64):
+    """A 300-atom ligand (600 channel atoms: more than the atom form stages at
+    once, so it stages one channel at a time) against the oracle and the
+    plane form, within the parity tolerance."""
+    g, prot = p64
+    rs = np.random.default_rng(931)
+    L = workloads.ligand(931, 300, 1)
+    # compact coordinates: a ball of radius 6 A keeps the box well inside 64^3
+    d = rs.normal(size=(300, 3))
+    x = 6.0 * d / np.linalg.norm(d, axis=1, keepdims=True) * rs.uniform(0, 1, (300, 1)) ** (1 / 3)
+    confs = x[None].astype(np.float32)
+    quats = workloads.quaternion_sets(932, 1, 3)
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    try:
+        res = {}
+        for pipeline in (2, 0):
+            ctx.set_option("pipeline", pipeline)
+            E, idx, mc = cdk.search(ctx, prot, lig, confs, quats)
+            assert ctx.get_option("last_pipeline") == pipeline
+            stats = check_search(prob, confs, quats, E, idx, mc)
+            say("300-atom ligand, pipeline", pipeline, stats)
+            assert stats.get("exact", 0) >= 2
+            res[pipeline] = (E, idx)
+        # both forms within the energy tolerance of each other
+        Ea, Ep = res[2][0], res[0][0]
+        assert np.all(np.abs(Ea - Ep) <= 1e-4 * np.maximum(np.abs(Ep), 1.0))
+    finally:
+        ctx.set_option("pipeline", -1)
+        lig.close()
+
+
 def test_atom_form_96_vs_oracle(ctx):
     """Atom form at 96^3 (nine 32 x 32 matrix-core blocks per kz plane, the
     twiddle index reduced mod 96, 96-point inverse transforms) against the
