@@ -458,7 +458,8 @@ void enqueue_reference(Context& ctx, const Protein& prot, Prepared& t, EventTime
 size_t fused_chunk(Context& ctx, const Protein& prot, const Prepared& t) {
   const int N = prot.N, ZH = N / 2 + 1;
   const size_t plane = (size_t)N * ZH;
-  const size_t per_slot = (size_t)t.cmax * t.boxmax * plane * sizeof(float2) + (size_t)N * plane * sizeof(float2) +
+  const size_t per_slot = std::max((size_t)t.cmax * t.boxmax * plane * sizeof(float2), atoms_work_bytes(N)) +
+                          (size_t)N * plane * sizeof(float2) +
                           (size_t)t.astride * (3 * sizeof(float) + sizeof(AtomGrid));
   size_t chunk = ctx.fused_chunk > 0 ? (size_t)ctx.fused_chunk : std::max<size_t>(1, ctx.ws_budget / per_slot);
   return std::min(chunk, t.slots.size());
@@ -469,7 +470,9 @@ void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, siz
   const size_t plane = (size_t)N * ZH;
   ensure_work(ctx, chunk * t.astride * 3 * sizeof(float), ctx.d_rot);
   ensure_work(ctx, chunk * t.astride * sizeof(AtomGrid), ctx.d_ag);
-  ensure_work(ctx, chunk * t.cmax * t.boxmax * plane * sizeof(float2), ctx.d_A);
+  // the plane and column forms' ligand spectra, or the atom form's product
+  // spectra at grid edge 96
+  ensure_work(ctx, chunk * std::max(t.cmax * t.boxmax * plane * sizeof(float2), atoms_work_bytes(N)), ctx.d_A);
   ensure_work(ctx, chunk * N * plane * sizeof(float2), ctx.d_T);
 }
 
@@ -716,7 +719,8 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
         return timed([&] {
           hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
                                      t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, n, g,
-                                     prot.spec_mf.as<float2>(), ctx.d_T.as<float2>(), invG, tw),
+                                     prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), ctx.d_T.as<float2>(), invG,
+                                     tw),
                     "tune atom mid");
         });
       };
@@ -787,7 +791,8 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
       e = timer.begin(2);
       hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
                                  t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, n, tu.atom_group,
-                                 prot.spec_mf.as<float2>(), ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+                                 prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), ctx.d_T.as<float2>(),
+                                 1.0f / (float)G, tw),
                 "atom mid");
       timer.end(e);
       e = timer.begin(3);

@@ -1965,15 +1965,22 @@ __global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict
   }
 }
 
-// Atom form at grid edge 96: the same matrix-core ligand spectra over nine
-// 32 x 32 blocks (one per wavefront, 576 threads), the table index reduced
-// mod 96 exactly by a multiply and shift ((t * 43691) >> 22 = t / 96 for
-// t <= 95 * 95).  The inverse uses 96-point register transforms (4 lanes x
-// 24 values per line, 16 lines per wavefront, 6 wavefronts carry the 96
-// lines) through a plane of row stride 97 in dynamic LDS: with 4 lanes per
-// line an odd stride puts every 16 consecutive lanes on distinct 8-byte bank
-// slots in both directions.  The staged atoms and the transforms' scratch
-// (three wavefront slices, transforms in two rounds) live in the plane.
+// Atom form at grid edge 96, in two kernels so that each runs at its own
+// occupancy (the 96-point inverse transforms hold 24 values per lane).
+//
+// k_atoms96_fwd: one workgroup per (kz, row of 32 x 32 kx blocks, group of
+// poses), three wavefronts, one block each: the same matrix-core ligand
+// spectra and product as k_mid_atoms, the table index reduced mod 96 exactly
+// by a multiply and shift ((t * 43691) >> 22 = t / 96 for t <= 95 * 95).  The
+// pose's product spectrum goes to E[slot][kz][block][r/2][lane][2] (the
+// matrix-core result layout: 1 KB per wavefront store).
+//
+// k_atoms96_inv: one workgroup per (pose, kz), six wavefronts: the nine
+// blocks into a plane of row stride 97 in dynamic LDS (with 4 lanes per line
+// an odd stride puts every 16 consecutive lanes on distinct 8-byte bank
+// slots in both directions), 96-point register inverse transforms over x
+// (ky lines) and y (x' lines; 16 lines per wavefront) three wavefronts at a
+// time with their scratch in the plane, window rows into T2.
 template <int N>
 __device__ __forceinline__ unsigned tw_mod96_bytes(unsigned k, unsigned i) {
   const unsigned t = __umul24(k, i);
@@ -1982,44 +1989,36 @@ __device__ __forceinline__ unsigned tw_mod96_bytes(unsigned k, unsigned i) {
 }
 
 template <class P>
-__global__ __launch_bounds__(576) void k_mid_atoms96(const PoseSlot* __restrict__ slots,
+__global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict__ slots,
                                                      const LigandDesc* __restrict__ ligs,
                                                      const int* __restrict__ chan_atoms,
                                                      const float* __restrict__ wv, const float* __restrict__ wq,
                                                      const AtomGrid* __restrict__ ag, int astride,
                                                      const RotMeta* __restrict__ meta, int nslots, int group,
-                                                     const float2* __restrict__ P3, float2* __restrict__ T2,
-                                                     float invG, const float2* __restrict__ twg) {
+                                                     const float2* __restrict__ P3, float2* __restrict__ Eout,
+                                                     const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
   static_assert(N == 96, "the 96-point atom form");
-  using RS = RegShape<N>;
-  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
-  constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks = wavefronts
-  constexpr int S = N + 1;                  // plane row stride
-  constexpr int NLW = N / LPW;              // wavefronts carrying transform lines
-  constexpr int RW = 3;                     // wavefronts transforming at a time
+  constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks per plane
   constexpr int KZX = (ZH + 7) / 8;
   constexpr int ENT = 512;
   constexpr int AU = CDK_ATOM_GROUP;
-  static_assert(RW * RS::SLICE <= N * S && ENT * 32 <= N * S * 8, "scratch fits the plane");
-  extern __shared__ float2 pl[];  // [N * S]
   __shared__ float2 stw[N];
-  __shared__ float2 stwr[L * V];
-  float4* ef = reinterpret_cast<float4*>(pl);
-  int4* ei = reinterpret_cast<int4*>(reinterpret_cast<unsigned char*>(pl) + ENT * sizeof(float4));
+  __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
+  __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0)
   const int w = blockIdx.x;
   const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
-  const int sg = (w >> 3) / KZX;
+  const int rest = (w >> 3) / KZX;
+  const int brow = rest % NB, sg = rest / NB;
   if (kz >= ZH) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
-  for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const int part = lane >> 5;
-  const int kxA = 32 * (wave / NB) + (lane & 31);
-  const int kyB = 32 * (wave % NB) + (lane & 31);
-  const float2* twr = stwr + (lane / LPW) * V;
+  const int blk = NB * brow + wave;
+  const int kxA = 32 * brow + (lane & 31);
+  const int kyB = 32 * wave + (lane & 31);
   const char* stwb = reinterpret_cast<const char*>(stw);
   auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
   __syncthreads();
@@ -2030,7 +2029,6 @@ __global__ __launch_bounds__(576) void k_mid_atoms96(const PoseSlot* __restrict_
     const int lig = uni(slots[s].lig);
     const LigandDesc& Ld = ligs[lig];
     const int nch = uni(Ld.nch);
-    const int wx = uni(meta[s].win[0]), wy = uni(meta[s].win[1]);
     const int list_off = uni(Ld.list_off), w_off = uni(Ld.w_off);
     const int ne = uni(Ld.ch_start[nch]), e_elec = uni(Ld.ch_start[nch - 1]);
     float2 E[16];
@@ -2038,7 +2036,7 @@ __global__ __launch_bounds__(576) void k_mid_atoms96(const PoseSlot* __restrict_
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
     int st0 = 0, st1 = 0;
     auto stage = [&](int e_lo, int e_hi) {
-      __syncthreads();  // the previous pose's plane and staged atoms are done with
+      __syncthreads();  // the previous staged atoms are done with
       for (int k = tid; k < e_hi - e_lo; k += blockDim.x) {
         const int e = e_lo + k;
         const int a = chan_atoms[list_off + e];
@@ -2101,7 +2099,7 @@ __global__ __launch_bounds__(576) void k_mid_atoms96(const PoseSlot* __restrict_
         ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bi, ci, 0, 0, 0);
       }
       const int pc = uni(Ld.pmap[c]);
-      const float2* pb = P3 + (((size_t)pc * ZH + kz) * NW + wave) * (8 * 128) + lane * 2;
+      const float2* pb = P3 + (((size_t)pc * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const float4 pq = *reinterpret_cast<const float4*>(pb + q * 128);
@@ -2116,61 +2114,98 @@ __global__ __launch_bounds__(576) void k_mid_atoms96(const PoseSlot* __restrict_
         }
       }
     }
-    // the sum through the plane [kx][ky] into ky lines (value j at kx = n2 + 4j)
-    __syncthreads();  // the staged atoms' reads are done
+    float2* eb = Eout + (((size_t)s * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int kx = 32 * (wave / NB) + 8 * (r >> 2) + 4 * part + (r & 3);
-      pl[kx * S + kyB] = E[r];
+    for (int q = 0; q < 8; ++q)
+      *reinterpret_cast<float4*>(eb + q * 128) = make_float4(E[2 * q].x, E[2 * q].y, E[2 * q + 1].x, E[2 * q + 1].y);
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(384) void k_atoms96_inv(const RotMeta* __restrict__ meta,
+                                                     const float2* __restrict__ Ein, float2* __restrict__ T2,
+                                                     float invG, const float2* __restrict__ twg) {
+  constexpr int N = P::N, ZH = P::ZH;
+  static_assert(N == 96, "the 96-point atom form");
+  using RS = RegShape<N>;
+  constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
+  constexpr int NB = N / 32, NW = NB * NB;
+  constexpr int S = N + 1;      // plane row stride
+  constexpr int NLW = N / LPW;  // wavefronts carrying transform lines (all six)
+  constexpr int RW = 3;         // wavefronts transforming at a time
+  static_assert(RW * RS::SLICE <= N * S, "scratch fits the plane");
+  extern __shared__ float2 pl[];  // [N * S]
+  __shared__ float2 stwr[L * V];
+  const int kz = blockIdx.x, s = blockIdx.y;
+  if (!meta[s].valid) return;
+  const int wx = meta[s].win[0], wy = meta[s].win[1];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int part = lane >> 5;
+  // the nine blocks into the plane [kx][ky]
+  for (int b = wave; b < NW; b += NLW) {
+    const float2* eb = Ein + (((size_t)s * ZH + kz) * NW + b) * (8 * 128) + lane * 2;
+    const int kx0 = 32 * (b / NB), ky = 32 * (b % NB) + (lane & 31);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float4 f = *reinterpret_cast<const float4*>(eb + q * 128);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = 2 * q + h;
+        const int kx = kx0 + 8 * (r >> 2) + 4 * part + (r & 3);
+        pl[kx * S + ky] = h ? make_float2(f.z, f.w) : make_float2(f.x, f.y);
+      }
+    }
+  }
+  __syncthreads();
+  const float2* twr = stwr + (lane / LPW) * V;
+  int ln = lane % LPW, n2 = lane / LPW;
+  asm volatile("" : "+v"(ln), "+v"(n2));
+  // ky lines: value j at kx = n2 + 4j
+  float2 v[V];
+  {
+    const int ky = wave * LPW + ln;
+#pragma unroll
+    for (int j = 0; j < V; ++j) v[j] = pl[(n2 + L * j) * S + ky];
+  }
+  __syncthreads();  // the plane is free: the transforms' scratch
+#pragma unroll
+  for (int rr = 0; rr < NLW / RW; ++rr) {
+    if (wave / RW == rr) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = c_scale(v[i], invG);
+      reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
     }
     __syncthreads();
-    int ln = lane % LPW, n2 = lane / LPW;
-    asm volatile("" : "+v"(ln), "+v"(n2));
-    const bool lines = wave < NLW;
-    float2 v[V];
-    if (lines) {
-      const int ky = wave * LPW + ln;
+  }
+  // x' rows back into the plane [x'][ky], then x' lines (value j at y' = n2 + 4j)
+  {
+    const int ky = wave * LPW + ln;
 #pragma unroll
-      for (int j = 0; j < V; ++j) v[j] = pl[(n2 + L * j) * S + ky];
-    }
-    __syncthreads();  // the plane is free: the transforms' scratch
+    for (int m = 0; m < V; ++m) pl[(n2 + L * m) * S + ky] = v[m];
+  }
+  __syncthreads();
+  const int xp = wave * LPW + ln;
+  const bool xl = wave * LPW < wx;
+  if (xl) {
 #pragma unroll
-    for (int rr = 0; rr < NLW / RW; ++rr) {
-      if (lines && wave / RW == rr) {
+    for (int j = 0; j < V; ++j) v[j] = pl[xp * S + n2 + L * j];
+  }
+  __syncthreads();  // the plane is free: the transforms' scratch
+  float2* tb = T2 + ((size_t)s * ZH + kz) * (size_t)(N * N);
 #pragma unroll
-        for (int i = 0; i < V; ++i) v[i] = c_scale(v[i], invG);
-        reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
+  for (int rr = 0; rr < NLW / RW; ++rr) {
+    if (xl && wave / RW == rr) {
+      reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
+      if (xp < wx) {
+#pragma unroll
+        for (int q = 0; q < V; ++q)
+          if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
       }
-      __syncthreads();
-    }
-    // x' rows back into the plane [x'][ky], then lines along x' (value j at
-    // y' = n2 + 4j)
-    if (lines) {
-      const int ky = wave * LPW + ln;
-#pragma unroll
-      for (int m = 0; m < V; ++m) pl[(n2 + L * m) * S + ky] = v[m];
     }
     __syncthreads();
-    const int xp = wave * LPW + ln;
-    const bool xl = lines && wave * LPW < wx;
-    if (xl) {
-#pragma unroll
-      for (int j = 0; j < V; ++j) v[j] = pl[xp * S + n2 + L * j];
-    }
-    __syncthreads();  // the plane is free: the transforms' scratch
-    float2* tb = T2 + ((size_t)s * ZH + kz) * (size_t)(N * N);
-#pragma unroll
-    for (int rr = 0; rr < NLW / RW; ++rr) {
-      if (xl && wave / RW == rr) {
-        reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
-        if (xp < wx) {
-#pragma unroll
-          for (int q = 0; q < V; ++q)
-            if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
-        }
-      }
-      __syncthreads();
-    }
   }
 }
 
@@ -2444,7 +2479,7 @@ hipError_t configure_fwd_reg(size_t max_dyn_lds) {
       }
       if constexpr (PL::N == 96) {
         if (e == hipSuccess && mid_atoms96_lds_bytes() + 2 * 96 * sizeof(float2) <= max_dyn_lds)
-          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mid_atoms96<PL>),
+          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_atoms96_inv<PL>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)mid_atoms96_lds_bytes());
       }
     });
@@ -2524,14 +2559,16 @@ hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, 
   return hipGetLastError();
 }
 
+size_t atoms_work_bytes(int N) { return N == 96 ? (size_t)N * N * (N / 2 + 1) * sizeof(float2) : 0; }
+
 bool atoms_supported(int N) {
   return N == 64 || (N == 96 && mid_atoms96_lds_bytes() + 2 * 96 * sizeof(float2) <= g_max_dyn_lds);
 }
 
 hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
                             const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
-                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* T2, float invG,
-                            const float2* tw) {
+                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* work, float2* T2,
+                            float invG, const float2* tw) {
   if (nslots <= 0) return hipSuccess;
   if (!atoms_supported(N) || group <= 0) return hipErrorInvalidValue;
   const int kzx = (N / 2 + 1 + 7) / 8;
@@ -2540,9 +2577,13 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
     if constexpr (PL::N == 64)
       hipLaunchKernelGGL(k_mid_atoms<PL>, grid, dim3(kBlock), 0, st, slots, ligs, chan_atoms, wv, wq, ag, astride,
                          meta, nslots, group, P3, T2, invG, tw);
-    if constexpr (PL::N == 96)
-      hipLaunchKernelGGL(k_mid_atoms96<PL>, grid, dim3(576), mid_atoms96_lds_bytes(), st, slots, ligs, chan_atoms, wv,
-                         wq, ag, astride, meta, nslots, group, P3, T2, invG, tw);
+    if constexpr (PL::N == 96) {
+      const dim3 gf((unsigned)(8 * kzx * (N / 32)) * ceil_div(nslots, group));
+      hipLaunchKernelGGL(k_atoms96_fwd<PL>, gf, dim3(192), 0, st, slots, ligs, chan_atoms, wv, wq, ag, astride, meta,
+                         nslots, group, P3, work, tw);
+      hipLaunchKernelGGL(k_atoms96_inv<PL>, dim3(PL::ZH, nslots), dim3(384), mid_atoms96_lds_bytes(), st, meta, work,
+                         T2, invG, tw);
+    }
   });
   return hipGetLastError();
 }
