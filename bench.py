@@ -42,6 +42,7 @@ sys.path.insert(0, ROOT)
 METRIC = "poses scored/sec at 64³×9ch FP32; p50/p99 request latency in serve loop"
 PEAK_HBM_GBPS = 8000.0
 PEAK_MFMA_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix cores
+PEAK_MFMA_FP16_TFLOPS = 2500.0  # MI355X dense FP16 matrix cores
 PEAK_VALU_FP32_TFLOPS = 78.6  # MI355X FP32 vector ALUs, unpacked: 256 CUs x 64 lanes x 2 flops x 2.4 GHz
 
 
@@ -283,14 +284,21 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
             "algorithmic_bytes_per_launch": round(dom_bytes)}
     if dom == "mid_atoms":
-        # the ligand spectra are FP32 matrix-core products: the algorithmic
-        # flops of those products against the FP32 matrix-core peak
+        # The ligand spectra are matrix-core products: their algorithmic flops
+        # against the peak of the matrix cores that run them.  Split-FP16 form:
+        # FP16 units, three FP16 products per FP32-accurate product (so a
+        # third of the FP16 peak is that form's ceiling); FP32 form: FP32 units.
         tf = atom_mfma_flops(geoms, N) / kernels[dom]["launches"] / dom_avg_s / 1e12
         hbm = dict(roof)
-        roof = {"bound": "mfma", "kernel": dom, "achieved": round(tf, 2), "peak": PEAK_MFMA_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(tf / PEAK_MFMA_FP32_TFLOPS, 4), "traffic": None,
+        f16 = ctx.get_option("atom_products") == 0
+        peak = PEAK_MFMA_FP16_TFLOPS if f16 else PEAK_MFMA_FP32_TFLOPS
+        roof = {"bound": "mfma", "kernel": dom, "achieved": round(tf, 2), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(tf / peak, 4), "traffic": None,
+                "products": "split-f16 (hi/lo halves, 3 FP16 products, FP32 accumulate)" if f16 else "f32",
                 "avg_launch_ms": hbm["avg_launch_ms"], "algorithmic_bytes_per_launch": hbm["algorithmic_bytes_per_launch"],
                 "hbm": {"achieved": hbm["achieved"], "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": hbm["frac"]}}
+        if f16:
+            roof["frac_of_split_peak"] = round(3 * tf / peak, 4)
     if dom in ("mid_kz", "mid_lines"):
         # The transforms run on the vector ALUs in FP32 without packed
         # instructions: 78.6 TFLOP/s is that ceiling (half the listed
@@ -393,7 +401,8 @@ def run_single(args, cdk, workloads):
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
-                      inv_variant=args.inv, pipeline=args.pipeline)
+                      inv_variant=args.inv, pipeline=args.pipeline,
+                      atom_products=args.atom_products)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     nu, used, _ = lig.channels()
@@ -417,6 +426,7 @@ def run_single(args, cdk, workloads):
     pc = max(1, min(args.plain_conformers, args.conformers))
     ref = reference_leg(ctx, cdk, prot, lig, L["conformers"][:pc], L["quats"][:pc], idx[:pc])
     ref["speedup_fused"] = round(value / ref["value"], 2)
+    f64 = fp64_check(args, g, L, E, idx, mc, args.fp64_sample, args.fp64_full) if args.fp64_sample > 0 else None
     cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
 
     # The same protein's grids built on the device from its atoms.
@@ -446,7 +456,7 @@ def run_single(args, cdk, workloads):
          roof, kernels, cpu,
          pipeline_equiv={"bytes_per_pose": reference_bytes(N, C), "GBps": round(value * reference_bytes(N, C) / 1e9, 1),
                          "note": "poses/s x B(N,C) of the unfused reference dataflow"},
-         reference_engine=ref, serving=serving, grid_builder=grid_builder)
+         reference_engine=ref, fp64_check=f64, serving=serving, grid_builder=grid_builder)
     batch.close()
     lig.close()
     prot.close()
@@ -460,7 +470,8 @@ def run_batched(args, cdk, workloads):
     w = workloads.config3()
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
-                      inv_variant=args.inv, pipeline=args.pipeline)
+                      inv_variant=args.inv, pipeline=args.pipeline,
+                      atom_products=args.atom_products)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     N = g["n"]
     ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
@@ -505,16 +516,45 @@ def run_batched(args, cdk, workloads):
     ctx.close()
 
 
+def fp64_check(args, g, L, E, idx, mc, nsample, nfull):
+    """Sampled rotations re-scored by the FP64 oracle at the GPU's translation
+    (relative energy error), and a few searched in full in FP64 (argmin)."""
+    from oracle import oracle as orc
+    orc.set_threads(args.cpu_threads)
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    nconf, nquat = E.shape
+    rs = np.random.default_rng(55)
+    flat = rs.choice(nconf * nquat, min(nsample, nconf * nquat), replace=False)
+    max_rel = 0.0
+    checked = 0
+    for f in flat:
+        c, i = divmod(int(f), nquat)
+        if not np.isfinite(E[c, i]):
+            continue
+        rot = orc.rotate(L["quats"][c, i:i + 1], L["conformers"][c])[0]
+        e64 = prob.sparse_energy(rot, mc[c, i], idx[c, i])
+        max_rel = max(max_rel, abs(float(E[c, i]) - e64) / max(abs(e64), 1e-30))
+        checked += 1
+    full = {"poses": 0, "exact": 0}
+    for f in flat[:nfull]:
+        c, i = divmod(int(f), nquat)
+        o = prob.search_pose(L["conformers"][c], L["quats"][c, i], double=True)
+        full["poses"] += 1
+        full["exact"] += int(o["valid"] and np.array_equal(o["index"], idx[c, i]))
+    log("fp64 check: %d poses, max rel err %.3g; full searches %s" % (checked, max_rel, full))
+    return {"sampled_rotations": checked, "max_rel_energy_err": float("%.3g" % max_rel), "full_search_argmin": full}
+
+
 def run_large(args, cdk, workloads):
     """128^3 x 9 at 0.375 A; top-10 poses per conformer; 1% FP64 re-score."""
-    from oracle import oracle as orc
     t0 = time.time()
     g = workloads.protein_grids("p128")
     w = workloads.config5()
     L = w["ligands"][0]
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
-                      inv_variant=args.inv, pipeline=args.pipeline)
+                      inv_variant=args.inv, pipeline=args.pipeline,
+                      atom_products=args.atom_products)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     C = lig.channels()[0] + 1
@@ -540,27 +580,7 @@ def run_large(args, cdk, workloads):
 
     # 1% of the rotations re-scored by the FP64 oracle at the GPU's translation,
     # and a few searched in full (FP64 dense) for the argmin.
-    orc.set_threads(args.cpu_threads)
-    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
-    rs = np.random.default_rng(55)
-    flat = rs.choice(npose, max(1, npose // 100), replace=False)
-    max_rel = 0.0
-    checked = 0
-    for f in flat:
-        c, i = divmod(int(f), nquat)
-        if not np.isfinite(E[c, i]):
-            continue
-        rot = orc.rotate(L["quats"][c, i:i + 1], L["conformers"][c])[0]
-        e64 = prob.sparse_energy(rot, mc[c, i], idx[c, i])
-        max_rel = max(max_rel, abs(float(E[c, i]) - e64) / max(abs(e64), 1e-30))
-        checked += 1
-    full = {"poses": 0, "exact": 0}
-    for f in flat[:args.large_full_checks]:
-        c, i = divmod(int(f), nquat)
-        o = prob.search_pose(L["conformers"][c], L["quats"][c, i], double=True)
-        full["poses"] += 1
-        full["exact"] += int(o["valid"] and np.array_equal(o["index"], idx[c, i]))
-    log("fp64 check: %d poses, max rel err %.3g; full searches %s" % (checked, max_rel, full))
+    f64 = fp64_check(args, g, L, E, idx, mc, max(1, npose // 100), args.large_full_checks)
     cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
     line(args, "poses scored/sec at 128³×9ch FP32", value, total,
          "large grid: 128^3 x 9ch FP32 at 0.375 A, 50-atom ligand, %d conformers x %d quaternions = %d poses, "
@@ -568,8 +588,7 @@ def run_large(args, cdk, workloads):
          {"grid_edge": N, "channels_used": C, "poses": npose, "valid_poses": int(np.isfinite(E).sum()),
           "engine": "fused" + ("" if args.no_graph else "+hipGraph")},
          roof, kernels, cpu,
-         fp64_check={"sampled_rotations": checked, "max_rel_energy_err": float("%.3g" % max_rel),
-                     "full_search_argmin": full},
+         fp64_check=f64,
          top10={"conformers": nconf, "host_s": round(top_s, 3)})
     batch.close()
     lig.close()
@@ -594,13 +613,18 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
     ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
     ap.add_argument("--pipeline", type=int, default=-1,
-                    help="fused engine form: 0 planes, 1 columns (64^3), -1 autotuned")
+                    help="fused engine form: 0 planes, 1 columns (64^3), 2 atoms (64^3, 96^3), -1 autotuned")
+    ap.add_argument("--atom-products", type=int, default=0,
+                    help="atom form products: 0 split-FP16 matrix cores, 1 FP32 matrix cores")
     ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--inv", type=int, default=-1, help="inverse plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="Poisson rate as a fraction of capacity")
     ap.add_argument("--serve-batch", type=int, default=4096, help="poses per server launch set")
     ap.add_argument("--large-full-checks", type=int, default=6, help="full FP64 searches in the large workload")
+    ap.add_argument("--fp64-sample", type=int, default=400,
+                    help="rotations re-scored by the FP64 oracle in the single-ligand workload (0 = skip)")
+    ap.add_argument("--fp64-full", type=int, default=4, help="full FP64 searches in the single-ligand workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the search shards "

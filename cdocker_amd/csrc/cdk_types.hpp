@@ -22,6 +22,12 @@ struct LigandDesc {
   int list_off;   // into chan_atoms
   int ch_start[kMaxCh + 1];
   int pmap[kMaxCh];  // protein channel used for ligand channel c (TranRotaConfSearchNoMini.cu:136-146)
+  // The same lists with each channel padded to a multiple of 8 entries by -1
+  // (no atom): channel c at pad_atoms[pad_off + pad_start[c] ..), pad_start[c]
+  // a multiple of 8, pad_start[nch] the padded total.  Used by the atom form,
+  // whose matrix-core steps take 8 entries at a time.
+  int pad_off;
+  int pad_start[kMaxCh + 1];
 };
 
 // One pose = one (ligand, conformer, rotation) correlation.

@@ -223,6 +223,14 @@ std::unique_ptr<Ligand> make_ligand(const Protein& p, int natom, const float* ch
   d.pmap[L->num_vdw_used] = p.nch - 1;
   for (int a = 0; a < natom; ++a) L->chan_atoms.push_back(a);
   d.ch_start[nch] = (int)L->chan_atoms.size();
+  d.pad_off = 0;
+  for (int c = 0; c < nch; ++c) {
+    d.pad_start[c] = (int)L->pad_atoms.size();
+    L->pad_atoms.insert(L->pad_atoms.end(), L->chan_atoms.begin() + d.ch_start[c],
+                        L->chan_atoms.begin() + d.ch_start[c + 1]);
+    while (L->pad_atoms.size() % 8) L->pad_atoms.push_back(-1);
+  }
+  d.pad_start[nch] = (int)L->pad_atoms.size();
   L->wv.resize(natom);
   L->wq.resize(natom);
   for (int a = 0; a < natom; ++a) {
@@ -282,8 +290,10 @@ void build_tables(const Protein& prot, const std::vector<Job>& jobs, Prepared& t
     const Ligand& L = *j.lig;
     LigandDesc d = L.desc;
     d.list_off = (int)t.chan.size();
+    d.pad_off = (int)t.pad.size();
     d.w_off = (int)t.wv.size();
     t.chan.insert(t.chan.end(), L.chan_atoms.begin(), L.chan_atoms.end());
+    t.pad.insert(t.pad.end(), L.pad_atoms.begin(), L.pad_atoms.end());
     t.wv.insert(t.wv.end(), L.wv.begin(), L.wv.end());
     t.wq.insert(t.wq.end(), L.wq.begin(), L.wq.end());
     for (int c = 0; c < d.nch; ++c)
@@ -717,10 +727,10 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     if (need_atom || (need_pipe && atoms)) {
       auto mida = [&](int g) {
         return timed([&] {
-          hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+          hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_pad.as<int>(), t.d_wv.as<float>(),
                                      t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, n, g,
                                      prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), ctx.d_T.as<float2>(), invG,
-                                     tw),
+                                     tw, ctx.atom_products),
                     "tune atom mid");
         });
       };
@@ -789,10 +799,10 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
     timer.end(e);
     if (tu.pipe == kPipeAtoms) {
       e = timer.begin(2);
-      hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
+      hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_pad.as<int>(), t.d_wv.as<float>(),
                                  t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, n, tu.atom_group,
                                  prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), ctx.d_T.as<float2>(),
-                                 1.0f / (float)G, tw),
+                                 1.0f / (float)G, tw, ctx.atom_products),
                 "atom mid");
       timer.end(e);
       e = timer.begin(3);
@@ -853,6 +863,7 @@ void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& j
   t.drop_graph();
   t.ligs.clear();
   t.chan.clear();
+  t.pad.clear();
   t.wv.clear();
   t.wq.clear();
   t.coords.clear();
@@ -868,6 +879,7 @@ void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& j
   t.prot = &prot;
   upload(ctx, t.d_ligs, t.ligs);
   upload(ctx, t.d_chan, t.chan);
+  upload(ctx, t.d_pad, t.pad);
   upload(ctx, t.d_wv, t.wv);
   upload(ctx, t.d_wq, t.wq);
   upload(ctx, t.d_coords, t.coords);
@@ -902,7 +914,7 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
     if (use_graph && !ctx.time_stages) {
       if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.mid_group() ||
           t.graph_fwd != tu.fwd || t.graph_inv != tu.inv || t.graph_depth != tu.depth ||
-          t.graph_poses != tu.poses || t.graph_pipe != tu.pipe) {
+          t.graph_poses != tu.poses || t.graph_pipe != tu.pipe || t.graph_products != ctx.atom_products) {
         t.drop_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -921,6 +933,7 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
         t.graph_chunk = chunk;
         t.graph_group = tu.mid_group();
         t.graph_pipe = tu.pipe;
+        t.graph_products = ctx.atom_products;
         t.graph_fwd = tu.fwd;
         t.graph_inv = tu.inv;
         t.graph_depth = tu.depth;

@@ -83,6 +83,7 @@ struct Ligand {
   // kernel-side tables for this ligand alone (list_off = w_off = 0)
   LigandDesc desc{};
   std::vector<int> chan_atoms;
+  std::vector<int> pad_atoms;  // chan_atoms with each channel padded to a multiple of 8 by -1
   std::vector<float> wv, wq;
   int nch() const { return num_vdw_used + 1; }
 };
@@ -130,17 +131,18 @@ struct StageTimes {
 struct Prepared {
   const Protein* prot = nullptr;
   std::vector<LigandDesc> ligs;
-  std::vector<int> chan;
+  std::vector<int> chan, pad;
   std::vector<float> wv, wq, coords;
   std::vector<PoseSlot> slots;
   std::vector<size_t> job_first, job_n;
   int astride = 0, cmax = 0, boxmax = 2, max_chan_atoms = 1;
   int dummy_lig = 0;
-  DevBuf d_ligs, d_chan, d_wv, d_wq, d_coords, d_slots, d_keys, d_meta;
+  DevBuf d_ligs, d_chan, d_pad, d_wv, d_wq, d_coords, d_slots, d_keys, d_meta;
   hipGraphExec_t graph = nullptr;
   uint64_t graph_gen = 0;
   size_t graph_chunk = 0;
-  int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0, graph_pipe = -1;
+  int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0, graph_pipe = -1,
+      graph_products = -1;
   std::vector<PoseResults> results;
   Prepared() = default;
   Prepared(const Prepared&) = delete;
@@ -162,6 +164,7 @@ struct Context {
   int mid_depth = 0;              // mid-stage register ring depth with a fixed group; 0 = 2
   int mid_poses = 0;              // poses sharing each protein load in the mid stage with a fixed group; 0 = 1
   int pipeline = -1;              // fused pipeline form (kPipePlanes, kPipeColumns, kPipeAtoms); -1 = tuned
+  int atom_products = 0;          // atom form products: kAtomF16 (split FP16 matrix cores) or kAtomF32
   int fwd_variant = -1;           // forward plane stage form; -1 = tuned
   int inv_variant = -1;           // inverse plane stage form; -1 = tuned
   bool use_graph = true;

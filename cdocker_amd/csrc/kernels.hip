@@ -1782,6 +1782,203 @@ typedef float mfma_acc16 __attribute__((ext_vector_type(16)));
 #define CDK_ATOM_GROUP 4
 #endif
 
+typedef _Float16 mfma_h8 __attribute__((ext_vector_type(8)));
+typedef unsigned mfma_u4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+
+// Full-rate 24-bit multiply (the compiler narrows __umul24 to a quarter-rate
+// 32-bit multiply when only the product's low bits are used).
+__device__ __forceinline__ unsigned mul_u24(unsigned a, unsigned b) {
+  unsigned r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// Byte offset of W^(k i) in a table of N float2 (i = 8 * index at N = 64).
+template <int N>
+__device__ __forceinline__ unsigned atom_tw_bytes(unsigned k, unsigned i);
+
+// Stages padded slots [t_lo, t_hi) of one pose's channel lists for the atom
+// form (LigandDesc::pad_atoms: each channel padded to a multiple of 8 entries
+// by -1) into LDS slots 0..: a listed atom as ef = (wZ, fx wZ), wZ = the
+// weight times Z_a(kz) = (1 - fz) W^(kz iz) + fz W^(kz (iz + 1)), and ei =
+// (ix, iy, fy bits, 0), ix and iy times 8 (table byte offsets) at grid edge
+// 64; a pad as zeros (zero weight: every product it enters is exactly 0).
+// ag: the pose's placements.
+template <int N>
+__device__ void stage_atoms(float4* ef, int4* ei, const LigandDesc& Ld, int t_lo, int t_hi,
+                            const int* __restrict__ pad_atoms, const float* __restrict__ wv,
+                            const float* __restrict__ wq, const AtomGrid* __restrict__ ag, int kz,
+                            const float2* stw) {
+  const int pad_off = Ld.pad_off, w_off = Ld.w_off, t_elec = Ld.pad_start[Ld.nch - 1];
+  __syncthreads();  // the previous staged atoms (and anything sharing their memory) are done with
+  for (int k = threadIdx.x; k < t_hi - t_lo; k += blockDim.x) {
+    const int t = t_lo + k;
+    const int a = pad_atoms[pad_off + t];
+    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+    int4 q = make_int4(0, 0, 0, 0);
+    if (a >= 0) {
+      const AtomGrid g = ag[a];
+      // elec channel: the charge; vdW channels: sqrt(eps) (FillLigandGrid.cpp:45, :97)
+      const float wa = t >= t_elec ? wq[w_off + a] : wv[w_off + a];
+      const float2 z0 = stw[(kz * g.iz) % N], z1 = stw[(kz * (g.iz + 1)) % N];
+      const float2 zf = c_fma_s(z1, (float)g.fz, c_scale(z0, (float)(1.0 - g.fz)));
+      const float2 wz = make_float2(wa * zf.x, wa * zf.y);
+      const float fx = (float)g.fx;
+      constexpr int S = N == 64 ? 8 : 1;
+      f = make_float4(wz.x, wz.y, fx * wz.x, fx * wz.y);
+      q = make_int4(S * g.ix, S * g.iy, __float_as_int((float)g.fy), 0);
+    }
+    ef[k] = f;
+    ei[k] = q;
+  }
+  __syncthreads();
+}
+
+// Per pose: stages the whole padded list when it fits `ent` slots, else one
+// channel at a time (a channel pads to at most kMaxAtoms entries), and runs
+// body(c, k0, n) per channel with its n entries at staged slots [k0, k0 + n)
+// and zero entries after them to the next multiple of 8.
+template <int N, class Body>
+__device__ __forceinline__ void for_staged_channels(float4* ef, int4* ei, int ent, const LigandDesc& Ld,
+                                                    const int* __restrict__ pad_atoms,
+                                                    const float* __restrict__ wv, const float* __restrict__ wq,
+                                                    const AtomGrid* __restrict__ ag, int kz, const float2* stw,
+                                                    Body&& body) {
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  const int nch = uni(Ld.nch);
+  const int ptot = uni(Ld.pad_start[nch]);
+  const bool whole = ptot <= ent;
+  if (whole) stage_atoms<N>(ef, ei, Ld, 0, ptot, pad_atoms, wv, wq, ag, kz, stw);
+  for (int c = 0; c < nch; ++c) {
+    const int n = uni(Ld.ch_start[c + 1]) - uni(Ld.ch_start[c]);
+    const int p0 = uni(Ld.pad_start[c]);
+    if (!whole) stage_atoms<N>(ef, ei, Ld, p0, uni(Ld.pad_start[c + 1]), pad_atoms, wv, wq, ag, kz, stw);
+    body(c, whole ? p0 : 0, n);
+  }
+}
+
+// x = hi + lo to within 2^-22 |x| (normal range): hi = f16(x), lo = f16(x -
+// hi), both rounded to nearest, two values per register.  r = x - hi (exact).
+__device__ __forceinline__ void split_f16(float a, float b, unsigned& hi, unsigned& lo, float& ra, float& rb) {
+  const half2v h = __builtin_convertvector((float2v){a, b}, half2v);
+  const float2v hf = __builtin_convertvector(h, float2v);
+  ra = a - hf.x;
+  rb = b - hf.y;
+  hi = __builtin_bit_cast(unsigned, h);
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){ra, rb}, half2v));
+}
+
+__device__ __forceinline__ unsigned pack_f16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){a, b}, half2v));
+}
+
+// One channel's conj(L_c) on this lane's 32 x 32 block: entries [k0, kend)
+// of the staged lists ef and ei (zero entries follow to the next multiple of
+// 8 after kend), with
+// A[kx][(a, part)] = the part of conj(w Z X_a(kx)) and B[(a, part)][ky] =
+// the parts of conj(Y_a(ky)) that give the real (cr) and imaginary (ci)
+// sums, X_a(k) = W^(k ix) (1 - fx D(k)), Y_a(k) = W^(k iy) (1 - fy D(k)),
+// D = 1 - W^k (dx, dy: this lane's D at kxA, kyB).
+//
+// F16: the products on the FP16 matrix cores (32 x 32 x 16: eight atoms per
+// instruction, lane half h builds both parts of atoms 4h..4h+3 of each
+// eight), every operand split x = hi + lo into two FP16 values and A B =
+// Ah Bh + Ah Bl + Al Bh accumulated in FP32; the dropped Al Bl and the
+// splits leave a relative error of about 2^-21 per product.  Slots past
+// kend are the staged zero entries.  !F16: FP32 matrix cores (32 x 32 x 2: one atom
+// per instruction, lane half = part), AU atoms' operands in flight at once.
+template <int N, bool F16>
+__device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, const char* stwb, int k0, int kend,
+                                             int kxA, int kyB, float2 dx, float2 dy, int part, mfma_acc16& cr,
+                                             mfma_acc16& ci) {
+  auto lookups = [&](int k, float4& f, float2& tx, float2& ty, float& fy) {
+    f = ef[k];
+    const int4 ii = ei[k];
+    tx = *reinterpret_cast<const float2*>(stwb + atom_tw_bytes<N>(kxA, ii.x));
+    ty = *reinterpret_cast<const float2*>(stwb + atom_tw_bytes<N>(kyB, ii.y));
+    fy = __int_as_float(ii.z);
+  };
+  if constexpr (F16) {
+    for (int k = k0; k < kend; k += 8) {
+      unsigned ah[4], al[4], brh[4], brl[4], bih[4], bil[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float4 f;
+        float2 tx, ty;
+        float fy;
+        lookups(k + 4 * part + j, f, tx, ty, fy);
+        const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));  // wZ - fx wZ D
+        const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
+        // (a0, a1) = conj(W^(kx ix) g), k slots (real, imaginary)
+        const float a0 = fmaf(tx.x, gx, -(tx.y * gy));
+        const float a1 = fmaf(-tx.x, gy, -(tx.y * gx));
+        const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;  // 1 - fy D
+        const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
+        float r0, r1, rx, ry;
+        split_f16(a0, a1, ah[j], al[j], r0, r1);
+        split_f16(yx, yy, brh[j], brl[j], rx, ry);  // real sum: a0 yx + a1 yy
+        bih[j] = pack_f16(-yy, yx);                 // imaginary sum: -a0 yy + a1 yx
+        bil[j] = pack_f16(-ry, rx);
+      }
+      const mfma_h8 Ah = __builtin_bit_cast(mfma_h8, (mfma_u4){ah[0], ah[1], ah[2], ah[3]});
+      const mfma_h8 Al = __builtin_bit_cast(mfma_h8, (mfma_u4){al[0], al[1], al[2], al[3]});
+      const mfma_h8 Brh = __builtin_bit_cast(mfma_h8, (mfma_u4){brh[0], brh[1], brh[2], brh[3]});
+      const mfma_h8 Brl = __builtin_bit_cast(mfma_h8, (mfma_u4){brl[0], brl[1], brl[2], brl[3]});
+      const mfma_h8 Bih = __builtin_bit_cast(mfma_h8, (mfma_u4){bih[0], bih[1], bih[2], bih[3]});
+      const mfma_h8 Bil = __builtin_bit_cast(mfma_h8, (mfma_u4){bil[0], bil[1], bil[2], bil[3]});
+      cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brh, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bih, ci, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brl, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bil, ci, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Brh, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bih, ci, 0, 0, 0);
+    }
+  } else {
+    constexpr int AU = CDK_ATOM_GROUP;
+    // A = conj(w Z X) = conj(W^(kx ix) (wZ - fx wZ D)) (one component per
+    // lane: k slot `part`), B = conj(Y) = conj(W^(ky iy) (1 - fy D)).
+    auto atom = [&](int k, float& av, float& br, float& bi) {
+      float4 f;
+      float2 tx, ty;
+      float fy;
+      lookups(k, f, tx, ty, fy);
+      const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));
+      const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
+      const float p1 = part ? -gy : gx, p2 = part ? -gx : -gy;  // (A.x) or (-A.y) of tx g
+      av = fmaf(tx.x, p1, tx.y * p2);
+      const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;
+      const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
+      // conj(Y) = (yx, -yy): real part Br Ar - Bi Ai, imaginary Bi Ar + Br Ai
+      br = part ? yy : yx;
+      bi = part ? yx : -yy;
+    };
+    int k = k0;
+    for (; k + AU <= kend; k += AU) {
+      float av[AU], br[AU], bi[AU];
+#pragma unroll
+      for (int u = 0; u < AU; ++u) atom(k + u, av[u], br[u], bi[u]);
+#pragma unroll
+      for (int u = 0; u < AU; ++u) {
+        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], br[u], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bi[u], ci, 0, 0, 0);
+      }
+    }
+    for (; k < kend; ++k) {
+      float av, br, bi;
+      atom(k, av, br, bi);
+      cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av, br, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bi, ci, 0, 0, 0);
+    }
+  }
+}
+
+template <>
+__device__ __forceinline__ unsigned atom_tw_bytes<64>(unsigned k, unsigned i) {
+  return mul_u24(k, i) & (8u * 64u - 8u);
+}
+
 // Atom form, mid: one workgroup per (kz, group of poses); wavefront w owns
 // the 32 x 32 (kx, ky) block (w / 2, w % 2).  Per pose the ligand's channel
 // atoms (channel order) are staged in LDS, all of them when they fit, else
@@ -1795,10 +1992,10 @@ typedef float mfma_acc16 __attribute__((ext_vector_type(16)));
 // col_inverse finishes.  Per pose the arithmetic order is fixed (atoms in
 // channel order, one accumulation chain per channel), so results do not
 // depend on the group size or on what else shares the launch.
-template <class P>
-__global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict__ slots,
+template <class P, bool F16>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_mid_atoms(const PoseSlot* __restrict__ slots,
                                                       const LigandDesc* __restrict__ ligs,
-                                                      const int* __restrict__ chan_atoms,
+                                                      const int* __restrict__ pad_atoms,
                                                       const float* __restrict__ wv, const float* __restrict__ wq,
                                                       const AtomGrid* __restrict__ ag, int astride,
                                                       const RotMeta* __restrict__ meta, int nslots, int group,
@@ -1812,7 +2009,6 @@ __global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict
   constexpr int NPASS = N / LPP;
   constexpr int KZX = (ZH + 7) / 8;
   constexpr int ENT = 512;  // channel atoms staged at a time
-  constexpr int AU = CDK_ATOM_GROUP;  // atoms per group in the matrix-core loop
   // The staged atoms, the plane's contents and the inverse transforms'
   // scratch are never live together: all three share the plane's memory.
   constexpr int SCR = 4 * RS::SLICE * (int)sizeof(float2);
@@ -1848,82 +2044,18 @@ __global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict
     if (!uni(meta[s].valid)) continue;
     const int lig = uni(slots[s].lig);
     const LigandDesc& Ld = ligs[lig];
-    const int nch = uni(Ld.nch);
     const int wx = uni(meta[s].win[0]), wy = uni(meta[s].win[1]);
-    const int list_off = uni(Ld.list_off), w_off = uni(Ld.w_off);
-    const int ne = uni(Ld.ch_start[nch]), e_elec = uni(Ld.ch_start[nch - 1]);
     float2 E[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
-    // Stage channel atoms [e_lo, e_hi) into LDS slots 0.. (the whole list when
-    // it fits, else one channel at a time: a channel has at most kMaxAtoms).
-    int st0 = 0, st1 = 0;
-    auto stage = [&](int e_lo, int e_hi) {
-      __syncthreads();  // the previous pose's scratch and staged atoms are done with
-      for (int k = tid; k < e_hi - e_lo; k += blockDim.x) {
-        const int e = e_lo + k;
-        const int a = chan_atoms[list_off + e];
-        const AtomGrid g = ag[(size_t)s * astride + a];
-        // elec channel: the charge; vdW channels: sqrt(eps) (FillLigandGrid.cpp:45, :97)
-        const float wa = e >= e_elec ? wq[w_off + a] : wv[w_off + a];
-        const float2 z0 = stw[(kz * g.iz) & (N - 1)], z1 = stw[(kz * (g.iz + 1)) & (N - 1)];
-        const float2 zf = c_fma_s(z1, (float)g.fz, c_scale(z0, (float)(1.0 - g.fz)));
-        const float2 wz = make_float2(wa * zf.x, wa * zf.y);
-        const float fx = (float)g.fx;
-        ef[k] = make_float4(wz.x, wz.y, fx * wz.x, fx * wz.y);
-        ei[k] = make_int4(8 * g.ix, 8 * g.iy, __float_as_int((float)g.fy), 0);
-      }
-      __syncthreads();
-      st0 = e_lo;
-      st1 = e_hi;
-    };
-    for (int c = 0; c < nch; ++c) {
-      const int e0 = uni(Ld.ch_start[c]), e1 = uni(Ld.ch_start[c + 1]);
-      if (e1 > st1) stage(e0, ne <= ENT ? ne : e1);
+    for_staged_channels<N>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw, [&](int c, int k0, int n) {
       mfma_acc16 cr, ci;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         cr[r] = 0.f;
         ci[r] = 0.f;
       }
-      // Atoms in groups of AU: a group's entries and table values are all
-      // in flight together, then its operands are built and multiplied.
-      // A = conj(w Z X) = conj(W^(kx ix) (wZ - fx wZ D)) (one component per
-      // lane: k slot `part`), B = conj(Y) = conj(W^(ky iy) (1 - fy D)).
-      auto atom = [&](int k, float& av, float& br, float& bi) {
-        const float4 f = ef[k];
-        const int4 ii = ei[k];
-        const float2 tx = *reinterpret_cast<const float2*>(stwb + (__umul24(kxA, ii.x) & (8 * N - 8)));
-        const float2 ty = *reinterpret_cast<const float2*>(stwb + (__umul24(kyB, ii.y) & (8 * N - 8)));
-        const float fy = __int_as_float(ii.z);
-        const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));   // wZ - fx wZ D
-        const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
-        const float p1 = part ? -gy : gx, p2 = part ? -gx : -gy;  // (A.x) or (-A.y) of tx g
-        av = fmaf(tx.x, p1, tx.y * p2);
-        const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;    // 1 - fy D
-        const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
-        // conj(Y) = (yx, -yy): real part Br Ar - Bi Ai, imaginary Bi Ar + Br Ai
-        br = part ? yy : yx;
-        bi = part ? yx : -yy;
-      };
-      int k = e0 - st0;
-      const int kend = e1 - st0;
-      for (; k + AU <= kend; k += AU) {
-        float av[AU], br[AU], bi[AU];
-#pragma unroll
-        for (int u = 0; u < AU; ++u) atom(k + u, av[u], br[u], bi[u]);
-#pragma unroll
-        for (int u = 0; u < AU; ++u) {
-          cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], br[u], cr, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bi[u], ci, 0, 0, 0);
-        }
-      }
-      for (; k < kend; ++k) {
-        float av, br, bi;
-        atom(k, av, br, bi);
-        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av, br, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bi, ci, 0, 0, 0);
-      }
+      atom_channel<N, F16>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci);
       // sum += conj(L_c) P_c
       const int pc = uni(Ld.pmap[c]);
       const float2* pb = P3 + (((size_t)pc * ZH + kz) * 4 + wave) * (8 * 128) + lane * 2;
@@ -1940,7 +2072,7 @@ __global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict
           E[r].y = fmaf(ci[r], px, E[r].y);
         }
       }
-    }
+    });
     // the sum through the plane into the ky-line register layout
     __syncthreads();  // the staged atoms' reads are done
 #pragma unroll
@@ -1981,17 +2113,17 @@ __global__ __launch_bounds__(kBlock) void k_mid_atoms(const PoseSlot* __restrict
 // slots in both directions), 96-point register inverse transforms over x
 // (ky lines) and y (x' lines; 16 lines per wavefront) three wavefronts at a
 // time with their scratch in the plane, window rows into T2.
-template <int N>
-__device__ __forceinline__ unsigned tw_mod96_bytes(unsigned k, unsigned i) {
-  const unsigned t = __umul24(k, i);
-  const unsigned q = __umul24(t, 43691u) >> 22;
+template <>
+__device__ __forceinline__ unsigned atom_tw_bytes<96>(unsigned k, unsigned i) {
+  const unsigned t = mul_u24(k, i);
+  const unsigned q = mul_u24(t, 43691u) >> 22;
   return (t - 96u * q) * 8u;
 }
 
-template <class P>
+template <class P, bool F16>
 __global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict__ slots,
                                                      const LigandDesc* __restrict__ ligs,
-                                                     const int* __restrict__ chan_atoms,
+                                                     const int* __restrict__ pad_atoms,
                                                      const float* __restrict__ wv, const float* __restrict__ wq,
                                                      const AtomGrid* __restrict__ ag, int astride,
                                                      const RotMeta* __restrict__ meta, int nslots, int group,
@@ -2002,7 +2134,6 @@ __global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict_
   constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks per plane
   constexpr int KZX = (ZH + 7) / 8;
   constexpr int ENT = 512;
-  constexpr int AU = CDK_ATOM_GROUP;
   __shared__ float2 stw[N];
   __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
   __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0)
@@ -2028,76 +2159,18 @@ __global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict_
     if (!uni(meta[s].valid)) continue;
     const int lig = uni(slots[s].lig);
     const LigandDesc& Ld = ligs[lig];
-    const int nch = uni(Ld.nch);
-    const int list_off = uni(Ld.list_off), w_off = uni(Ld.w_off);
-    const int ne = uni(Ld.ch_start[nch]), e_elec = uni(Ld.ch_start[nch - 1]);
     float2 E[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
-    int st0 = 0, st1 = 0;
-    auto stage = [&](int e_lo, int e_hi) {
-      __syncthreads();  // the previous staged atoms are done with
-      for (int k = tid; k < e_hi - e_lo; k += blockDim.x) {
-        const int e = e_lo + k;
-        const int a = chan_atoms[list_off + e];
-        const AtomGrid g = ag[(size_t)s * astride + a];
-        // elec channel: the charge; vdW channels: sqrt(eps) (FillLigandGrid.cpp:45, :97)
-        const float wa = e >= e_elec ? wq[w_off + a] : wv[w_off + a];
-        const float2 z0 = stw[(kz * g.iz) % N], z1 = stw[(kz * (g.iz + 1)) % N];
-        const float2 zf = c_fma_s(z1, (float)g.fz, c_scale(z0, (float)(1.0 - g.fz)));
-        const float2 wz = make_float2(wa * zf.x, wa * zf.y);
-        const float fx = (float)g.fx;
-        ef[k] = make_float4(wz.x, wz.y, fx * wz.x, fx * wz.y);
-        ei[k] = make_int4(g.ix, g.iy, __float_as_int((float)g.fy), 0);
-      }
-      __syncthreads();
-      st0 = e_lo;
-      st1 = e_hi;
-    };
-    for (int c = 0; c < nch; ++c) {
-      const int e0 = uni(Ld.ch_start[c]), e1 = uni(Ld.ch_start[c + 1]);
-      if (e1 > st1) stage(e0, ne <= ENT ? ne : e1);
+    for_staged_channels<N>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw, [&](int c, int k0, int n) {
       mfma_acc16 cr, ci;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         cr[r] = 0.f;
         ci[r] = 0.f;
       }
-      // operands as k_mid_atoms: A = conj(w Z X), B = conj(Y), X_a(k) =
-      // W^(k ix) (1 - fx D), one component per lane slot `part`
-      auto atom = [&](int k, float& av, float& br, float& bi) {
-        const float4 f = ef[k];
-        const int4 ii = ei[k];
-        const float2 tx = *reinterpret_cast<const float2*>(stwb + tw_mod96_bytes<N>(kxA, ii.x));
-        const float2 ty = *reinterpret_cast<const float2*>(stwb + tw_mod96_bytes<N>(kyB, ii.y));
-        const float fy = __int_as_float(ii.z);
-        const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));
-        const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
-        const float p1 = part ? -gy : gx, p2 = part ? -gx : -gy;
-        av = fmaf(tx.x, p1, tx.y * p2);
-        const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;
-        const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
-        br = part ? yy : yx;
-        bi = part ? yx : -yy;
-      };
-      int k = e0 - st0;
-      const int kend = e1 - st0;
-      for (; k + AU <= kend; k += AU) {
-        float av[AU], br[AU], bi[AU];
-#pragma unroll
-        for (int u = 0; u < AU; ++u) atom(k + u, av[u], br[u], bi[u]);
-#pragma unroll
-        for (int u = 0; u < AU; ++u) {
-          cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], br[u], cr, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bi[u], ci, 0, 0, 0);
-        }
-      }
-      for (; k < kend; ++k) {
-        float av, br, bi;
-        atom(k, av, br, bi);
-        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(av, br, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bi, ci, 0, 0, 0);
-      }
+      atom_channel<N, F16>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci);
+      // sum += conj(L_c) P_c
       const int pc = uni(Ld.pmap[c]);
       const float2* pb = P3 + (((size_t)pc * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
 #pragma unroll
@@ -2113,7 +2186,7 @@ __global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict_
           E[r].y = fmaf(ci[r], px, E[r].y);
         }
       }
-    }
+    });
     float2* eb = Eout + (((size_t)s * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
@@ -2566,21 +2639,23 @@ bool atoms_supported(int N) {
 }
 
 hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
-                            const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
+                            const int* pad_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
                             const RotMeta* meta, int nslots, int group, const float2* P3, float2* work, float2* T2,
-                            float invG, const float2* tw) {
+                            float invG, const float2* tw, int products) {
   if (nslots <= 0) return hipSuccess;
-  if (!atoms_supported(N) || group <= 0) return hipErrorInvalidValue;
+  if (!atoms_supported(N) || group <= 0 || (products != kAtomF16 && products != kAtomF32))
+    return hipErrorInvalidValue;
   const int kzx = (N / 2 + 1 + 7) / 8;
   const dim3 grid((unsigned)(8 * kzx) * ceil_div(nslots, group));
+  const bool f16 = products == kAtomF16;
   CDK_DISPATCH_N(N, {
     if constexpr (PL::N == 64)
-      hipLaunchKernelGGL(k_mid_atoms<PL>, grid, dim3(kBlock), 0, st, slots, ligs, chan_atoms, wv, wq, ag, astride,
-                         meta, nslots, group, P3, T2, invG, tw);
+      hipLaunchKernelGGL((f16 ? k_mid_atoms<PL, true> : k_mid_atoms<PL, false>), grid, dim3(kBlock), 0, st, slots, ligs,
+                         pad_atoms, wv, wq, ag, astride, meta, nslots, group, P3, T2, invG, tw);
     if constexpr (PL::N == 96) {
       const dim3 gf((unsigned)(8 * kzx * (N / 32)) * ceil_div(nslots, group));
-      hipLaunchKernelGGL(k_atoms96_fwd<PL>, gf, dim3(192), 0, st, slots, ligs, chan_atoms, wv, wq, ag, astride, meta,
-                         nslots, group, P3, work, tw);
+      hipLaunchKernelGGL((f16 ? k_atoms96_fwd<PL, true> : k_atoms96_fwd<PL, false>), gf, dim3(192), 0, st, slots, ligs,
+                         pad_atoms, wv, wq, ag, astride, meta, nslots, group, P3, work, tw);
       hipLaunchKernelGGL(k_atoms96_inv<PL>, dim3(PL::ZH, nslots), dim3(384), mid_atoms96_lds_bytes(), st, meta, work,
                          T2, invG, tw);
     }

@@ -76,13 +76,17 @@ hipError_t launch_to_kz_blocks(hipStream_t st, int N, const float2* in, float2* 
 // (launch_to_mfma_blocks).
 constexpr int kPipeAtoms = 2;
 bool atoms_supported(int N);
+// Atom form ligand-spectrum products: FP16 matrix cores on operands split
+// into two FP16 halves each, or FP32 matrix cores.
+constexpr int kAtomF16 = 0, kAtomF32 = 1;
+// pad_atoms: the ligands' padded channel lists (LigandDesc::pad_atoms).
 // work: at grid edge 96 the product spectra between the two kernels,
 // atoms_work_bytes(N) per pose (unused at 64).
 size_t atoms_work_bytes(int N);
 hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
-                            const int* chan_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
+                            const int* pad_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
                             const RotMeta* meta, int nslots, int group, const float2* P3, float2* work, float2* T2,
-                            float invG, const float2* tw);
+                            float invG, const float2* tw, int products);
 hipError_t launch_to_mfma_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
 // Inverse plane stage variants: one workgroup per (pose, window plane) with
 // block-wide LDS FFTs, or one workgroup per (pose, 4 window planes) with

@@ -253,6 +253,41 @@ def test_atom_form_large_ligand(ctx, p64):
         lig.close()
 
 
+@pytest.mark.parametrize("name", ["p64", "p96"])
+def test_atom_products_split_f16_and_f32(ctx, name):
+    """The atom form's two product forms (split-FP16 matrix cores, the
+    default, and FP32 matrix cores) against the oracle, each within the parity
+    tolerance with a scaled error well under it, and the option in effect
+    (the two forms round differently)."""
+    g = workloads.protein_grids(name)
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    ctx.set_option("pipeline", 2)
+    try:
+        assert ctx.get_option("atom_products") == 0
+        for seed, na, nq in ((941, 23, 5), (942, 60, 4)):
+            L = workloads.ligand(seed, na, 1)
+            quats = workloads.quaternion_sets(seed + 1, 1, nq)
+            lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+            res = {}
+            for products in (0, 1):
+                ctx.set_option("atom_products", products)
+                E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats)
+                assert ctx.get_option("last_pipeline") == 2
+                stats = check_search(prob, L["conformers"], quats, E, idx, mc)
+                say(name, "atom products", products, "ligand", na, stats)
+                assert stats.get("exact", 0) >= nq - 1
+                assert stats["max_scaled_err"] <= 1e-5
+                res[products] = E
+            assert not np.array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+            ctx.set_option("atom_products", 0)
+            lig.close()
+    finally:
+        ctx.set_option("atom_products", 0)
+        ctx.set_option("pipeline", -1)
+        prot.close()
+
+
 def test_atom_form_96_vs_oracle(ctx):
     """Atom form at 96^3 (nine 32 x 32 matrix-core blocks per kz plane, the
     twiddle index reduced mod 96, 96-point inverse transforms) against the
