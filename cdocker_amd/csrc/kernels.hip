@@ -1803,8 +1803,8 @@ __device__ __forceinline__ unsigned atom_tw_bytes(unsigned k, unsigned i);
 // form (LigandDesc::pad_atoms: each channel padded to a multiple of 8 entries
 // by -1) into LDS slots 0..: a listed atom as ef = (wZ, fx wZ), wZ = the
 // weight times Z_a(kz) = (1 - fz) W^(kz iz) + fz W^(kz (iz + 1)), and ei =
-// (ix, iy, fy bits, 0), ix and iy times 8 (table byte offsets) at grid edge
-// 64; a pad as zeros (zero weight: every product it enters is exactly 0).
+// (ix, iy, fy bits, 0), ix and iy times 8 (table byte offsets) at grid edges
+// 64 and 128; a pad as zeros (zero weight: every product it enters is exactly 0).
 // ag: the pose's placements.
 template <int N>
 __device__ void stage_atoms(float4* ef, int4* ei, const LigandDesc& Ld, int t_lo, int t_hi,
@@ -1826,7 +1826,7 @@ __device__ void stage_atoms(float4* ef, int4* ei, const LigandDesc& Ld, int t_lo
       const float2 zf = c_fma_s(z1, (float)g.fz, c_scale(z0, (float)(1.0 - g.fz)));
       const float2 wz = make_float2(wa * zf.x, wa * zf.y);
       const float fx = (float)g.fx;
-      constexpr int S = N == 64 ? 8 : 1;
+      constexpr int S = (N & (N - 1)) == 0 ? 8 : 1;  // byte offsets for a power-of-two N
       f = make_float4(wz.x, wz.y, fx * wz.x, fx * wz.y);
       q = make_int4(S * g.ix, S * g.iy, __float_as_int((float)g.fy), 0);
     }
@@ -1860,18 +1860,21 @@ __device__ __forceinline__ void for_staged_channels(float4* ef, int4* ei, int en
 }
 
 // x = hi + lo to within 2^-22 |x| (normal range): hi = f16(x), lo = f16(x -
-// hi), both rounded to nearest, two values per register.  r = x - hi (exact).
-__device__ __forceinline__ void split_f16(float a, float b, unsigned& hi, unsigned& lo, float& ra, float& rb) {
-  const half2v h = __builtin_convertvector((float2v){a, b}, half2v);
-  const float2v hf = __builtin_convertvector(h, float2v);
-  ra = a - hf.x;
-  rb = b - hf.y;
-  hi = __builtin_bit_cast(unsigned, h);
+// hi), both rounded to nearest, two values per register.  The residual x - hi
+// (exact in FP32) comes from one mixed-precision fma per value.
+__device__ __forceinline__ void split_f16(float a, float b, unsigned& hi, unsigned& lo) {
+  hi = __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){a, b}, half2v));
+  float ra, rb;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(ra) : "v"(hi), "v"(a));                // a - hi.lo
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(rb) : "v"(hi), "v"(b));  // b - hi.hi
   lo = __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){ra, rb}, half2v));
 }
 
-__device__ __forceinline__ unsigned pack_f16(float a, float b) {
-  return __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){a, b}, half2v));
+// (x, y) -> (-y, x) on a packed FP16 pair (an exact multiply by 1).
+__device__ __forceinline__ unsigned swap_neg_f16(unsigned h) {
+  unsigned r;
+  asm("v_pk_mul_f16 %0, %1, 1.0 op_sel:[1,0] op_sel_hi:[0,0] neg_lo:[1,0]" : "=v"(r) : "v"(h));
+  return r;
 }
 
 // One channel's conj(L_c) on this lane's 32 x 32 block: entries [k0, kend)
@@ -1916,11 +1919,10 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
         const float a1 = fmaf(-tx.x, gy, -(tx.y * gx));
         const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;  // 1 - fy D
         const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
-        float r0, r1, rx, ry;
-        split_f16(a0, a1, ah[j], al[j], r0, r1);
-        split_f16(yx, yy, brh[j], brl[j], rx, ry);  // real sum: a0 yx + a1 yy
-        bih[j] = pack_f16(-yy, yx);                 // imaginary sum: -a0 yy + a1 yx
-        bil[j] = pack_f16(-ry, rx);
+        split_f16(a0, a1, ah[j], al[j]);
+        split_f16(yx, yy, brh[j], brl[j]);  // real sum: a0 yx + a1 yy
+        bih[j] = swap_neg_f16(brh[j]);      // imaginary sum: -a0 yy + a1 yx
+        bil[j] = swap_neg_f16(brl[j]);
       }
       const mfma_h8 Ah = __builtin_bit_cast(mfma_h8, (mfma_u4){ah[0], ah[1], ah[2], ah[3]});
       const mfma_h8 Al = __builtin_bit_cast(mfma_h8, (mfma_u4){al[0], al[1], al[2], al[3]});
@@ -1977,6 +1979,11 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
 template <>
 __device__ __forceinline__ unsigned atom_tw_bytes<64>(unsigned k, unsigned i) {
   return mul_u24(k, i) & (8u * 64u - 8u);
+}
+
+template <>
+__device__ __forceinline__ unsigned atom_tw_bytes<128>(unsigned k, unsigned i) {
+  return mul_u24(k, i) & (8u * 128u - 8u);
 }
 
 // Atom form, mid: one workgroup per (kz, group of poses); wavefront w owns
@@ -2097,22 +2104,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   }
 }
 
-// Atom form at grid edge 96, in two kernels so that each runs at its own
-// occupancy (the 96-point inverse transforms hold 24 values per lane).
+// Atom form at grid edges 96 and 128, in two kernels so that each runs at its
+// own occupancy (the 96- and 128-point inverse transforms hold 24 and 16
+// values per lane and need the whole plane in LDS).
 //
-// k_atoms96_fwd: one workgroup per (kz, row of 32 x 32 kx blocks, group of
-// poses), three wavefronts, one block each: the same matrix-core ligand
-// spectra and product as k_mid_atoms, the table index reduced mod 96 exactly
-// by a multiply and shift ((t * 43691) >> 22 = t / 96 for t <= 95 * 95).  The
-// pose's product spectrum goes to E[slot][kz][block][r/2][lane][2] (the
-// matrix-core result layout: 1 KB per wavefront store).
+// k_atoms_fwd: one workgroup per (kz, row of 32 x 32 kx blocks, group of
+// poses), N / 32 wavefronts, one block each: the same matrix-core ligand
+// spectra and product as k_mid_atoms (at 96 the table index reduced mod 96
+// exactly by a multiply and shift: (t * 43691) >> 22 = t / 96 for t <= 95 *
+// 95).  The pose's product spectrum goes to E[slot][kz][block][r/2][lane][2]
+// (the matrix-core result layout: 1 KB per wavefront store).
 //
-// k_atoms96_inv: one workgroup per (pose, kz), six wavefronts: the nine
-// blocks into a plane of row stride 97 in dynamic LDS (with 4 lanes per line
-// an odd stride puts every 16 consecutive lanes on distinct 8-byte bank
-// slots in both directions), 96-point register inverse transforms over x
-// (ky lines) and y (x' lines; 16 lines per wavefront) three wavefronts at a
-// time with their scratch in the plane, window rows into T2.
+// k_atoms_inv: one workgroup per (pose, kz), N / (lines per wavefront)
+// wavefronts (6 at 96, 16 at 128): the blocks into a plane of odd row stride
+// N + 1 in dynamic LDS (consecutive lanes then fall on distinct 8-byte bank
+// slots reading along either axis), register inverse transforms over x (ky
+// lines) and y (x' lines) half of the wavefronts at a time with their scratch
+// in the plane, window rows into T2.
 template <>
 __device__ __forceinline__ unsigned atom_tw_bytes<96>(unsigned k, unsigned i) {
   const unsigned t = mul_u24(k, i);
@@ -2121,7 +2129,7 @@ __device__ __forceinline__ unsigned atom_tw_bytes<96>(unsigned k, unsigned i) {
 }
 
 template <class P, bool F16>
-__global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict__ slots,
+__global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ slots,
                                                      const LigandDesc* __restrict__ ligs,
                                                      const int* __restrict__ pad_atoms,
                                                      const float* __restrict__ wv, const float* __restrict__ wq,
@@ -2130,13 +2138,13 @@ __global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict_
                                                      const float2* __restrict__ P3, float2* __restrict__ Eout,
                                                      const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
-  static_assert(N == 96, "the 96-point atom form");
+  static_assert(N == 96 || N == 128, "the two-kernel atom form: grid edges 96 and 128");
   constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks per plane
   constexpr int KZX = (ZH + 7) / 8;
   constexpr int ENT = 512;
   __shared__ float2 stw[N];
   __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
-  __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0)
+  __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0; times 8 at 128)
   const int w = blockIdx.x;
   const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
   const int rest = (w >> 3) / KZX;
@@ -2195,17 +2203,17 @@ __global__ __launch_bounds__(192) void k_atoms96_fwd(const PoseSlot* __restrict_
 }
 
 template <class P>
-__global__ __launch_bounds__(384) void k_atoms96_inv(const RotMeta* __restrict__ meta,
+__global__ __launch_bounds__(1024) void k_atoms_inv(const RotMeta* __restrict__ meta,
                                                      const float2* __restrict__ Ein, float2* __restrict__ T2,
                                                      float invG, const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
-  static_assert(N == 96, "the 96-point atom form");
+  static_assert(N == 96 || N == 128, "the two-kernel atom form: grid edges 96 and 128");
   using RS = RegShape<N>;
   constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
   constexpr int NB = N / 32, NW = NB * NB;
   constexpr int S = N + 1;      // plane row stride
-  constexpr int NLW = N / LPW;  // wavefronts carrying transform lines (all six)
-  constexpr int RW = 3;         // wavefronts transforming at a time
+  constexpr int NLW = N / LPW;  // wavefronts, each carrying LPW transform lines
+  constexpr int RW = NLW / 2;   // wavefronts transforming at a time
   static_assert(RW * RS::SLICE <= N * S, "scratch fits the plane");
   extern __shared__ float2 pl[];  // [N * S]
   __shared__ float2 stwr[L * V];
@@ -2217,7 +2225,7 @@ __global__ __launch_bounds__(384) void k_atoms96_inv(const RotMeta* __restrict__
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const int part = lane >> 5;
-  // the nine blocks into the plane [kx][ky]
+  // the blocks into the plane [kx][ky]
   for (int b = wave; b < NW; b += NLW) {
     const float2* eb = Ein + (((size_t)s * ZH + kz) * NW + b) * (8 * 128) + lane * 2;
     const int kx0 = 32 * (b / NB), ky = 32 * (b % NB) + (lane & 31);
@@ -2282,7 +2290,7 @@ __global__ __launch_bounds__(384) void k_atoms96_inv(const RotMeta* __restrict__
   }
 }
 
-size_t mid_atoms96_lds_bytes() { return (size_t)96 * 97 * sizeof(float2); }
+size_t atoms_inv_lds_bytes(int N) { return (size_t)N * (N + 1) * sizeof(float2); }
 
 // Column form, last stage: one workgroup per (pose, block of 64/L * 4 row
 // pairs).  Row pair (x', y0 = L*m + 2h, y0 + 1) is one complex line A + iB
@@ -2296,7 +2304,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
   constexpr int N = P::N, ZH = P::ZH;
   using RS = RegShape<N>;
   constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
-  static_assert(N == 64 || N == 96, "column and atom forms: grid edges 64 and 96");
+  static_assert(N == 64 || N == 96 || N == 128, "column and atom forms: grid edges 64, 96 and 128");
   constexpr int H = L / 2;        // row pairs per L rows
   constexpr int LPB = 4 * LPW;    // row pairs per workgroup
   __shared__ float2 sbuf[4][RS::SLICE];
@@ -2550,10 +2558,10 @@ hipError_t configure_fwd_reg(size_t max_dyn_lds) {
           e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_z<PL>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
       }
-      if constexpr (PL::N == 96) {
-        if (e == hipSuccess && mid_atoms96_lds_bytes() + 2 * 96 * sizeof(float2) <= max_dyn_lds)
-          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_atoms96_inv<PL>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)mid_atoms96_lds_bytes());
+      if constexpr (PL::N == 96 || PL::N == 128) {
+        if (e == hipSuccess && atoms_inv_lds_bytes(PL::N) + PL::N * sizeof(float2) <= max_dyn_lds)
+          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_atoms_inv<PL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)atoms_inv_lds_bytes(PL::N));
       }
     });
     if (e != hipSuccess) return e;
@@ -2620,7 +2628,7 @@ hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, 
   if (nslots <= 0) return hipSuccess;
   if (!col_supported(N) && !atoms_supported(N)) return hipErrorInvalidValue;
   CDK_DISPATCH_N(N, {
-    if constexpr (PL::N == 64 || PL::N == 96) {
+    if constexpr (PL::N == 64 || PL::N == 96 || PL::N == 128) {
       using RS = RegShape<PL::N>;
       // windows are at most N - 2 wide (boxes are at least 3 cells)
       const int w = PL::N - 2;
@@ -2632,10 +2640,13 @@ hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, 
   return hipGetLastError();
 }
 
-size_t atoms_work_bytes(int N) { return N == 96 ? (size_t)N * N * (N / 2 + 1) * sizeof(float2) : 0; }
+size_t atoms_work_bytes(int N) {
+  return N == 96 || N == 128 ? (size_t)N * N * (N / 2 + 1) * sizeof(float2) : 0;
+}
 
 bool atoms_supported(int N) {
-  return N == 64 || (N == 96 && mid_atoms96_lds_bytes() + 2 * 96 * sizeof(float2) <= g_max_dyn_lds);
+  // the inverse kernel's plane (dynamic) and register-FFT twiddles (static)
+  return N == 64 || ((N == 96 || N == 128) && atoms_inv_lds_bytes(N) + N * sizeof(float2) <= g_max_dyn_lds);
 }
 
 hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
@@ -2652,12 +2663,13 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
     if constexpr (PL::N == 64)
       hipLaunchKernelGGL((f16 ? k_mid_atoms<PL, true> : k_mid_atoms<PL, false>), grid, dim3(kBlock), 0, st, slots, ligs,
                          pad_atoms, wv, wq, ag, astride, meta, nslots, group, P3, T2, invG, tw);
-    if constexpr (PL::N == 96) {
-      const dim3 gf((unsigned)(8 * kzx * (N / 32)) * ceil_div(nslots, group));
-      hipLaunchKernelGGL((f16 ? k_atoms96_fwd<PL, true> : k_atoms96_fwd<PL, false>), gf, dim3(192), 0, st, slots, ligs,
-                         pad_atoms, wv, wq, ag, astride, meta, nslots, group, P3, work, tw);
-      hipLaunchKernelGGL(k_atoms96_inv<PL>, dim3(PL::ZH, nslots), dim3(384), mid_atoms96_lds_bytes(), st, meta, work,
-                         T2, invG, tw);
+    if constexpr (PL::N == 96 || PL::N == 128) {
+      constexpr int NB = PL::N / 32, NLW = PL::N / RegShape<PL::N>::LPW;
+      const dim3 gf((unsigned)(8 * kzx * NB) * ceil_div(nslots, group));
+      hipLaunchKernelGGL((f16 ? k_atoms_fwd<PL, true> : k_atoms_fwd<PL, false>), gf, dim3(64 * NB), 0, st, slots,
+                         ligs, pad_atoms, wv, wq, ag, astride, meta, nslots, group, P3, work, tw);
+      hipLaunchKernelGGL(k_atoms_inv<PL>, dim3(PL::ZH, nslots), dim3(64 * NLW), atoms_inv_lds_bytes(PL::N), st, meta,
+                         work, T2, invG, tw);
     }
   });
   return hipGetLastError();
@@ -2668,7 +2680,7 @@ hipError_t launch_to_mfma_blocks(hipStream_t st, int N, const float2* in, float2
   if (n == 0) return hipSuccess;
   if (!atoms_supported(N)) return hipErrorInvalidValue;
   CDK_DISPATCH_N(N, {
-    if constexpr (PL::N == 64 || PL::N == 96)
+    if constexpr (PL::N == 64 || PL::N == 96 || PL::N == 128)
       hipLaunchKernelGGL(k_to_mfma_blocks<PL>, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, in, out, nch);
   });
   return hipGetLastError();

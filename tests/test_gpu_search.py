@@ -288,11 +288,13 @@ def test_atom_products_split_f16_and_f32(ctx, name):
         prot.close()
 
 
-def test_atom_form_96_vs_oracle(ctx):
-    """Atom form at 96^3 (nine 32 x 32 matrix-core blocks per kz plane, the
-    twiddle index reduced mod 96, 96-point inverse transforms) against the
-    oracle, and the same bits for every mid-stage group size."""
-    g = workloads.protein_grids("p96")
+@pytest.mark.parametrize("name", ["p96", "p128"])
+def test_atom_form_96_128_vs_oracle(ctx, name):
+    """Atom form at 96^3 and 128^3 (nine or sixteen 32 x 32 matrix-core blocks
+    per kz plane, at 96 the twiddle index reduced mod 96; the product spectra
+    through HBM to a second kernel's 96- or 128-point inverse transforms)
+    against the oracle, and the same bits for every mid-stage group size."""
+    g = workloads.protein_grids(name)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     ctx.set_option("pipeline", 2)
     try:
@@ -300,12 +302,12 @@ def test_atom_form_96_vs_oracle(ctx):
             L = workloads.ligand(seed, na, 1)
             quats = workloads.quaternion_sets(seed + 1, 1, nq)
             lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
-            say("atom form 96 ligand", na)
+            say("atom form", name, "ligand", na)
             E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats)
             assert ctx.get_option("last_pipeline") == 2
             prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
             stats = check_search(prob, L["conformers"], quats, E, idx, mc)
-            say("atom form 96", na, stats)
+            say("atom form", name, na, stats)
             assert stats.get("exact", 0) >= nq - 1
             for grp in (1, 3):
                 ctx.set_option("mid_group", grp)
