@@ -10,9 +10,11 @@
 //     inverse x FFT, protein spectra staged in LDS and reused across poses)
 //     -> (inverse y FFT + z C2R of the window rows only + argmin).
 #include <hip/hip_runtime.h>
+
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "cdk_types.hpp"
 #include "fft_lds.hpp"
@@ -1877,7 +1879,8 @@ __device__ __forceinline__ unsigned swap_neg_f16(unsigned h) {
   return r;
 }
 
-// One channel's conj(L_c) on this lane's 32 x 32 block: entries [k0, kend)
+// One channel's conj(L_c) on this lane's 32 x 32 block (cr, ci: real and
+// imaginary parts, written here): entries [k0, kend)
 // of the staged lists ef and ei (zero entries follow to the next multiple of
 // 8 after kend), with
 // A[kx][(a, part)] = the part of conj(w Z X_a(kx)) and B[(a, part)][ky] =
@@ -1904,7 +1907,10 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
     fy = __int_as_float(ii.z);
   };
   if constexpr (F16) {
-    for (int k = k0; k < kend; k += 8) {
+    // At 96 and 128 the first step starts from a zero accumulator operand (no
+    // zeroing moves); the fused 64-point kernel, at its register limit, runs
+    // faster with one copy of the step.
+    auto step = [&](int k, auto first) {
       unsigned ah[4], al[4], brh[4], brl[4], bih[4], bil[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -1930,14 +1936,38 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
       const mfma_h8 Brl = __builtin_bit_cast(mfma_h8, (mfma_u4){brl[0], brl[1], brl[2], brl[3]});
       const mfma_h8 Bih = __builtin_bit_cast(mfma_h8, (mfma_u4){bih[0], bih[1], bih[2], bih[3]});
       const mfma_h8 Bil = __builtin_bit_cast(mfma_h8, (mfma_u4){bil[0], bil[1], bil[2], bil[3]});
-      cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brh, cr, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bih, ci, 0, 0, 0);
+      if constexpr (decltype(first)::value) {
+        const mfma_acc16 zero = {};
+        cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brh, zero, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bih, zero, 0, 0, 0);
+      } else {
+        cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brh, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bih, ci, 0, 0, 0);
+      }
       cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brl, cr, 0, 0, 0);
       ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bil, ci, 0, 0, 0);
       cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Brh, cr, 0, 0, 0);
       ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bih, ci, 0, 0, 0);
+    };
+    if (N == 64 || k0 >= kend) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        cr[r] = 0.f;
+        ci[r] = 0.f;
+      }
+    }
+    if constexpr (N == 64) {
+      for (int k = k0; k < kend; k += 8) step(k, std::false_type{});
+    } else if (k0 < kend) {
+      step(k0, std::true_type{});
+      for (int k = k0 + 8; k < kend; k += 8) step(k, std::false_type{});
     }
   } else {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      cr[r] = 0.f;
+      ci[r] = 0.f;
+    }
     constexpr int AU = CDK_ATOM_GROUP;
     // A = conj(w Z X) = conj(W^(kx ix) (wZ - fx wZ D)) (one component per
     // lane: k slot `part`), B = conj(Y) = conj(W^(ky iy) (1 - fy D)).
@@ -2057,11 +2087,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
     for_staged_channels<N>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw, [&](int c, int k0, int n) {
       mfma_acc16 cr, ci;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        cr[r] = 0.f;
-        ci[r] = 0.f;
-      }
       atom_channel<N, F16>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci);
       // sum += conj(L_c) P_c
       const int pc = uni(Ld.pmap[c]);
@@ -2172,11 +2197,6 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
     for_staged_channels<N>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw, [&](int c, int k0, int n) {
       mfma_acc16 cr, ci;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        cr[r] = 0.f;
-        ci[r] = 0.f;
-      }
       atom_channel<N, F16>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci);
       // sum += conj(L_c) P_c
       const int pc = uni(Ld.pmap[c]);
