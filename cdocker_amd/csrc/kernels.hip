@@ -1895,10 +1895,23 @@ __device__ __forceinline__ unsigned swap_neg_f16(unsigned h) {
 // splits leave a relative error of about 2^-21 per product.  Slots past
 // kend are the staged zero entries.  !F16: FP32 matrix cores (32 x 32 x 2: one atom
 // per instruction, lane half = part), AU atoms' operands in flight at once.
-template <int N, bool F16>
+// A operands shared by the SW wavefronts of a workgroup that own blocks of the
+// same kx rows (SW = 1: each wavefront builds its own): per 8-entry step,
+// wavefront w builds and splits the A values of entries j = w, w + SW, ...
+// (j < 4 per lane half) into buf[par][j][lane] = (hi, lo), one barrier, and
+// every wavefront reads all four; two buffers alternate, so one barrier per
+// step suffices (a buffer is rewritten only after every wavefront passed the
+// barrier that follows its last reads).
+struct AShare {
+  uint2* buf;  // [2][4][64]
+  int wave;    // this wavefront's index among the SW
+  int par;     // buffer of the next step
+};
+
+template <int N, bool F16, int SW = 1>
 __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, const char* stwb, int k0, int kend,
                                              int kxA, int kyB, float2 dx, float2 dy, int part, mfma_acc16& cr,
-                                             mfma_acc16& ci) {
+                                             mfma_acc16& ci, AShare* share = nullptr) {
   auto lookups = [&](int k, float4& f, float2& tx, float2& ty, float& fy) {
     f = ef[k];
     const int4 ii = ei[k];
@@ -1910,25 +1923,64 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
     // At 96 and 128 the first step starts from a zero accumulator operand (no
     // zeroing moves); the fused 64-point kernel, at its register limit, runs
     // faster with one copy of the step.
+    // (a0, a1) = conj(W^(kx ix) g), g = wZ - fx wZ D, k slots (real, imaginary)
+    auto a_op = [&](int kk, unsigned& hi, unsigned& lo) {
+      const float4 f = ef[kk];
+      const float2 tx = *reinterpret_cast<const float2*>(stwb + atom_tw_bytes<N>(kxA, ei[kk].x));
+      const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));
+      const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
+      const float a0 = fmaf(tx.x, gx, -(tx.y * gy));
+      const float a1 = fmaf(-tx.x, gy, -(tx.y * gx));
+      split_f16(a0, a1, hi, lo);
+    };
     auto step = [&](int k, auto first) {
       unsigned ah[4], al[4], brh[4], brl[4], bih[4], bil[4];
+      if constexpr (SW > 1) {
+        uint2* sb = share->buf + share->par * (4 * 64);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j % SW == share->wave) {
+            unsigned hi, lo;
+            a_op(k + 4 * part + j, hi, lo);
+            sb[j * 64 + (threadIdx.x & 63)] = make_uint2(hi, lo);
+          }
+      }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        float4 f;
-        float2 tx, ty;
-        float fy;
-        lookups(k + 4 * part + j, f, tx, ty, fy);
-        const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));  // wZ - fx wZ D
-        const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
-        // (a0, a1) = conj(W^(kx ix) g), k slots (real, imaginary)
-        const float a0 = fmaf(tx.x, gx, -(tx.y * gy));
-        const float a1 = fmaf(-tx.x, gy, -(tx.y * gx));
+        const int kk = k + 4 * part + j;
+        float2 ty;
+        float fy, a0 = 0.f, a1 = 0.f;
+        if constexpr (SW == 1) {
+          // this lane's A and B for entry kk, lookups together
+          float4 f;
+          float2 tx;
+          lookups(kk, f, tx, ty, fy);
+          const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));
+          const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
+          a0 = fmaf(tx.x, gx, -(tx.y * gy));
+          a1 = fmaf(-tx.x, gy, -(tx.y * gx));
+        } else {
+          const int4 ii = ei[kk];
+          ty = *reinterpret_cast<const float2*>(stwb + atom_tw_bytes<N>(kyB, ii.y));
+          fy = __int_as_float(ii.z);
+        }
         const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;  // 1 - fy D
         const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
-        split_f16(a0, a1, ah[j], al[j]);
+        if constexpr (SW == 1) split_f16(a0, a1, ah[j], al[j]);
         split_f16(yx, yy, brh[j], brl[j]);  // real sum: a0 yx + a1 yy
         bih[j] = swap_neg_f16(brh[j]);      // imaginary sum: -a0 yy + a1 yx
         bil[j] = swap_neg_f16(brl[j]);
+      }
+      if constexpr (SW > 1) {
+        __syncthreads();
+        const uint2* sb = share->buf + share->par * (4 * 64);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint2 v = sb[j * 64 + (threadIdx.x & 63)];
+          ah[j] = v.x;
+          al[j] = v.y;
+        }
+        share->par ^= 1;
       }
       const mfma_h8 Ah = __builtin_bit_cast(mfma_h8, (mfma_u4){ah[0], ah[1], ah[2], ah[3]});
       const mfma_h8 Al = __builtin_bit_cast(mfma_h8, (mfma_u4){al[0], al[1], al[2], al[3]});
@@ -2170,6 +2222,7 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
   __shared__ float2 stw[N];
   __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
   __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0; times 8 at 128)
+  __shared__ uint2 abuf[2 * 4 * 64];  // the shared A operands (the row's wavefronts share kx)
   const int w = blockIdx.x;
   const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
   const int rest = (w >> 3) / KZX;
@@ -2187,6 +2240,7 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
   auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
   __syncthreads();
   const float2 dx = make_float2(1.f - stw[kxA].x, -stw[kxA].y), dy = make_float2(1.f - stw[kyB].x, -stw[kyB].y);
+  AShare share{abuf, wave, 0};
   const int s0 = sg * group, s1 = min(nslots, s0 + group);
   for (int s = s0; s < s1; ++s) {
     if (!uni(meta[s].valid)) continue;
@@ -2197,7 +2251,7 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
     for_staged_channels<N>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw, [&](int c, int k0, int n) {
       mfma_acc16 cr, ci;
-      atom_channel<N, F16>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci);
+      atom_channel<N, F16, F16 ? NB : 1>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci, &share);
       // sum += conj(L_c) P_c
       const int pc = uni(Ld.pmap[c]);
       const float2* pb = P3 + (((size_t)pc * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
