@@ -253,7 +253,7 @@ def test_atom_form_large_ligand(ctx, p64):
         lig.close()
 
 
-@pytest.mark.parametrize("name", ["p64", "p96"])
+@pytest.mark.parametrize("name", ["p64", "p96", "p128"])
 def test_atom_products_split_f16_and_f32(ctx, name):
     """The atom form's two product forms (split-FP16 matrix cores, the
     default, and FP32 matrix cores) against the oracle, each within the parity
@@ -293,13 +293,21 @@ def test_atom_form_96_128_vs_oracle(ctx, name):
     """Atom form at 96^3 and 128^3 (nine or sixteen 32 x 32 matrix-core blocks
     per kz plane, at 96 the twiddle index reduced mod 96; the product spectra
     through HBM to a second kernel's 96- or 128-point inverse transforms)
-    against the oracle, and the same bits for every mid-stage group size."""
+    against the oracle, and the same bits for every mid-stage group size.  The
+    300-atom ligand's padded channel lists exceed the 512 staged entries, so
+    the forward kernel stages them one channel at a time."""
     g = workloads.protein_grids(name)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     ctx.set_option("pipeline", 2)
     try:
-        for seed, na, nq in ((911, 18, 5), (912, 45, 4)):
+        for seed, na, nq in ((911, 18, 5), (912, 45, 4), (913, 300, 2)):
             L = workloads.ligand(seed, na, 1)
+            if na > 100:
+                # compact coordinates: a ball of radius 6 A keeps the box well inside the grid
+                rs = np.random.default_rng(seed)
+                d = rs.normal(size=(na, 3))
+                x = 6.0 * d / np.linalg.norm(d, axis=1, keepdims=True) * rs.uniform(0, 1, (na, 1)) ** (1 / 3)
+                L["conformers"] = x[None].astype(np.float32)
             quats = workloads.quaternion_sets(seed + 1, 1, nq)
             lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
             say("atom form", name, "ligand", na)
