@@ -408,6 +408,9 @@ def line(args, metric, value, total, workload, extra_config, roof, kernels, cpu,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "arithmetic": ("FP32 grids, transforms, protein products and sums; ligand-spectrum products on the FP16 "
+                       "matrix cores with hi/lo operand splits (3 FP16 products per FP32-accurate product, FP32 "
+                       "accumulate)") if str((roof or {}).get("products", "")).startswith("split") else "FP32",
         "data": "synthetic (seeded pseudo-protein grids, random-walk ligands, torsion conformers, Shoemake quaternions)",
         "config": dict({"workload": workload}, **extra_config),
         "roofline": roof,
