@@ -639,7 +639,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
     ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
     ap.add_argument("--pipeline", type=int, default=-1,
-                    help="fused engine form: 0 planes, 1 columns (64^3), 2 atoms (64^3, 96^3), -1 autotuned")
+                    help="fused engine form: 0 planes, 1 columns (64^3), 2 atoms (64^3, 96^3, 128^3), -1 autotuned")
     ap.add_argument("--atom-products", type=int, default=0,
                     help="atom form products: 0 split-FP16 matrix cores, 1 FP32 matrix cores")
     ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 register, -1 autotuned")

@@ -63,7 +63,7 @@ int cdk_context_destroy(cdk_context* ctx);
  * form, whose forward stage writes ligand planes after the z and y
  * transforms; 1 column form at 64^3, whose forward stage stops after the z
  * transform and whose mid stage runs the y and x transforms per kz plane;
- * 2 atom form at 64^3 and 96^3, whose mid stage builds each channel's ligand
+ * 2 atom form at 64^3, 96^3 and 128^3, whose mid stage builds each channel's ligand
  * spectrum per kz plane from the atoms as a sum of outer products on the
  * matrix cores, with no forward stage; -1 = autotuned once per grid size),
  * "atom_products" (the atom form's products: 0 = FP16 matrix cores on
