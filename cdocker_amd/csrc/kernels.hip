@@ -2276,6 +2276,18 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
   }
 }
 
+// float2 of the inverse kernel's dynamic LDS: the plane (row stride N + 1),
+// at 128 enlarged to every wavefront's transform scratch.
+template <int N>
+constexpr int atoms_inv_lds_floats() {
+  constexpr int plane = N * (N + 1);
+  if constexpr (N == 128) {
+    constexpr int scr = (N / RegShape<N>::LPW) * RegShape<N>::SLICE;
+    return plane > scr ? plane : scr;
+  }
+  return plane;
+}
+
 template <class P>
 __global__ __launch_bounds__(1024) void k_atoms_inv(const RotMeta* __restrict__ meta,
                                                      const float2* __restrict__ Ein, float2* __restrict__ T2,
@@ -2287,9 +2299,12 @@ __global__ __launch_bounds__(1024) void k_atoms_inv(const RotMeta* __restrict__ 
   constexpr int NB = N / 32, NW = NB * NB;
   constexpr int S = N + 1;      // plane row stride
   constexpr int NLW = N / LPW;  // wavefronts, each carrying LPW transform lines
-  constexpr int RW = NLW / 2;   // wavefronts transforming at a time
-  static_assert(RW * RS::SLICE <= N * S, "scratch fits the plane");
-  extern __shared__ float2 pl[];  // [N * S]
+  // wavefronts transforming at a time: all at 128 (one workgroup per CU
+  // either way, so the LDS holds every wavefront's scratch), half at 96 (two
+  // workgroups per CU fit only with the plane-sized LDS)
+  constexpr int RW = N == 128 ? NLW : NLW / 2;
+  static_assert(RW * RS::SLICE <= atoms_inv_lds_floats<N>(), "scratch fits the LDS");
+  extern __shared__ float2 pl[];  // [N * S], then the transforms' scratch
   __shared__ float2 stwr[L * V];
   const int kz = blockIdx.x, s = blockIdx.y;
   if (!meta[s].valid) return;
@@ -2364,7 +2379,9 @@ __global__ __launch_bounds__(1024) void k_atoms_inv(const RotMeta* __restrict__ 
   }
 }
 
-size_t atoms_inv_lds_bytes(int N) { return (size_t)N * (N + 1) * sizeof(float2); }
+size_t atoms_inv_lds_bytes(int N) {
+  return (size_t)(N == 128 ? atoms_inv_lds_floats<128>() : N * (N + 1)) * sizeof(float2);
+}
 
 // Column form, last stage: one workgroup per (pose, block of 64/L * 4 row
 // pairs).  Row pair (x', y0 = L*m + 2h, y0 + 1) is one complex line A + iB
