@@ -1879,6 +1879,19 @@ __device__ __forceinline__ unsigned swap_neg_f16(unsigned h) {
   return r;
 }
 
+// A operands shared by the SW wavefronts of a workgroup that own blocks of the
+// same kx rows (SW = 1: each wavefront builds its own): per 8-entry step,
+// wavefront w builds and splits the A values of entries j = w, w + SW, ...
+// (j < 4 per lane half) into buf[par][j][lane] = (hi, lo), one barrier, and
+// every wavefront reads all four; two buffers alternate, so one barrier per
+// step suffices (a buffer is rewritten only after every wavefront passed the
+// barrier that follows its last reads).
+struct AShare {
+  uint2* buf;  // [2][4][64]
+  int wave;    // this wavefront's index among the SW
+  int par;     // buffer of the next step
+};
+
 // One channel's conj(L_c) on this lane's 32 x 32 block (cr, ci: real and
 // imaginary parts, written here): entries [k0, kend)
 // of the staged lists ef and ei (zero entries follow to the next multiple of
@@ -1895,19 +1908,6 @@ __device__ __forceinline__ unsigned swap_neg_f16(unsigned h) {
 // splits leave a relative error of about 2^-21 per product.  Slots past
 // kend are the staged zero entries.  !F16: FP32 matrix cores (32 x 32 x 2: one atom
 // per instruction, lane half = part), AU atoms' operands in flight at once.
-// A operands shared by the SW wavefronts of a workgroup that own blocks of the
-// same kx rows (SW = 1: each wavefront builds its own): per 8-entry step,
-// wavefront w builds and splits the A values of entries j = w, w + SW, ...
-// (j < 4 per lane half) into buf[par][j][lane] = (hi, lo), one barrier, and
-// every wavefront reads all four; two buffers alternate, so one barrier per
-// step suffices (a buffer is rewritten only after every wavefront passed the
-// barrier that follows its last reads).
-struct AShare {
-  uint2* buf;  // [2][4][64]
-  int wave;    // this wavefront's index among the SW
-  int par;     // buffer of the next step
-};
-
 template <int N, bool F16, int SW = 1>
 __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, const char* stwb, int k0, int kend,
                                              int kxA, int kyB, float2 dx, float2 dy, int part, mfma_acc16& cr,
@@ -1920,9 +1920,6 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
     fy = __int_as_float(ii.z);
   };
   if constexpr (F16) {
-    // At 96 and 128 the first step starts from a zero accumulator operand (no
-    // zeroing moves); the fused 64-point kernel, at its register limit, runs
-    // faster with one copy of the step.
     // (a0, a1) = conj(W^(kx ix) g), g = wZ - fx wZ D, k slots (real, imaginary)
     auto a_op = [&](int kk, unsigned& hi, unsigned& lo) {
       const float4 f = ef[kk];
@@ -2001,6 +1998,9 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
       cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Brh, cr, 0, 0, 0);
       ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bih, ci, 0, 0, 0);
     };
+    // At 96 and 128 the first step starts from a zero accumulator operand (no
+    // zeroing moves); the fused 64-point kernel, at its register limit, runs
+    // faster with one copy of the step.
     if (N == 64 || k0 >= kend) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
