@@ -264,6 +264,9 @@ def test_atom_products_split_f16_and_f32(ctx, name):
     ctx.set_option("pipeline", 2)
     try:
         assert ctx.get_option("atom_products") == 0
+        with pytest.raises(cdk.CdkError):
+            ctx.set_option("atom_products", 2)
+        assert ctx.get_option("atom_products") == 0
         for seed, na, nq in ((941, 23, 5), (942, 60, 4)):
             L = workloads.ligand(seed, na, 1)
             quats = workloads.quaternion_sets(seed + 1, 1, nq)
