@@ -429,7 +429,7 @@ def run_single(args, cdk, workloads):
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
                       inv_variant=args.inv, pipeline=args.pipeline,
-                      atom_products=args.atom_products)
+                      atom_products=args.atom_products, overlap_z=args.overlap_z)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     nu, used, _ = lig.channels()
@@ -497,7 +497,7 @@ def run_batched(args, cdk, workloads):
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
                       inv_variant=args.inv, pipeline=args.pipeline,
-                      atom_products=args.atom_products)
+                      atom_products=args.atom_products, overlap_z=args.overlap_z)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     N = g["n"]
     ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
@@ -580,7 +580,7 @@ def run_large(args, cdk, workloads):
     log("inputs ready in %.1fs" % (time.time() - t0))
     ctx = cdk.Context(0, use_graph=int(not args.no_graph), fused_chunk=args.chunk, mid_group=args.group, fwd_variant=args.fwd,
                       inv_variant=args.inv, pipeline=args.pipeline,
-                      atom_products=args.atom_products)
+                      atom_products=args.atom_products, overlap_z=args.overlap_z)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
     C = lig.channels()[0] + 1
@@ -640,6 +640,8 @@ def main():
     ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
     ap.add_argument("--pipeline", type=int, default=-1,
                     help="fused engine form: 0 planes, 1 columns (64^3), 2 atoms (64^3, 96^3, 128^3), -1 autotuned")
+    ap.add_argument("--overlap-z", type=int, default=1,
+                    help="atom form: overlap each launch set's z-inverse stage with the next set (1) or not (0)")
     ap.add_argument("--atom-products", type=int, default=0,
                     help="atom form products: 0 split-FP16 matrix cores, 1 FP32 matrix cores")
     ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 register, -1 autotuned")

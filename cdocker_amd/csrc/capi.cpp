@@ -161,6 +161,9 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
     } else if (k == "mid_poses") {
       require(v == 0 || v == 1 || v == 2 || v == 4, "mid_poses must be 0, 1, 2 or 4");
       c.mid_poses = (int)v;
+    } else if (k == "overlap_z") {
+      require(v == 0 || v == 1, "overlap_z must be 0 or 1");
+      c.overlap_z = (int)v;
     } else if (k == "atom_products") {
       require(v == cdk::kAtomF16 || v == cdk::kAtomF32, "atom_products must be 0 (split FP16 matrix cores) or 1 (FP32)");
       c.atom_products = (int)v;
@@ -205,6 +208,7 @@ int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value) {
     else if (k == "mid_poses") *value = c.mid_poses;
     else if (k == "pipeline") *value = c.pipeline;
     else if (k == "atom_products") *value = c.atom_products;
+    else if (k == "overlap_z") *value = c.overlap_z;
     else if (k == "last_pipeline") *value = c.last_pipe;
     else if (k == "last_mid_poses") *value = c.last_poses;
     else if (k == "max_dyn_lds") *value = (int64_t)c.max_dyn_lds;

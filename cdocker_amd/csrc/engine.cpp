@@ -43,6 +43,7 @@ Context::Context(int dev) : device(dev) {
   if (dev < 0 || dev >= n) throw Error(kErrInvalid, "device index out of range");
   hip_check(hipSetDevice(dev), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamCreateWithFlags(&stream_z, hipStreamNonBlocking), "hipStreamCreate");
   int lds = 0;
   hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
   max_dyn_lds = (size_t)lds;
@@ -52,9 +53,12 @@ Context::Context(int dev) : device(dev) {
 Context::~Context() {
   (void)hipSetDevice(device);
   if (stream) (void)hipStreamSynchronize(stream);
+  if (stream_z) (void)hipStreamSynchronize(stream_z);
   for (auto& kv : twiddles) delete kv.second;
   if (h_keys) (void)hipHostFree(h_keys);
   if (h_meta) (void)hipHostFree(h_meta);
+  for (hipEvent_t e : set_events) (void)hipEventDestroy(e);
+  if (stream_z) (void)hipStreamDestroy(stream_z);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -775,6 +779,25 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   return tu;
 }
 
+// The atom form's z-inverse stage runs on its own stream, one launch set
+// behind (not when stages are timed: their events are on the launch stream).
+bool overlap_z_stage(const Context& ctx, const Tuned& tu) {
+  return tu.pipe == kPipeAtoms && ctx.overlap_z && !ctx.time_stages;
+}
+
+// The second T2 buffer and the ordering events of an overlapped search, made
+// before any capture.
+void ensure_overlap(Context& ctx, const Protein& prot, const Prepared& t, size_t chunk) {
+  const int N = prot.N;
+  ensure_work(ctx, chunk * N * (size_t)N * (N / 2 + 1) * sizeof(float2), ctx.d_T2b);
+  const size_t sets = (t.slots.size() + chunk - 1) / chunk;
+  while (ctx.set_events.size() < 2 * sets) {
+    hipEvent_t e = nullptr;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    ctx.set_events.push_back(e);
+  }
+}
+
 // Fused pipeline: four launches per chunk of poses, any mix of ligands, in
 // the plane or the column form.
 void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk, const Tuned& tu,
@@ -788,9 +811,12 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
   RotMeta* d_meta = t.d_meta.as<RotMeta>();
   const size_t total = t.slots.size();
   const int group = std::max(1, tu.group);
+  const bool ovl = overlap_z_stage(ctx, tu);
+  size_t last_set = 0;
   for (size_t s = 0; s < total; s += chunk) {
     const int n = (int)std::min(chunk, total - s);
     const PoseSlot* sl = d_slots + s;
+    const size_t si = s / chunk;  // launch set index
     hipEvent_t e;
     e = timer.begin(0);
     hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, t.boxmax,
@@ -798,17 +824,28 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
               "prep");
     timer.end(e);
     if (tu.pipe == kPipeAtoms) {
+      // overlapped: T2 alternates between two buffers; set si's mid stage
+      // waits for set si - 2's z-inverse, the last reader of its buffer
+      float2* T2 = ovl && (si & 1) ? ctx.d_T2b.as<float2>() : ctx.d_T.as<float2>();
+      if (ovl && si >= 2) hip_check(hipStreamWaitEvent(ctx.stream, ctx.set_events[2 * (si - 2) + 1], 0), "wait");
       e = timer.begin(2);
       hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_pad.as<int>(), t.d_wv.as<float>(),
                                  t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, n, tu.atom_group,
-                                 prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), ctx.d_T.as<float2>(),
-                                 1.0f / (float)G, tw, ctx.atom_products),
+                                 prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), T2, 1.0f / (float)G, tw,
+                                 ctx.atom_products),
                 "atom mid");
       timer.end(e);
-      e = timer.begin(3);
-      hip_check(launch_inv_z(ctx.stream, N, d_meta + s, n, ctx.d_T.as<float2>(), d_keys + s, tw),
-                "atom inverse + argmin");
-      timer.end(e);
+      if (ovl) {
+        hip_check(hipEventRecord(ctx.set_events[2 * si], ctx.stream), "record");
+        hip_check(hipStreamWaitEvent(ctx.stream_z, ctx.set_events[2 * si], 0), "wait");
+        hip_check(launch_inv_z(ctx.stream_z, N, d_meta + s, n, T2, d_keys + s, tw), "atom inverse + argmin");
+        hip_check(hipEventRecord(ctx.set_events[2 * si + 1], ctx.stream_z), "record");
+        last_set = si;
+      } else {
+        e = timer.begin(3);
+        hip_check(launch_inv_z(ctx.stream, N, d_meta + s, n, T2, d_keys + s, tw), "atom inverse + argmin");
+        timer.end(e);
+      }
       continue;
     }
     if (tu.pipe == kPipeColumns) {
@@ -845,6 +882,12 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
     hip_check(launch_inv_argmin(ctx.stream, N, tu.inv, d_meta + s, n, N - 2, ctx.d_T.as<float2>(), d_keys + s, tw),
               "inverse + argmin");
     timer.end(e);
+  }
+  // join: the last two sets' z-inverse stages (earlier ones are ordered
+  // before later mid stages) back into the launch stream
+  if (ovl) {
+    for (size_t si = last_set > 0 ? last_set - 1 : 0; si <= last_set; ++si)
+      hip_check(hipStreamWaitEvent(ctx.stream, ctx.set_events[2 * si + 1], 0), "join");
   }
 }
 
@@ -911,10 +954,13 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
     ctx.last_group = tu.mid_group();
     ctx.last_depth = tu.depth;
     ctx.last_poses = tu.poses;
+    const int ovl = overlap_z_stage(ctx, tu) ? 1 : 0;
+    if (ovl) ensure_overlap(ctx, prot, t, chunk);
     if (use_graph && !ctx.time_stages) {
       if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.mid_group() ||
           t.graph_fwd != tu.fwd || t.graph_inv != tu.inv || t.graph_depth != tu.depth ||
-          t.graph_poses != tu.poses || t.graph_pipe != tu.pipe || t.graph_products != ctx.atom_products) {
+          t.graph_poses != tu.poses || t.graph_pipe != tu.pipe || t.graph_products != ctx.atom_products ||
+          t.graph_overlap != ovl) {
         t.drop_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -934,6 +980,7 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
         t.graph_group = tu.mid_group();
         t.graph_pipe = tu.pipe;
         t.graph_products = ctx.atom_products;
+        t.graph_overlap = ovl;
         t.graph_fwd = tu.fwd;
         t.graph_inv = tu.inv;
         t.graph_depth = tu.depth;

@@ -142,7 +142,7 @@ struct Prepared {
   uint64_t graph_gen = 0;
   size_t graph_chunk = 0;
   int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0, graph_pipe = -1,
-      graph_products = -1;
+      graph_products = -1, graph_overlap = -1;
   std::vector<PoseResults> results;
   Prepared() = default;
   Prepared(const Prepared&) = delete;
@@ -154,6 +154,11 @@ struct Prepared {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // The atom form's z-inverse stage of launch set i runs here, overlapped
+  // with set i + 1's mid stage on `stream` (two T2 buffers; the ordering
+  // events, two per launch set, are created before any capture).
+  hipStream_t stream_z = nullptr;
+  std::vector<hipEvent_t> set_events;
   std::mutex mu;
   std::map<int, DevBuf*> twiddles;
   // options
@@ -164,6 +169,7 @@ struct Context {
   int mid_depth = 0;              // mid-stage register ring depth with a fixed group; 0 = 2
   int mid_poses = 0;              // poses sharing each protein load in the mid stage with a fixed group; 0 = 1
   int pipeline = -1;              // fused pipeline form (kPipePlanes, kPipeColumns, kPipeAtoms); -1 = tuned
+  int overlap_z = 1;              // atom form: overlap each launch set's z-inverse stage with the next set
   int atom_products = 0;          // atom form products: kAtomF16 (split FP16 matrix cores) or kAtomF32
   int fwd_variant = -1;           // forward plane stage form; -1 = tuned
   int inv_variant = -1;           // inverse plane stage form; -1 = tuned
@@ -188,7 +194,7 @@ struct Context {
   // per-launch-set working arrays; ws_generation counts reallocations so a
   // captured graph that baked in the old addresses is rebuilt
   DevBuf d_rot, d_ag;
-  DevBuf d_A, d_T, d_L, d_Lhat, d_S, d_E;
+  DevBuf d_A, d_T, d_T2b, d_L, d_Lhat, d_S, d_E;
   uint64_t ws_generation = 1;
   // pinned host staging
   void* h_keys = nullptr;

@@ -69,7 +69,9 @@ int cdk_context_destroy(cdk_context* ctx);
  * "atom_products" (the atom form's products: 0 = FP16 matrix cores on
  * operands split into two FP16 halves each, three products per FP32-accurate
  * product with FP32 accumulation, about 2^-21 relative error per product;
- * 1 = FP32 matrix cores; default 0),
+ * 1 = FP32 matrix cores; default 0), "overlap_z" (atom form: 1 = each
+ * launch set's z-inverse stage runs on a second stream beside the next set's
+ * mid stage, with two T2 buffers; 0 = one stream; default 1),
  * "fwd_variant" (forward
  * plane stage: 0 block-wide LDS Stockham FFTs, 1 register four-step FFTs,
  * -1 = autotuned once per grid size), "inv_variant" (inverse
