@@ -69,6 +69,38 @@ def test_chunking_does_not_change_results(ctx, prots):
     assert same(E0, E1) and np.array_equal(i0, i1)
 
 
+def test_overlapped_z_stage_equals_one_stream(ctx, prots):
+    """The atom form's z-inverse stage on a second stream beside the next
+    launch set (two T2 buffers, event-ordered) gives the same bits as the
+    one-stream order, replayed as a hipGraph and eagerly, over many small
+    launch sets so both buffers and every cross-stream wait are used."""
+    say("overlapped z stage")
+    g, prot = prots[1]
+    w = workloads.config2(nconf=3, nquat=120)
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    ctx.set_option("pipeline", 2)
+    ctx.set_option("fused_chunk", 37)
+    try:
+        res = {}
+        for ovl in (0, 1):
+            for graph in (1, 0):
+                ctx.set_option("overlap_z", ovl)
+                ctx.set_option("use_graph", graph)
+                for rep in range(2):  # a second replay of the same graph
+                    res[(ovl, graph, rep)] = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+                assert ctx.get_option("last_pipeline") == 2
+        E0, i0, m0 = res[(0, 1, 0)]
+        for key, (E, i, m) in res.items():
+            assert same(E, E0) and np.array_equal(i, i0) and np.array_equal(m, m0), key
+    finally:
+        ctx.set_option("overlap_z", 1)
+        ctx.set_option("use_graph", 1)
+        ctx.set_option("fused_chunk", 0)
+        ctx.set_option("pipeline", -1)
+        lig.close()
+
+
 def _fixed_mid(ctx, grp, depth, poses):
     ctx.set_option("pipeline", 0)  # the schedules below are the plane form's
     ctx.set_option("mid_group", grp)
