@@ -291,6 +291,37 @@ def test_atom_products_split_f16_and_f32(ctx, name):
         prot.close()
 
 
+@pytest.mark.parametrize("name", ["p64", "p96", "p128"])
+def test_atom_form_channel_padding_edges(ctx, name):
+    """Channel lists padded to multiples of 8 entries: a one-atom ligand (7
+    pad slots per channel), one vdW channel of exactly 8 atoms (no pad) and
+    one of 9 (a second step with one atom and 7 pads), against the oracle."""
+    g = workloads.protein_grids(name)
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    ctx.set_option("pipeline", 2)
+    try:
+        for seed, na, rad in ((951, 1, None), (952, 8, 1.7), (953, 9, 1.7), (954, 17, None)):
+            L = workloads.ligand(seed, na, 1)
+            radius = np.full(na, rad, np.float32) if rad is not None else L["radius"]
+            quats = workloads.quaternion_sets(seed + 1, 1, 4)
+            lig = cdk.Ligand(prot, L["charge"], L["epsilon"], radius)
+            nu, used, ari = lig.channels()
+            counts = [int((ari == u).sum()) for u in used] + [na]
+            say("padding edges", name, "atoms", na, "channel counts", counts)
+            if rad is not None:
+                assert counts == [na, na]
+            E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats)
+            assert ctx.get_option("last_pipeline") == 2
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], radius)
+            stats = check_search(prob, L["conformers"], quats, E, idx, mc)
+            say("padding edges", name, na, stats)
+            assert stats.get("exact", 0) + stats.get("empty", 0) >= 3
+            lig.close()
+    finally:
+        ctx.set_option("pipeline", -1)
+        prot.close()
+
+
 @pytest.mark.parametrize("name", ["p96", "p128"])
 def test_atom_form_96_128_vs_oracle(ctx, name):
     """Atom form at 96^3 and 128^3 (nine or sixteen 32 x 32 matrix-core blocks
