@@ -168,6 +168,23 @@ def profiled_traffic(kernel, poses_per_launch):
     return None
 
 
+def profiled_issue(kernel):
+    """The SIMD-cycle fractions (vector-instruction issue, matrix-core busy)
+    of `kernel` from the committed rocprofv3 SQ-counter summary of the
+    single-ligand workload; None if absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for ln in f:
+                if ln.startswith("issue-per-simd-cycle:"):
+                    kv = dict(t.split("=") for t in ln.split(":", 1)[1].split())
+                    if kv.get("kernel") == kernel:
+                        return {"valu_issue": float(kv["valu_issue"]), "mfma_busy": float(kv["mfma_busy"]),
+                                "source": "profiles/traffic_single_ligand_64.md (rocprofv3 SQ counters)"}
+    except OSError:
+        pass
+    return None
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
@@ -449,6 +466,10 @@ def run_single(args, cdk, workloads):
     if tr is not None:
         roof["traffic"] = round(tr)
         roof["traffic_source"] = "profiles/traffic_single_ligand_64.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per pose)"
+    iss = profiled_issue(roof["kernel"]) if args.protein == "p64" and roof.get("products", "").startswith("split") else None
+    if iss is not None:
+        # what binds the kernel: vector-instruction issue, not the matrix cores
+        roof["issue"] = iss
 
     pc = max(1, min(args.plain_conformers, args.conformers))
     ref = reference_leg(ctx, cdk, prot, lig, L["conformers"][:pc], L["quats"][:pc], idx[:pc])
