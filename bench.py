@@ -209,21 +209,35 @@ def serve_leg(ctx, cdk, workloads, nreq, load, max_batch, seed):
     # ligand (its cost is the transforms', nearly independent of atom count):
     # the time it would take for the same requests, each on its own grid.
     L = workloads.config2(nconf=1, nquat=1000)["ligands"][0]
-    plain = []
+    plain, fused = [], []
     for p in prots:
         lig = cdk.Ligand(p, L["charge"], L["epsilon"], L["radius"])
         pb = cdk.Batch(ctx, p)
         pb.add(lig, L["conformers"][:1], L["quats"][:1])
-        pb.run(cdk.ENGINE_REFERENCE)
-        best = 1e30
-        for _ in range(2):
-            ts = time.perf_counter()
-            pb.run(cdk.ENGINE_REFERENCE)
-            best = min(best, time.perf_counter() - ts)
-        plain.append(L["quats"].shape[1] / best)
+        for engine, out in ((cdk.ENGINE_REFERENCE, plain), (cdk.ENGINE_FUSED, fused)):
+            pb.run(engine)
+            best = 1e30
+            for _ in range(2):
+                ts = time.perf_counter()
+                pb.run(engine)
+                best = min(best, time.perf_counter() - ts)
+            out.append(L["quats"].shape[1] / best)
         pb.close()
         lig.close()
     plain_s = float(sum(n / plain[r["protein_index"]] for n, r in zip(poses, reqs)))
+    # The same requests through the plain loop at its HBM roofline: the
+    # reference dataflow's algorithmic bytes B(N, C) per pose (each stage
+    # boundary written and read once) at 8 TB/s, C the request's channels.
+    edges = [p.n for p in prots]
+    chans = []
+    for r in reqs:
+        lg = cdk.Ligand(prots[r["protein_index"]], r["charge"], r["epsilon"], r["radius"])
+        chans.append(lg.channels()[0] + 1)
+        lg.close()
+    roof_s = float(sum(n * reference_bytes(edges[r["protein_index"]], c) / (PEAK_HBM_GBPS * 1e9)
+                       for n, r, c in zip(poses, reqs, chans)))
+    # each request alone on the fused engine (its poses at that protein's rate)
+    alone = np.array([n / fused[r["protein_index"]] for n, r in zip(poses, reqs)])
     budget = (8 << 30) + (8 << 30)  # context workspace + resident request state
 
     def run(arrivals):
@@ -269,9 +283,17 @@ def serve_leg(ctx, cdk, workloads, nreq, load, max_batch, seed):
         "peak_admitted_bytes": int(st["peak_admitted_bytes"]),
         "max_batch_poses": max_batch,
         "proteins": "64^3 x 9 and 96^3 x 9 FP32 at 0.5 A, spectra resident",
+        "slowdown_p50": round(percentile(lat / alone, 50), 2),
+        "slowdown_p99": round(percentile(lat / alone, 99), 2),
+        "slowdown_note": "latency / the request's own GPU time alone (its poses at the fused engine's rate on its "
+                         "protein); request sizes span 500 to 250,000 poses on two grid sizes, so p99/p50 of "
+                         "latency mostly reflects request size",
+        "fused_alone_poses_per_s": {"64": round(fused[0], 1), "96": round(fused[1], 1)},
         "plain_loop_poses_per_s": {"64": round(plain[0], 1), "96": round(plain[1], 1)},
         "plain_loop_s": round(plain_s, 3),
         "capacity_vs_plain_loop": round(plain_s / span0, 2),
+        "plain_loop_roofline_s": round(roof_s, 3),
+        "capacity_vs_roofline_plain_loop": round(roof_s / span0, 2),
     }
     return out
 
@@ -331,15 +353,20 @@ def kernel_leg(ctx, cdk, batch, geoms, npch, N):
         # third of the FP16 peak is that form's ceiling); FP32 form: FP32 units.
         tf = atom_mfma_flops(geoms, N) / kernels[dom]["launches"] / dom_avg_s / 1e12
         hbm = dict(roof)
-        f16 = ctx.get_option("atom_products") == 0
+        products = ctx.get_option("atom_products")
+        f16 = products in (0, 2)
+        nsplit = {0: 3, 2: 6}.get(products, 1)  # FP16 partial products per FP32-accurate product
         peak = PEAK_MFMA_FP16_TFLOPS if f16 else PEAK_MFMA_FP32_TFLOPS
         roof = {"bound": "mfma", "kernel": dom, "achieved": round(tf, 2), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(tf / peak, 4), "traffic": None,
-                "products": "split-f16 (hi/lo halves, 3 FP16 products, FP32 accumulate)" if f16 else "f32",
+                "products": {0: "split-f16 x2 (hi/lo parts, 3 FP16 partial products, FP32 accumulate, ~2^-21)",
+                             2: "split-f16 x3 (exact hi/mid/lo parts, 6 FP16 partial products, FP32 accumulate, "
+                                "~2^-32: FP32-accurate)",
+                             1: "f32"}[products],
                 "avg_launch_ms": hbm["avg_launch_ms"], "algorithmic_bytes_per_launch": hbm["algorithmic_bytes_per_launch"],
                 "hbm": {"achieved": hbm["achieved"], "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": hbm["frac"]}}
         if f16:
-            roof["frac_of_split_peak"] = round(3 * tf / peak, 4)
+            roof["frac_of_split_peak"] = round(nsplit * tf / peak, 4)
     if dom in ("mid_kz", "mid_lines"):
         # The transforms run on the vector ALUs in FP32 without packed
         # instructions: 78.6 TFLOP/s is that ceiling (half the listed
@@ -388,28 +415,69 @@ def reference_leg(ctx, cdk, prot, lig, confs, quats, fused_idx):
             "argmin_agreement": round(agree, 4)}
 
 
-def cpu_leg(g, L, nsample, threads, N, budget_s=25.0):
-    """The literal cpuNaive dense FP32 correlation on the host cores over the
-    first rotations of conformer 0 (a bounded sample of the same poses: at
-    most nsample rotations, stopping once budget_s has been spent)."""
+def host_cpu():
+    """The host the CPU baseline ran on: logical CPUs, CPUs this process may
+    use, and the model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
+def cpu_threads_default():
+    # every host core this process may use, at most the GPU box's CPU share
+    # of 16 cores per GPU
+    return max(1, min(16, host_cpu()["usable"]))
+
+
+def cpu_leg(g, L, nsample, threads, N, budget_s=20.0):
+    """The literal cpuNaive dense FP32 correlation (cpuNaive.cpp:184-222) on
+    the host cores over the first rotations of conformer 0 (a bounded sample
+    of the same poses: at most nsample rotations, stopping once budget_s has
+    been spent), poses/s extrapolated from that sample."""
     from oracle import oracle as orc
-    orc.set_threads(threads)
     prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
     ts = time.perf_counter()
     ns = 0
     for r in range(min(nsample, L["quats"].shape[1])):
         Lb, bx, wn, _ = prob.pose_box(L["conformers"][0], L["quats"][0, r])
         if (wn > 0).all() and (bx <= N).all():
-            orc.correlate_window_full(prob.Pused, Lb, wn)
+            orc.correlate_window_full_timed(prob.Pused, Lb, wn, threads)
         ns += 1
         if time.perf_counter() - ts > budget_s:
             break
     cpu_s = time.perf_counter() - ts
     cpu = {"value": round(ns / cpu_s, 4), "unit": "poses/s", "cores": int(threads), "kind": "port",
-           "sample": "first %d rotations of conformer 0, literal cpuNaive dense FP32 loop (cpuNaive.cpp:194-216) "
-                     "over the ligand box and window, %d threads over x, %.1f s" % (ns, threads, cpu_s)}
+           "host": host_cpu(), "build": orc.CPUBASE_FLAGS,
+           "sample": "extrapolated from the first %d rotations of conformer 0 (%.1f s): the literal cpuNaive dense "
+                     "FP32 loop (cpuNaive.cpp:194-216) over the ligand box and window, %d threads over x"
+                     % (ns, cpu_s, threads)}
     log("cpu baseline", cpu)
     return cpu
+
+
+def arithmetic_note(roof):
+    prod = str((roof or {}).get("products", ""))
+    if prod.startswith("split-f16 x3"):
+        return ("FP32 grids, transforms, protein products and sums; ligand-spectrum products on the FP16 matrix "
+                "cores with every FP32 operand split exactly into three FP16 parts after power-of-two scaling "
+                "(6 FP16 partial products per FP32 product, exact in FP32, FP32 accumulate: about 2^-32 per "
+                "product, below FP32 rounding)")
+    if prod.startswith("split-f16 x2"):
+        return ("FP32 grids, transforms, protein products and sums; ligand-spectrum products on the FP16 matrix "
+                "cores with hi/lo operand splits (3 FP16 partial products per product, FP32 accumulate, about "
+                "2^-21 per product: narrower than FP32)")
+    return "FP32"
 
 
 def line(args, metric, value, total, workload, extra_config, roof, kernels, cpu, **more):
@@ -425,9 +493,7 @@ def line(args, metric, value, total, workload, extra_config, roof, kernels, cpu,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "arithmetic": ("FP32 grids, transforms, protein products and sums; ligand-spectrum products on the FP16 "
-                       "matrix cores with hi/lo operand splits (3 FP16 products per FP32-accurate product, FP32 "
-                       "accumulate)") if str((roof or {}).get("products", "")).startswith("split") else "FP32",
+        "arithmetic": arithmetic_note(roof),
         "data": "synthetic (seeded pseudo-protein grids, random-walk ligands, torsion conformers, Shoemake quaternions)",
         "config": dict({"workload": workload}, **extra_config),
         "roofline": roof,
@@ -474,6 +540,9 @@ def run_single(args, cdk, workloads):
     pc = max(1, min(args.plain_conformers, args.conformers))
     ref = reference_leg(ctx, cdk, prot, lig, L["conformers"][:pc], L["quats"][:pc], idx[:pc])
     ref["speedup_fused"] = round(value / ref["value"], 2)
+    # the plain loop at its HBM roofline: B(N, C) algorithmic bytes per pose at 8 TB/s
+    ref["roofline_poses_per_s"] = round(PEAK_HBM_GBPS * 1e9 / reference_bytes(N, C), 1)
+    ref["speedup_fused_vs_roofline"] = round(value / ref["roofline_poses_per_s"], 2)
     f64 = fp64_check(args, g, L, E, idx, mc, args.fp64_sample, args.fp64_full) if args.fp64_sample > 0 else None
     cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
 
@@ -654,8 +723,10 @@ def main():
     ap.add_argument("--conformers", type=int, default=100)
     ap.add_argument("--quaternions", type=int, default=2000)
     ap.add_argument("--plain-conformers", type=int, default=10)
-    ap.add_argument("--cpu-sample", type=int, default=12, help="rotations timed on the CPU (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=200,
+                    help="at most this many rotations timed on the CPU, within 20 s (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = every usable host core, at most the box's share of 16)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="poses per fused launch set (0 = from the HBM budget)")
     ap.add_argument("--group", type=int, default=0, help="poses per mid-kernel workgroup (0 = autotuned)")
@@ -663,8 +734,9 @@ def main():
                     help="fused engine form: 0 planes, 1 columns (64^3), 2 atoms (64^3, 96^3, 128^3), -1 autotuned")
     ap.add_argument("--overlap-z", type=int, default=1,
                     help="atom form: overlap each launch set's z-inverse stage with the next set (1) or not (0)")
-    ap.add_argument("--atom-products", type=int, default=0,
-                    help="atom form products: 0 split-FP16 matrix cores, 1 FP32 matrix cores")
+    ap.add_argument("--atom-products", type=int, default=2,
+                    help="atom form products: 2 exact three-part FP16 splits (FP32-accurate), 0 two-part FP16 "
+                         "splits, 1 FP32 matrix cores")
     ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--inv", type=int, default=-1, help="inverse plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving leg (0 = skip)")
@@ -681,6 +753,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_threads <= 0:
+        args.cpu_threads = cpu_threads_default()
     import cdocker_amd as cdk
     from cdocker_amd import workloads
     {"single": run_single, "batched": run_batched, "large": run_large}[args.workload](args, cdk, workloads)

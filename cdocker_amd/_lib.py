@@ -8,7 +8,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcdk.so")
+# CDK_LIBRARY: another build of the same library (the host-sanitizer
+# variant the CPU tests can run under); default the in-tree build.
+LIB_PATH = os.environ.get("CDK_LIBRARY") or os.path.join(_HERE, "libcdk.so")
 
 OK = 0
 ERR_INVALID = 1
@@ -94,7 +96,7 @@ _SIGS = {
     "cdk_server_create": [P, I, PP, I64, I, PP],
     "cdk_server_submit": [P, I, I, FP, FP, FP, I, FP, I, FP, I, I64P],
     "cdk_server_result": [P, I64, I, FP, IP, FP, DP],
-    "cdk_server_stats": [P, DP],
+    "cdk_server_stats": [P, I, DP],
     "cdk_server_destroy": [P],
     "cdk_rotate": [P, I, I, FP, FP, FP],
     "cdk_rotate_aabb": [P, I, I, FP, FP, I, F, FP, FP, FP, IP],
@@ -104,6 +106,10 @@ _SIGS = {
     "cdk_conj_mult": [P, I, I, I64, FP, FP],
     "cdk_sum_grids": [P, I, I, I64, I64, FP, FP],
     "cdk_argmin_window": [P, I, I, FP, IP, FP, IP],
+    "cdk_rank_poses_quats": [P, I, I, FP, FP, FP, IP, FP, I, IP, DP],
+    "cdk_rotate_host": [I, I, FP, FP, FP],
+    "cdk_min_max_coors": [I, I, FP, FP, FP],
+    "cdk_quaternion_uniform_sampling_mt": [P, I, FP],
     "cdk_rng_create": [U64, PP],
     "cdk_rng_destroy": [P],
     "cdk_quaternion_uniform_sampling": [P, I, FP],

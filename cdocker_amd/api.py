@@ -468,14 +468,15 @@ class Server:
         rc = lib.cdk_server_result(self.h, C.c_int64(rid), int(wait_ms), _fp(e), _ip(idx), _fp(mc), C.byref(lat))
         if rc == _lib.ERR_PENDING:
             return None
+        self.shapes.pop(rid, None)  # the server released the request
         check(rc, "cdk_server_result")
         return e, idx, mc, lat.value
 
     def stats(self):
-        s = np.zeros(8, np.float64)
-        check(lib.cdk_server_stats(self.h, _dp(s)))
         keys = ["requests_done", "poses_done", "launch_sets", "mean_batch_poses", "busy_s", "admitted_bytes",
-                "queued", "peak_admitted_bytes"]
+                "queued", "peak_admitted_bytes", "held_requests", "sets_in_flight"]
+        s = np.zeros(len(keys), np.float64)
+        check(lib.cdk_server_stats(self.h, len(keys), _dp(s)))
         return dict(zip(keys, s.tolist()))
 
     def close(self):

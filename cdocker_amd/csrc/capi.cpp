@@ -165,7 +165,9 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t v) {
       require(v == 0 || v == 1, "overlap_z must be 0 or 1");
       c.overlap_z = (int)v;
     } else if (k == "atom_products") {
-      require(v == cdk::kAtomF16 || v == cdk::kAtomF32, "atom_products must be 0 (split FP16 matrix cores) or 1 (FP32)");
+      require(v == cdk::kAtomF16 || v == cdk::kAtomF32 || v == cdk::kAtomF16x3,
+              "atom_products must be 0 (FP16 matrix cores, two-part splits), 1 (FP32 matrix cores) or 2 (FP16 "
+              "matrix cores, exact three-part splits)");
       c.atom_products = (int)v;
     } else if (k == "pipeline") {
       require(v >= -1 && v <= 2, "pipeline must be -1 (tuned), 0 (plane form), 1 (column form) or 2 (atom form)");
@@ -212,6 +214,8 @@ int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value) {
     else if (k == "last_pipeline") *value = c.last_pipe;
     else if (k == "last_mid_poses") *value = c.last_poses;
     else if (k == "max_dyn_lds") *value = (int64_t)c.max_dyn_lds;
+    else if (k == "workspace_bytes")
+      *value = (int64_t)(c.d_rot.cap + c.d_ag.cap + c.d_A.cap + c.d_T.cap + c.d_T2b.cap);
     else throw cdk::Error(CDK_ERR_INVALID, "unknown option " + k);
   });
 }
@@ -391,6 +395,48 @@ int cdk_rank_poses(cdk_protein* p, int natom, int nquat, const float* rotated, c
   });
 }
 
+int cdk_rank_poses_quats(cdk_protein* p, int natom, int nquat, const float* conformer, const float* quats,
+                         const float* energy, const int* index, const float* mincoor, int n_lowest, int* out_rank,
+                         double* out_pose) {
+  return guard([&] {
+    require(p && conformer && quats && energy && index && mincoor && out_rank, "NULL argument");
+    require(natom > 0 && nquat >= 0 && n_lowest >= 0, "bad counts");
+    // rotate only the kept rotations (Rotate.cpp:4-27, bit-identical to the search's)
+    std::vector<int> order(nquat);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return energy[a] < energy[b]; });
+    const int n = std::min(n_lowest, nquat);
+    const cdk::Protein& P = p->p;
+    std::vector<float> rot((size_t)natom * 3);
+    for (int i = 0; i < n; ++i) {
+      const int r = order[i];
+      out_rank[i] = r;
+      if (!out_pose) continue;
+      cdk::rotate_host(1, natom, quats + (size_t)r * 4, conformer, rot.data());
+      for (int k = 0; k < 3; ++k) {
+        const double shift = P.grid_min(k) - (double)mincoor[r * 3 + k] + (double)((float)index[r * 3 + k] * P.h);
+        for (int a = 0; a < natom; ++a) out_pose[((size_t)i * natom + a) * 3 + k] = (double)rot[a * 3 + k] + shift;
+      }
+    }
+  });
+}
+
+int cdk_rotate_host(int nq, int natom, const float* quats, const float* coords, float* out) {
+  return guard([&] {
+    require(nq >= 0 && natom >= 0, "bad counts");
+    require((nq == 0 || natom == 0) || (quats && coords && out), "NULL argument");
+    cdk::rotate_host(nq, natom, quats, coords, out);
+  });
+}
+
+int cdk_min_max_coors(int nq, int natom, const float* coors, float* mincoors, float* maxcoors) {
+  return guard([&] {
+    require(nq >= 0 && natom > 0, "bad counts");
+    require(nq == 0 || coors, "NULL argument");
+    cdk::min_max_coors(nq, natom, coors, mincoors, maxcoors);
+  });
+}
+
 int cdk_rescore_poses(cdk_protein* p, const cdk_ligand* l, int npose, const double* coords, double* energy,
                       double* grad, int* n_inside) {
   return guard([&] {
@@ -552,10 +598,10 @@ int cdk_server_result(cdk_server* s, int64_t request_id, int wait_ms, float* ene
   return g != CDK_OK ? g : rc;
 }
 
-int cdk_server_stats(cdk_server* s, double* stats) {
+int cdk_server_stats(cdk_server* s, int n, double* stats) {
   return guard([&] {
-    require(s && stats, "NULL argument");
-    s->s->stats(stats);
+    require(s && (stats || n <= 0), "NULL argument");
+    s->s->stats(n, stats);
   });
 }
 
@@ -768,6 +814,13 @@ int cdk_rng_create(uint64_t seed, cdk_rng** out) {
 
 int cdk_rng_destroy(cdk_rng* r) {
   return guard([&] { delete r; });
+}
+
+int cdk_quaternion_uniform_sampling_mt(void* mt19937_64, int n, float* out) {
+  return guard([&] {
+    require(mt19937_64 && out && n >= 0, "bad arguments");
+    cdk::quaternion_uniform_sampling(*static_cast<std::mt19937_64*>(mt19937_64), n, out);
+  });
 }
 
 int cdk_quaternion_uniform_sampling(cdk_rng* r, int n, float* out) {

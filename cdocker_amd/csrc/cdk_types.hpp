@@ -28,6 +28,10 @@ struct LigandDesc {
   // whose matrix-core steps take 8 entries at a time.
   int pad_off;
   int pad_start[kMaxCh + 1];
+  // Power of two that brings the largest atom weight (|sqrt(eps)| or |q|) of
+  // the ligand to [2^12, 2^13), and its inverse; the atom form's FP16 split
+  // products use it so no operand part underflows.
+  float ascale, aunscale;
 };
 
 // One pose = one (ligand, conformer, rotation) correlation.

@@ -237,10 +237,19 @@ std::unique_ptr<Ligand> make_ligand(const Protein& p, int natom, const float* ch
   d.pad_start[nch] = (int)L->pad_atoms.size();
   L->wv.resize(natom);
   L->wq.resize(natom);
+  float wmax = 0.f;
   for (int a = 0; a < natom; ++a) {
     L->wv[a] = std::sqrt(eps[a]);  // float overload, as FillLigandGrid.cpp:45
     L->wq[a] = charge[a];
+    if (!std::isfinite(L->wv[a]) || !std::isfinite(L->wq[a])) throw Error(kErrInvalid, "non-finite atom weight");
+    wmax = std::max({wmax, std::abs(L->wv[a]), std::abs(L->wq[a])});
   }
+  // wmax * 2^e in [2^12, 2^13): e = 13 - exponent of wmax (frexp: wmax = m 2^x, m in [0.5, 1))
+  int ex = 0;
+  if (wmax > 0.f) (void)std::frexp(wmax, &ex);
+  const int e = wmax > 0.f ? std::max(-100, std::min(100, 13 - ex)) : 0;
+  d.ascale = std::ldexp(1.f, e);
+  d.aunscale = std::ldexp(1.f, -e);
   return L;
 }
 
@@ -312,6 +321,7 @@ void build_tables(const Protein& prot, const std::vector<Job>& jobs, Prepared& t
   LigandDesc dummy;
   std::memset(&dummy, 0, sizeof(dummy));
   dummy.nch = 1;
+  dummy.ascale = dummy.aunscale = 1.f;
   t.dummy_lig = (int)t.ligs.size();
   t.ligs.push_back(dummy);
   if (t.chan.empty()) t.chan.push_back(0);
@@ -469,11 +479,16 @@ void enqueue_reference(Context& ctx, const Protein& prot, Prepared& t, EventTime
   }
 }
 
+// Poses per launch set: the fused working arrays of a set (ligand or product
+// spectra A, correlation T, the second T2 buffer of the overlapped atom form
+// when that form may run, rotated atoms and placements) within ws_budget.
 size_t fused_chunk(Context& ctx, const Protein& prot, const Prepared& t) {
   const int N = prot.N, ZH = N / 2 + 1;
   const size_t plane = (size_t)N * ZH;
+  const bool overlap_possible = ctx.overlap_z && !ctx.time_stages && atoms_supported(N) &&
+                                prot.spec_mf.p != nullptr && (ctx.pipeline < 0 || ctx.pipeline == kPipeAtoms);
   const size_t per_slot = std::max((size_t)t.cmax * t.boxmax * plane * sizeof(float2), atoms_work_bytes(N)) +
-                          (size_t)N * plane * sizeof(float2) +
+                          (size_t)N * plane * sizeof(float2) * (overlap_possible ? 2 : 1) +
                           (size_t)t.astride * (3 * sizeof(float) + sizeof(AtomGrid));
   size_t chunk = ctx.fused_chunk > 0 ? (size_t)ctx.fused_chunk : std::max<size_t>(1, ctx.ws_budget / per_slot);
   return std::min(chunk, t.slots.size());
@@ -891,16 +906,20 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
 
 }  // namespace
 
-Prepared::~Prepared() { drop_graph(); }
+Prepared::~Prepared() {
+  drop_graph();
+  if (h_keys) (void)hipHostFree(h_keys);
+  if (h_meta) (void)hipHostFree(h_meta);
+  if (start) (void)hipEventDestroy(start);
+  if (done) (void)hipEventDestroy(done);
+}
 
 void Prepared::drop_graph() {
   if (graph) (void)hipGraphExecDestroy(graph);
   graph = nullptr;
 }
 
-void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Prepared& t) {
-  std::lock_guard<std::mutex> lock(ctx.mu);
-  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+void build_search(const Protein& prot, const std::vector<Job>& jobs, Prepared& t) {
   t.drop_graph();
   t.ligs.clear();
   t.chan.clear();
@@ -918,6 +937,12 @@ void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& j
   build_tables(prot, jobs, t);
   for (const Job& j : jobs) t.job_n.push_back(j.npose());
   t.prot = &prot;
+}
+
+namespace {
+
+// t's tables to the device on the context stream (context lock held).
+void upload_tables(Context& ctx, Prepared& t) {
   upload(ctx, t.d_ligs, t.ligs);
   upload(ctx, t.d_chan, t.chan);
   upload(ctx, t.d_pad, t.pad);
@@ -928,17 +953,26 @@ void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& j
   const size_t total = std::max<size_t>(t.slots.size(), 1);
   t.d_keys.ensure(total * sizeof(unsigned long long));
   t.d_meta.ensure(total * sizeof(RotMeta));
+}
+
+}  // namespace
+
+void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Prepared& t) {
+  build_search(prot, jobs, t);
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  upload_tables(ctx, t);
   hip_check(hipStreamSynchronize(ctx.stream), "prepare sync");
 }
 
-void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
-  std::lock_guard<std::mutex> lock(ctx.mu);
-  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
-  if (!t.prot) throw Error(kErrInvalid, "search not prepared");
+namespace {
+
+// Enqueues t's launch sequence and the copies of its keys and per-pose
+// metadata to host staging (t's own when t.own_staging, else the context's);
+// context lock held.
+void enqueue_search(Context& ctx, Prepared& t, Engine engine, bool use_graph, EventTimer& timer) {
   const Protein& prot = *t.prot;
   const size_t total = t.slots.size();
-  if (total == 0) return;
-  EventTimer timer(ctx);
   if (engine == kEngineReference) {
     enqueue_reference(ctx, prot, t, timer);
   } else {
@@ -989,17 +1023,35 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
       enqueue_fused(ctx, prot, t, chunk, tu, timer);
     }
   }
-  ctx.ensure_host(total);
-  hip_check(hipMemcpyAsync(ctx.h_keys, t.d_keys.p, total * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           ctx.stream),
+  void* hk = ctx.h_keys;
+  void* hm = ctx.h_meta;
+  if (t.own_staging) {
+    if (total > t.h_cap) {
+      if (t.h_keys) (void)hipHostFree(t.h_keys);
+      if (t.h_meta) (void)hipHostFree(t.h_meta);
+      t.h_keys = t.h_meta = nullptr;
+      t.h_cap = 0;
+      hip_check(hipHostMalloc(&t.h_keys, total * sizeof(unsigned long long)), "hipHostMalloc");
+      hip_check(hipHostMalloc(&t.h_meta, total * sizeof(RotMeta)), "hipHostMalloc");
+      t.h_cap = total;
+    }
+    hk = t.h_keys;
+    hm = t.h_meta;
+  } else {
+    ctx.ensure_host(total);
+    hk = ctx.h_keys;
+    hm = ctx.h_meta;
+  }
+  hip_check(hipMemcpyAsync(hk, t.d_keys.p, total * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx.stream),
             "keys D2H");
-  hip_check(hipMemcpyAsync(ctx.h_meta, t.d_meta.p, total * sizeof(RotMeta), hipMemcpyDeviceToHost, ctx.stream),
-            "meta D2H");
-  hip_check(hipStreamSynchronize(ctx.stream), "search sync");
-  timer.collect();
-  // Decode while the pinned staging is ours (context lock held).
-  const unsigned long long* keys = static_cast<const unsigned long long*>(ctx.h_keys);
-  const RotMeta* meta = static_cast<const RotMeta*>(ctx.h_meta);
+  hip_check(hipMemcpyAsync(hm, t.d_meta.p, total * sizeof(RotMeta), hipMemcpyDeviceToHost, ctx.stream), "meta D2H");
+}
+
+// Per-pose results of t from host staging (keys, metadata).
+void decode_results(Prepared& t, const void* hkeys, const void* hmeta) {
+  const Protein& prot = *t.prot;
+  const unsigned long long* keys = static_cast<const unsigned long long*>(hkeys);
+  const RotMeta* meta = static_cast<const RotMeta*>(hmeta);
   const int N = prot.N;
   t.results.assign(t.job_first.size(), PoseResults());
   for (size_t j = 0; j < t.job_first.size(); ++j) {
@@ -1031,6 +1083,61 @@ void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
       }
     }
   }
+}
+
+}  // namespace
+
+void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  if (!t.prot) throw Error(kErrInvalid, "search not prepared");
+  t.results.assign(t.job_first.size(), PoseResults());
+  if (t.slots.empty()) return;
+  EventTimer timer(ctx);
+  t.own_staging = false;
+  enqueue_search(ctx, t, engine, use_graph, timer);
+  hip_check(hipStreamSynchronize(ctx.stream), "search sync");
+  timer.collect();
+  // Decode while the pinned staging is ours (context lock held).
+  decode_results(t, ctx.h_keys, ctx.h_meta);
+}
+
+void submit_search(Context& ctx, Prepared& t) {
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  if (!t.prot) throw Error(kErrInvalid, "search not built");
+  if (!t.start) hip_check(hipEventCreate(&t.start), "hipEventCreate");
+  if (!t.done) hip_check(hipEventCreate(&t.done), "hipEventCreate");
+  t.own_staging = true;
+  upload_tables(ctx, t);
+  hip_check(hipEventRecord(t.start, ctx.stream), "hipEventRecord");
+  if (!t.slots.empty()) {
+    EventTimer timer(ctx);  // stages are timed only by the synchronous path
+    const bool ts = ctx.time_stages;
+    ctx.time_stages = false;
+    try {
+      enqueue_search(ctx, t, kEngineFused, false, timer);
+    } catch (...) {
+      ctx.time_stages = ts;
+      throw;
+    }
+    ctx.time_stages = ts;
+  }
+  hip_check(hipEventRecord(t.done, ctx.stream), "hipEventRecord");
+}
+
+void finish_search(Context& ctx, Prepared& t) {
+  (void)ctx;
+  if (!t.done) throw Error(kErrInvalid, "search not submitted");
+  hip_check(hipEventSynchronize(t.done), "search wait");
+  t.results.assign(t.job_first.size(), PoseResults());
+  if (!t.slots.empty()) decode_results(t, t.h_keys, t.h_meta);
+}
+
+float search_gpu_ms(Prepared& t) {
+  float ms = 0.f;
+  if (t.start && t.done) hip_check(hipEventElapsedTime(&ms, t.start, t.done), "hipEventElapsedTime");
+  return ms;
 }
 
 void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,

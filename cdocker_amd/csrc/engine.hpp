@@ -144,6 +144,15 @@ struct Prepared {
   int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0, graph_pipe = -1,
       graph_products = -1, graph_overlap = -1;
   std::vector<PoseResults> results;
+  // Pipelined callers (the server): results are copied into this table's own
+  // pinned staging, and `done` marks the end of its launch sequence on the
+  // context stream (`start` its beginning), so the host can build the next
+  // table while this one runs.
+  bool own_staging = false;
+  void* h_keys = nullptr;
+  void* h_meta = nullptr;
+  size_t h_cap = 0;
+  hipEvent_t start = nullptr, done = nullptr;
   Prepared() = default;
   Prepared(const Prepared&) = delete;
   Prepared& operator=(const Prepared&) = delete;
@@ -170,7 +179,7 @@ struct Context {
   int mid_poses = 0;              // poses sharing each protein load in the mid stage with a fixed group; 0 = 1
   int pipeline = -1;              // fused pipeline form (kPipePlanes, kPipeColumns, kPipeAtoms); -1 = tuned
   int overlap_z = 1;              // atom form: overlap each launch set's z-inverse stage with the next set
-  int atom_products = 0;          // atom form products: kAtomF16 (split FP16 matrix cores) or kAtomF32
+  int atom_products = 2;          // atom form products: kAtomF16x3 (exact 3-part FP16 splits), kAtomF16, kAtomF32
   int fwd_variant = -1;           // forward plane stage form; -1 = tuned
   int inv_variant = -1;           // inverse plane stage form; -1 = tuned
   bool use_graph = true;
@@ -220,6 +229,17 @@ void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& j
 // Runs every prepared pose; results land in t.results (one entry per job).
 // With use_graph the fused launch sequence is captured once and replayed.
 void execute_search(Context& ctx, Prepared& t, Engine engine, bool use_graph);
+// The same in pipelined pieces: build_search fills t's host tables (no
+// device work, no lock); submit_search uploads them and enqueues the fused
+// launch sequence and the result copies on the context stream without
+// waiting (t.own_staging is set; t's host tables must stay unchanged until
+// finish_search); finish_search waits for t's sequence and decodes
+// t.results.  Successive submissions run in stream order.
+void build_search(const Protein& prot, const std::vector<Job>& jobs, Prepared& t);
+void submit_search(Context& ctx, Prepared& t);
+void finish_search(Context& ctx, Prepared& t);
+// GPU milliseconds of t's last submitted sequence (after finish_search).
+float search_gpu_ms(Prepared& t);
 // prepare + execute without a graph.
 void run_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Engine engine,
                 std::vector<PoseResults>& out);

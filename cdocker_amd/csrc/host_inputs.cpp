@@ -73,6 +73,43 @@ void quaternion_uniform_sampling(std::mt19937_64& gen, int n, float* out) {
   }
 }
 
+void rotate_host(int nq, int natom, const float* quats, const float* coords, float* out) {
+  // q (x) (0, p) (x) conj(q), Hamilton products with QuaternionMultiply.cpp:7-10's
+  // term order, the zero terms kept (they decide signed zeros)
+  for (int i = 0; i < nq; ++i) {
+    const float* q = quats + (size_t)i * 4;
+    const float c0 = q[0], c1 = -q[1], c2 = -q[2], c3 = -q[3];
+    for (int a = 0; a < natom; ++a) {
+      const float px = coords[a * 3], py = coords[a * 3 + 1], pz = coords[a * 3 + 2];
+      const float zero = 0.0f;
+      const float t0 = q[0] * zero - q[1] * px - q[2] * py - q[3] * pz;
+      const float t1 = q[0] * px + q[1] * zero + q[2] * pz - q[3] * py;
+      const float t2 = q[0] * py - q[1] * pz + q[2] * zero + q[3] * px;
+      const float t3 = q[0] * pz + q[1] * py - q[2] * px + q[3] * zero;
+      float* o = out + ((size_t)i * natom + a) * 3;
+      o[0] = t0 * c1 + t1 * c0 + t2 * c3 - t3 * c2;
+      o[1] = t0 * c2 - t1 * c3 + t2 * c0 + t3 * c1;
+      o[2] = t0 * c3 + t1 * c2 - t2 * c1 + t3 * c0;
+    }
+  }
+}
+
+void min_max_coors(int nq, int natom, const float* coors, float* mincoors, float* maxcoors) {
+  for (int i = 0; i < nq; ++i) {
+    const float* c = coors + (size_t)i * natom * 3;
+    for (int k = 0; k < 3; ++k) {
+      float mn = c[k], mx = c[k];
+      for (int a = 1; a < natom; ++a) {
+        const float v = c[a * 3 + k];
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+      }
+      if (mincoors) mincoors[i * 3 + k] = mn;
+      if (maxcoors) maxcoors[i * 3 + k] = mx;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Pseudo-protein potential grids.
 //   LJ channel p at grid point g:  sum_i sqrt(eps_i) [(R/d)^12 - 2 (R/d)^6],

@@ -18,6 +18,13 @@ struct Xoshiro {
 extern const float kProbeRadii[8];
 
 void quaternion_uniform_sampling(std::mt19937_64& gen, int n, float* out);
+// Rotate.cpp:4-27 on the host, the same FP32 operations in the same order as
+// the device's (contraction off): out [nq][natom][3].
+void rotate_host(int nq, int natom, const float* quats, const float* coords, float* out);
+// GetMinCoors.cpp:4-29 / GetMaxCoors.cpp:3-27 over rotated coordinates
+// [nq][natom][3]: the first atom's value, replaced only by a strictly
+// smaller (larger) one.  Either output may be null.
+void min_max_coors(int nq, int natom, const float* coors, float* mincoors, float* maxcoors);
 // The atoms behind synth_protein: positions uniform in [0, box)^3, charge
 // N(0, 0.3), radius from the probe set, epsilon uniform [0.05, 0.25].
 void synth_protein_atoms(uint64_t seed, int natom, float box, double* xyz, double* charge, double* epsilon,

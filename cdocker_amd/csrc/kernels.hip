@@ -1807,8 +1807,9 @@ __device__ __forceinline__ unsigned atom_tw_bytes(unsigned k, unsigned i);
 // weight times Z_a(kz) = (1 - fz) W^(kz iz) + fz W^(kz (iz + 1)), and ei =
 // (ix, iy, fy bits, 0), ix and iy times 8 (table byte offsets) at grid edges
 // 64 and 128; a pad as zeros (zero weight: every product it enters is exactly 0).
-// ag: the pose's placements.
-template <int N>
+// ag: the pose's placements.  SC (the FP16 split products): the weight is
+// scaled by Ld.ascale and fy by kAtomYScale (powers of two, exact).
+template <int N, bool SC>
 __device__ void stage_atoms(float4* ef, int4* ei, const LigandDesc& Ld, int t_lo, int t_hi,
                             const int* __restrict__ pad_atoms, const float* __restrict__ wv,
                             const float* __restrict__ wq, const AtomGrid* __restrict__ ag, int kz,
@@ -1823,14 +1824,14 @@ __device__ void stage_atoms(float4* ef, int4* ei, const LigandDesc& Ld, int t_lo
     if (a >= 0) {
       const AtomGrid g = ag[a];
       // elec channel: the charge; vdW channels: sqrt(eps) (FillLigandGrid.cpp:45, :97)
-      const float wa = t >= t_elec ? wq[w_off + a] : wv[w_off + a];
+      const float wa = (t >= t_elec ? wq[w_off + a] : wv[w_off + a]) * (SC ? Ld.ascale : 1.f);
       const float2 z0 = stw[(kz * g.iz) % N], z1 = stw[(kz * (g.iz + 1)) % N];
       const float2 zf = c_fma_s(z1, (float)g.fz, c_scale(z0, (float)(1.0 - g.fz)));
       const float2 wz = make_float2(wa * zf.x, wa * zf.y);
       const float fx = (float)g.fx;
       constexpr int S = (N & (N - 1)) == 0 ? 8 : 1;  // byte offsets for a power-of-two N
       f = make_float4(wz.x, wz.y, fx * wz.x, fx * wz.y);
-      q = make_int4(S * g.ix, S * g.iy, __float_as_int((float)g.fy), 0);
+      q = make_int4(S * g.ix, S * g.iy, __float_as_int((float)g.fy * (SC ? kAtomYScale : 1.f)), 0);
     }
     ef[k] = f;
     ei[k] = q;
@@ -1842,7 +1843,7 @@ __device__ void stage_atoms(float4* ef, int4* ei, const LigandDesc& Ld, int t_lo
 // channel at a time (a channel pads to at most kMaxAtoms entries), and runs
 // body(c, k0, n) per channel with its n entries at staged slots [k0, k0 + n)
 // and zero entries after them to the next multiple of 8.
-template <int N, class Body>
+template <int N, bool SC, class Body>
 __device__ __forceinline__ void for_staged_channels(float4* ef, int4* ei, int ent, const LigandDesc& Ld,
                                                     const int* __restrict__ pad_atoms,
                                                     const float* __restrict__ wv, const float* __restrict__ wq,
@@ -1852,11 +1853,11 @@ __device__ __forceinline__ void for_staged_channels(float4* ef, int4* ei, int en
   const int nch = uni(Ld.nch);
   const int ptot = uni(Ld.pad_start[nch]);
   const bool whole = ptot <= ent;
-  if (whole) stage_atoms<N>(ef, ei, Ld, 0, ptot, pad_atoms, wv, wq, ag, kz, stw);
+  if (whole) stage_atoms<N, SC>(ef, ei, Ld, 0, ptot, pad_atoms, wv, wq, ag, kz, stw);
   for (int c = 0; c < nch; ++c) {
     const int n = uni(Ld.ch_start[c + 1]) - uni(Ld.ch_start[c]);
     const int p0 = uni(Ld.pad_start[c]);
-    if (!whole) stage_atoms<N>(ef, ei, Ld, p0, uni(Ld.pad_start[c + 1]), pad_atoms, wv, wq, ag, kz, stw);
+    if (!whole) stage_atoms<N, SC>(ef, ei, Ld, p0, uni(Ld.pad_start[c + 1]), pad_atoms, wv, wq, ag, kz, stw);
     body(c, whole ? p0 : 0, n);
   }
 }
@@ -1870,6 +1871,22 @@ __device__ __forceinline__ void split_f16(float a, float b, unsigned& hi, unsign
   asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(ra) : "v"(hi), "v"(a));                // a - hi.lo
   asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(rb) : "v"(hi), "v"(b));  // b - hi.hi
   lo = __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){ra, rb}, half2v));
+}
+
+// x = hi + mid + lo exactly (FP32 x of magnitude in [2^-1, 2^15); below
+// that lo rounds to the FP16 subnormal spacing 2^-24): hi = f16(x), mid =
+// f16(x - hi), lo = f16(x - hi - mid), both residuals exact in FP32 (x - hi
+// has at most 13 significant bits, x - hi - mid at most 2), each from one
+// mixed-precision fma per value; two values per register.
+__device__ __forceinline__ void split3_f16(float a, float b, unsigned& hi, unsigned& mid, unsigned& lo) {
+  hi = __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){a, b}, half2v));
+  float ra, rb, sa, sb;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(ra) : "v"(hi), "v"(a));                // a - hi.lo
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(rb) : "v"(hi), "v"(b));  // b - hi.hi
+  mid = __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){ra, rb}, half2v));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(sa) : "v"(mid), "v"(ra));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(sb) : "v"(mid), "v"(rb));
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector((float2v){sa, sb}, half2v));
 }
 
 // (x, y) -> (-y, x) on a packed FP16 pair (an exact multiply by 1).
@@ -1887,7 +1904,7 @@ __device__ __forceinline__ unsigned swap_neg_f16(unsigned h) {
 // step suffices (a buffer is rewritten only after every wavefront passed the
 // barrier that follows its last reads).
 struct AShare {
-  uint2* buf;  // [2][4][64]
+  uint4* buf;  // [2][4][64]: (hi, lo) or (hi, mid, lo)
   int wave;    // this wavefront's index among the SW
   int par;     // buffer of the next step
 };
@@ -1901,17 +1918,24 @@ struct AShare {
 // sums, X_a(k) = W^(k ix) (1 - fx D(k)), Y_a(k) = W^(k iy) (1 - fy D(k)),
 // D = 1 - W^k (dx, dy: this lane's D at kxA, kyB).
 //
-// F16: the products on the FP16 matrix cores (32 x 32 x 16: eight atoms per
-// instruction, lane half h builds both parts of atoms 4h..4h+3 of each
-// eight), every operand split x = hi + lo into two FP16 values and A B =
-// Ah Bh + Ah Bl + Al Bh accumulated in FP32; the dropped Al Bl and the
-// splits leave a relative error of about 2^-21 per product.  Slots past
-// kend are the staged zero entries.  !F16: FP32 matrix cores (32 x 32 x 2: one atom
-// per instruction, lane half = part), AU atoms' operands in flight at once.
-template <int N, bool F16, int SW = 1>
+// PR = kAtomF16 / kAtomF16x3: the products on the FP16 matrix cores (32 x 32
+// x 16: eight atoms per instruction, lane half h builds both parts of atoms
+// 4h..4h+3 of each eight).  The staged weights and fy carry the power-of-two
+// scales (stage_atoms), so A is scaled by ascale and B by kAtomYScale.
+// kAtomF16: every operand split x = hi + lo and A B = Ah Bh + Ah Bl + Al Bh
+// accumulated in FP32 (the dropped Al Bl and the splits leave about 2^-21 per
+// product).  kAtomF16x3: x = hi + mid + lo exactly and the six partial
+// products down to 2^-22 (Ah Bh, Ah Bm, Am Bh, Ah Bl, Al Bh, Am Bm; the
+// dropped ones are below 2^-32), each FP16 product exact in FP32.  Slots past
+// kend are the staged zero entries.  kAtomF32: FP32 matrix cores (32 x 32 x
+// 2: one atom per instruction, lane half = part), AU atoms' operands in
+// flight at once, no scales.
+template <int N, int PR, int SW = 1>
 __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, const char* stwb, int k0, int kend,
                                              int kxA, int kyB, float2 dx, float2 dy, int part, mfma_acc16& cr,
                                              mfma_acc16& ci, AShare* share = nullptr) {
+  constexpr bool F16 = PR != kAtomF32;
+  constexpr bool X3 = PR == kAtomF16x3;
   auto lookups = [&](int k, float4& f, float2& tx, float2& ty, float& fy) {
     f = ef[k];
     const int4 ii = ei[k];
@@ -1921,25 +1945,31 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
   };
   if constexpr (F16) {
     // (a0, a1) = conj(W^(kx ix) g), g = wZ - fx wZ D, k slots (real, imaginary)
-    auto a_op = [&](int kk, unsigned& hi, unsigned& lo) {
+    auto a_vals = [&](int kk, float& a0, float& a1) {
       const float4 f = ef[kk];
       const float2 tx = *reinterpret_cast<const float2*>(stwb + atom_tw_bytes<N>(kxA, ei[kk].x));
       const float gx = fmaf(-f.z, dx.x, fmaf(f.w, dx.y, f.x));
       const float gy = fmaf(-f.z, dx.y, fmaf(-f.w, dx.x, f.y));
-      const float a0 = fmaf(tx.x, gx, -(tx.y * gy));
-      const float a1 = fmaf(-tx.x, gy, -(tx.y * gx));
-      split_f16(a0, a1, hi, lo);
+      a0 = fmaf(tx.x, gx, -(tx.y * gy));
+      a1 = fmaf(-tx.x, gy, -(tx.y * gx));
     };
     auto step = [&](int k, auto first) {
-      unsigned ah[4], al[4], brh[4], brl[4], bih[4], bil[4];
+      // parts: [0] hi, [1] lo (kAtomF16) or [1] mid, [2] lo (kAtomF16x3)
+      constexpr int NP = X3 ? 3 : 2;
+      unsigned a[NP][4], b[NP][4];
       if constexpr (SW > 1) {
-        uint2* sb = share->buf + share->par * (4 * 64);
+        uint4* sb = share->buf + share->par * (4 * 64);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (j % SW == share->wave) {
-            unsigned hi, lo;
-            a_op(k + 4 * part + j, hi, lo);
-            sb[j * 64 + (threadIdx.x & 63)] = make_uint2(hi, lo);
+            float a0, a1;
+            a_vals(k + 4 * part + j, a0, a1);
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if constexpr (X3)
+              split3_f16(a0, a1, v.x, v.y, v.z);
+            else
+              split_f16(a0, a1, v.x, v.y);
+            sb[j * 64 + (threadIdx.x & 63)] = v;
           }
       }
 #pragma unroll
@@ -1961,42 +1991,53 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
           ty = *reinterpret_cast<const float2*>(stwb + atom_tw_bytes<N>(kyB, ii.y));
           fy = __int_as_float(ii.z);
         }
-        const float hx = fmaf(-fy, dy.x, 1.f), hy = -fy * dy.y;  // 1 - fy D
+        // kAtomYScale (1 - fy D), fy staged times kAtomYScale
+        const float hx = fmaf(-fy, dy.x, kAtomYScale), hy = -fy * dy.y;
         const float yx = fmaf(ty.x, hx, -(ty.y * hy)), yy = fmaf(ty.x, hy, ty.y * hx);
-        if constexpr (SW == 1) split_f16(a0, a1, ah[j], al[j]);
-        split_f16(yx, yy, brh[j], brl[j]);  // real sum: a0 yx + a1 yy
-        bih[j] = swap_neg_f16(brh[j]);      // imaginary sum: -a0 yy + a1 yx
-        bil[j] = swap_neg_f16(brl[j]);
+        if constexpr (X3) {
+          if constexpr (SW == 1) split3_f16(a0, a1, a[0][j], a[1][j], a[2][j]);
+          split3_f16(yx, yy, b[0][j], b[1][j], b[2][j]);  // real sum: a0 yx + a1 yy
+        } else {
+          if constexpr (SW == 1) split_f16(a0, a1, a[0][j], a[1][j]);
+          split_f16(yx, yy, b[0][j], b[1][j]);
+        }
       }
       if constexpr (SW > 1) {
         __syncthreads();
-        const uint2* sb = share->buf + share->par * (4 * 64);
+        const uint4* sb = share->buf + share->par * (4 * 64);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const uint2 v = sb[j * 64 + (threadIdx.x & 63)];
-          ah[j] = v.x;
-          al[j] = v.y;
+          const uint4 v = sb[j * 64 + (threadIdx.x & 63)];
+          a[0][j] = v.x;
+          a[1][j] = v.y;
+          if constexpr (X3) a[NP - 1][j] = v.z;
         }
         share->par ^= 1;
       }
-      const mfma_h8 Ah = __builtin_bit_cast(mfma_h8, (mfma_u4){ah[0], ah[1], ah[2], ah[3]});
-      const mfma_h8 Al = __builtin_bit_cast(mfma_h8, (mfma_u4){al[0], al[1], al[2], al[3]});
-      const mfma_h8 Brh = __builtin_bit_cast(mfma_h8, (mfma_u4){brh[0], brh[1], brh[2], brh[3]});
-      const mfma_h8 Brl = __builtin_bit_cast(mfma_h8, (mfma_u4){brl[0], brl[1], brl[2], brl[3]});
-      const mfma_h8 Bih = __builtin_bit_cast(mfma_h8, (mfma_u4){bih[0], bih[1], bih[2], bih[3]});
-      const mfma_h8 Bil = __builtin_bit_cast(mfma_h8, (mfma_u4){bil[0], bil[1], bil[2], bil[3]});
-      if constexpr (decltype(first)::value) {
-        const mfma_acc16 zero = {};
-        cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brh, zero, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bih, zero, 0, 0, 0);
-      } else {
-        cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brh, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bih, ci, 0, 0, 0);
-      }
-      cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Brl, cr, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bil, ci, 0, 0, 0);
-      cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Brh, cr, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bih, ci, 0, 0, 0);
+      auto op = [](const unsigned (&r)[4]) { return __builtin_bit_cast(mfma_h8, (mfma_u4){r[0], r[1], r[2], r[3]}); };
+      // (i, j) part pairs of the kept partial products, largest first
+      auto run = [&](mfma_acc16& acc, const unsigned (&bb)[NP][4]) {
+        if constexpr (decltype(first)::value) {
+          const mfma_acc16 zero = {};
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(op(a[0]), op(bb[0]), zero, 0, 0, 0);
+        } else {
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(op(a[0]), op(bb[0]), acc, 0, 0, 0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(op(a[0]), op(bb[1]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(op(a[1]), op(bb[0]), acc, 0, 0, 0);
+        if constexpr (X3) {
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(op(a[0]), op(bb[2]), acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(op(a[2]), op(bb[0]), acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(op(a[1]), op(bb[1]), acc, 0, 0, 0);
+        }
+      };
+      run(cr, b);
+      // imaginary sum: -a0 yy + a1 yx, B parts swapped to (-yy, yx)
+#pragma unroll
+      for (int i = 0; i < NP; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[i][j] = swap_neg_f16(b[i][j]);
+      run(ci, b);
     };
     // At 96 and 128 the first step starts from a zero accumulator operand (no
     // zeroing moves); the fused 64-point kernel, at its register limit, runs
@@ -2081,7 +2122,7 @@ __device__ __forceinline__ unsigned atom_tw_bytes<128>(unsigned k, unsigned i) {
 // col_inverse finishes.  Per pose the arithmetic order is fixed (atoms in
 // channel order, one accumulation chain per channel), so results do not
 // depend on the group size or on what else shares the launch.
-template <class P, bool F16>
+template <class P, int PR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_mid_atoms(const PoseSlot* __restrict__ slots,
                                                       const LigandDesc* __restrict__ ligs,
                                                       const int* __restrict__ pad_atoms,
@@ -2137,9 +2178,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     float2 E[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
-    for_staged_channels<N>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw, [&](int c, int k0, int n) {
+    for_staged_channels<N, PR != kAtomF32>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw,
+                                           [&](int c, int k0, int n) {
       mfma_acc16 cr, ci;
-      atom_channel<N, F16>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci);
+      atom_channel<N, PR>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci);
       // sum += conj(L_c) P_c
       const int pc = uni(Ld.pmap[c]);
       const float2* pb = P3 + (((size_t)pc * ZH + kz) * 4 + wave) * (8 * 128) + lane * 2;
@@ -2176,8 +2218,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (int j = 0; j < V; ++j) acc[p][j] = pl[col_at_k<N>(rb, j)];
       }
     }
-    col_inverse<N, N, true>(acc, pl, tr, twr, lane, wave, T2 + ((size_t)s * ZH + kz) * (size_t)(N * N), wx, wy,
-                            invG);
+    // the split products' scales out with 1/G (powers of two: exact)
+    const float sc = PR == kAtomF32 ? invG : invG * __int_as_float(uni(__float_as_int(Ld.aunscale))) * (1.f / kAtomYScale);
+    col_inverse<N, N, true>(acc, pl, tr, twr, lane, wave, T2 + ((size_t)s * ZH + kz) * (size_t)(N * N), wx, wy, sc);
   }
 }
 
@@ -2205,7 +2248,7 @@ __device__ __forceinline__ unsigned atom_tw_bytes<96>(unsigned k, unsigned i) {
   return (t - 96u * q) * 8u;
 }
 
-template <class P, bool F16>
+template <class P, int PR>
 __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ slots,
                                                      const LigandDesc* __restrict__ ligs,
                                                      const int* __restrict__ pad_atoms,
@@ -2222,7 +2265,7 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
   __shared__ float2 stw[N];
   __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
   __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0; times 8 at 128)
-  __shared__ uint2 abuf[2 * 4 * 64];  // the shared A operands (the row's wavefronts share kx)
+  __shared__ uint4 abuf[2 * 4 * 64];  // the shared A operands (the row's wavefronts share kx)
   const int w = blockIdx.x;
   const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
   const int rest = (w >> 3) / KZX;
@@ -2249,9 +2292,10 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
     float2 E[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) E[r] = make_float2(0.f, 0.f);
-    for_staged_channels<N>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw, [&](int c, int k0, int n) {
+    for_staged_channels<N, PR != kAtomF32>(ef, ei, ENT, Ld, pad_atoms, wv, wq, ag + (size_t)s * astride, kz, stw,
+                                           [&](int c, int k0, int n) {
       mfma_acc16 cr, ci;
-      atom_channel<N, F16, F16 ? NB : 1>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci, &share);
+      atom_channel<N, PR, PR != kAtomF32 ? NB : 1>(ef, ei, stwb, k0, k0 + n, kxA, kyB, dx, dy, part, cr, ci, &share);
       // sum += conj(L_c) P_c
       const int pc = uni(Ld.pmap[c]);
       const float2* pb = P3 + (((size_t)pc * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
@@ -2269,6 +2313,12 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
         }
       }
     });
+    if constexpr (PR != kAtomF32) {
+      // the split products' scales out (powers of two: exact)
+      const float un = __int_as_float(uni(__float_as_int(Ld.aunscale))) * (1.f / kAtomYScale);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) E[r] = make_float2(E[r].x * un, E[r].y * un);
+    }
     float2* eb = Eout + (((size_t)s * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
@@ -2745,20 +2795,25 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
                             const RotMeta* meta, int nslots, int group, const float2* P3, float2* work, float2* T2,
                             float invG, const float2* tw, int products) {
   if (nslots <= 0) return hipSuccess;
-  if (!atoms_supported(N) || group <= 0 || (products != kAtomF16 && products != kAtomF32))
+  if (!atoms_supported(N) || group <= 0 || (products != kAtomF16 && products != kAtomF32 && products != kAtomF16x3))
     return hipErrorInvalidValue;
   const int kzx = (N / 2 + 1 + 7) / 8;
   const dim3 grid((unsigned)(8 * kzx) * ceil_div(nslots, group));
-  const bool f16 = products == kAtomF16;
   CDK_DISPATCH_N(N, {
     if constexpr (PL::N == 64)
-      hipLaunchKernelGGL((f16 ? k_mid_atoms<PL, true> : k_mid_atoms<PL, false>), grid, dim3(kBlock), 0, st, slots, ligs,
-                         pad_atoms, wv, wq, ag, astride, meta, nslots, group, P3, T2, invG, tw);
+      hipLaunchKernelGGL((products == kAtomF16x3 ? k_mid_atoms<PL, kAtomF16x3>
+                          : products == kAtomF16 ? k_mid_atoms<PL, kAtomF16>
+                                                 : k_mid_atoms<PL, kAtomF32>),
+                         grid, dim3(kBlock), 0, st, slots, ligs, pad_atoms, wv, wq, ag, astride, meta, nslots, group,
+                         P3, T2, invG, tw);
     if constexpr (PL::N == 96 || PL::N == 128) {
       constexpr int NB = PL::N / 32, NLW = PL::N / RegShape<PL::N>::LPW;
       const dim3 gf((unsigned)(8 * kzx * NB) * ceil_div(nslots, group));
-      hipLaunchKernelGGL((f16 ? k_atoms_fwd<PL, true> : k_atoms_fwd<PL, false>), gf, dim3(64 * NB), 0, st, slots,
-                         ligs, pad_atoms, wv, wq, ag, astride, meta, nslots, group, P3, work, tw);
+      hipLaunchKernelGGL((products == kAtomF16x3 ? k_atoms_fwd<PL, kAtomF16x3>
+                          : products == kAtomF16 ? k_atoms_fwd<PL, kAtomF16>
+                                                 : k_atoms_fwd<PL, kAtomF32>),
+                         gf, dim3(64 * NB), 0, st, slots, ligs, pad_atoms, wv, wq, ag, astride, meta, nslots, group,
+                         P3, work, tw);
       hipLaunchKernelGGL(k_atoms_inv<PL>, dim3(PL::ZH, nslots), dim3(64 * NLW), atoms_inv_lds_bytes(PL::N), st, meta,
                          work, T2, invG, tw);
     }

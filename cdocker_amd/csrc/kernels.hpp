@@ -77,8 +77,15 @@ hipError_t launch_to_kz_blocks(hipStream_t st, int N, const float2* in, float2* 
 constexpr int kPipeAtoms = 2;
 bool atoms_supported(int N);
 // Atom form ligand-spectrum products: FP16 matrix cores on operands split
-// into two FP16 halves each, or FP32 matrix cores.
-constexpr int kAtomF16 = 0, kAtomF32 = 1;
+// into two FP16 parts each (hi, lo; about 2^-21 per product), FP32 matrix
+// cores, or FP16 matrix cores on operands split exactly into three FP16 parts
+// (hi, mid, lo) with the six partial products down to 2^-22 (an FP32 x FP32
+// product to about 2^-32, below FP32's own rounding).  In the split forms the
+// operands are scaled by powers of two (LigandDesc::ascale for the atom
+// weights, 2^12 for the y factors) so no part is an FP16 subnormal above
+// 2^-25 of the largest; the scale is taken out exactly before the inverse.
+constexpr int kAtomF16 = 0, kAtomF32 = 1, kAtomF16x3 = 2;
+constexpr float kAtomYScale = 4096.f;
 // pad_atoms: the ligands' padded channel lists (LigandDesc::pad_atoms).
 // work: at grid edge 96 the product spectra between the two kernels,
 // atoms_work_bytes(N) per pose (unused at 64).

@@ -66,10 +66,12 @@ int cdk_context_destroy(cdk_context* ctx);
  * 2 atom form at 64^3, 96^3 and 128^3, whose mid stage builds each channel's ligand
  * spectrum per kz plane from the atoms as a sum of outer products on the
  * matrix cores, with no forward stage; -1 = autotuned once per grid size),
- * "atom_products" (the atom form's products: 0 = FP16 matrix cores on
- * operands split into two FP16 halves each, three products per FP32-accurate
- * product with FP32 accumulation, about 2^-21 relative error per product;
- * 1 = FP32 matrix cores; default 0), "overlap_z" (atom form: 1 = each
+ * "atom_products" (the atom form's products: 2 = FP16 matrix cores on
+ * operands split exactly into three FP16 parts each (after power-of-two
+ * scaling), the six partial products down to 2^-22 with FP32 accumulation:
+ * an FP32 x FP32 product to about 2^-32, below FP32 rounding; 0 = two FP16
+ * parts each, three partial products, about 2^-21 relative error per
+ * product; 1 = FP32 matrix cores; default 2), "overlap_z" (atom form: 1 = each
  * launch set's z-inverse stage runs on a second stream beside the next set's
  * mid stage, with two T2 buffers; 0 = one stream; default 1),
  * "fwd_variant" (forward
@@ -81,10 +83,10 @@ int cdk_context_destroy(cdk_context* ctx);
 int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t value);
 /* Reads an option back; also "last_fwd_variant", "last_inv_variant",
  * "last_pipeline", "last_mid_group", "last_mid_depth" and "last_mid_poses"
- * (what the autotuner chose for the
- * last fused search) and "max_dyn_lds" (the
- * device's dynamic LDS limit per workgroup, bytes).  No reference
- * counterpart. */
+ * (what the autotuner chose for the last fused search), "max_dyn_lds" (the
+ * device's dynamic LDS limit per workgroup, bytes) and "workspace_bytes"
+ * (device bytes the fused engine's launch-set working arrays hold now; never
+ * above "ws_budget_bytes").  No reference counterpart. */
 int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value);
 /* Per-stage device time (HIP events on the context stream) accumulated while
  * "time_stages" is on: ms[8], launches[8].  Stage ids: reference engine
@@ -155,6 +157,14 @@ int cdk_search(cdk_context* ctx, cdk_protein* p, const cdk_ligand* l, int nconf,
 int cdk_rank_poses(cdk_protein* p, int natom, int nquat, const float* rotated, const float* energy,
                    const int* index, const float* mincoor, int n_lowest, int* out_rank, double* out_pose);
 
+/* cdk_rank_poses from the conformer and the rotations themselves (what the
+ * driver holds at NoMini:476-480) instead of pre-rotated coordinates:
+ * conformer [natom][3], quats [nquat][4]; only the kept rotations are
+ * rotated (Rotate.cpp:4-27, bit-identical to the search's rotation). */
+int cdk_rank_poses_quats(cdk_protein* p, int natom, int nquat, const float* conformer, const float* quats,
+                         const float* energy, const int* index, const float* mincoor, int n_lowest, int* out_rank,
+                         double* out_pose);
+
 /* ---- retained poses: grid rescoring and rigid refinement ------------- */
 /* Grid score of explicit poses, in FP64: coords [npose][natom][3] absolute
  * (A, the frame of the poses cdk_rank_poses rebuilds); energy [npose] =
@@ -219,21 +229,29 @@ int cdk_batch_destroy(cdk_batch* b);
  * the resident state of admitted requests (pose slots, conformer coordinates,
  * quaternions and per-pose results, about 104 B per pose).  Requests are
  * admitted in arrival order while that fits, and their poses are batched
- * with whatever else is in flight (at most max_batch_poses per launch set,
- * shared fairly).  Results of a request do not depend on what it was batched
- * with. */
+ * with whatever else is in flight (at most max_batch_poses per launch set).
+ * Launch sets go least-attained-service first: the admitted request with the
+ * fewest GPU seconds served so far picks the protein, and every admitted
+ * request of that protein shares the set.  Two launch sets are in flight at a
+ * time (the next one built and queued while the current one runs).  Results
+ * of a request do not depend on what it was batched with.  One worker thread
+ * per server; submit/result/stats may be called from any thread. */
 int cdk_server_create(cdk_context* ctx, int nprot, cdk_protein* const* prots, int64_t hbm_budget_bytes,
                       int max_batch_poses, cdk_server** out);
 int cdk_server_submit(cdk_server* s, int protein, int natom, const float* charge, const float* epsilon,
                       const float* radius, int nconf, const float* conf_coords, int nquat, const float* quats,
                       int shared_quats, int64_t* request_id);
 /* wait_ms < 0 blocks until done; 0 polls.  Returns CDK_ERR_PENDING if not
- * finished.  latency_s: submit -> results available on the host. */
+ * finished.  latency_s: submit -> results available on the host.  The call
+ * that returns a finished request's results (or its error) releases the
+ * request: its id is unknown afterwards (CDK_ERR_INVALID). */
 int cdk_server_result(cdk_server* s, int64_t request_id, int wait_ms, float* energy, int* index, float* mincoor,
                       double* latency_s);
-/* stats[8]: requests done, poses done, launch sets, mean batch poses,
- * busy seconds, admitted bytes now, queued requests, peak admitted bytes. */
-int cdk_server_stats(cdk_server* s, double* stats);
+/* The first n of: requests done, poses done, launch sets, mean poses per
+ * launch set, GPU seconds of the launch sets, admitted bytes now, queued
+ * requests, peak admitted bytes, requests held (not yet collected), launch
+ * sets in flight.  Entries past the tenth are not written. */
+int cdk_server_stats(cdk_server* s, int n, double* stats);
 int cdk_server_destroy(cdk_server* s);
 
 /* ---- reference pose-preparation surface, on the device --------------- */
@@ -273,6 +291,16 @@ int cdk_rng_create(uint64_t seed, cdk_rng** out);
 int cdk_rng_destroy(cdk_rng* r);
 /* QuaternionUniformSampling.cpp:5-24 (Shoemake): out [n][4]. */
 int cdk_quaternion_uniform_sampling(cdk_rng* r, int n, float* out);
+/* The same draws from a caller-owned generator: mt19937_64 points to a
+ * std::mt19937_64 of a C++ caller built against the same C++ standard
+ * library (QuaternionUniformSampling.h:10 takes std::mt19937_64&). */
+int cdk_quaternion_uniform_sampling_mt(void* mt19937_64, int n, float* out);
+/* Rotate.cpp:4-27 on the host (no context), bit-identical to cdk_rotate:
+ * out [nq][natom][3]. */
+int cdk_rotate_host(int nq, int natom, const float* quats, const float* coords, float* out);
+/* GetMinCoors.cpp:4-29 and GetMaxCoors.cpp:3-27 over rotated coordinates
+ * [nq][natom][3] (host); either output may be NULL. */
+int cdk_min_max_coors(int nq, int natom, const float* coors, float* mincoors, float* maxcoors);
 /* FilterQuaternions.cpp:11-83: keeps quaternions whose rotated extent is
  * below the grid length on every axis; if more than max_used remain, a
  * seeded shuffle (the reference uses rand()-driven random_shuffle) picks

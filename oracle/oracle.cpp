@@ -20,6 +20,8 @@
 //   orc_search                    NoMini:352-372, 390-409, 441-464 per pose
 //   orc_sparse_energy_f64         the same energy as a per-atom trilinear
 //                                 gather, an independent FP64 formulation
+//   orc_sparse_search_f64         that gather over the whole window: FP64
+//                                 argmin of many poses on worker threads
 //   orc_build_grids               the protein potential grids from atoms (the
 //                                 synthetic recipe's definition), brute force
 //
@@ -421,6 +423,117 @@ extern "C" double orc_sparse_energy_f64(const float* Pused, int N, float h, int 
   return e;
 }
 
+
+// ---- Full-window FP64 search by the same per-atom trilinear gather
+// (NoMini:441-464 window and first-minimum rule over E64(t) of
+// orc_sparse_energy_f64), for many poses on worker threads.  Per pose p:
+// rotate conformer conf[p] by quats[p] (Rotate.cpp:4-27), AABB
+// (GetMinCoors.cpp:4-29, GetMaxCoors.cpp:3-27), window N - int(len/h) - 2;
+// then over every window translation E64(t) = sum_a w_a sum_corners W P[..+t]
+// with the corner weights and grid offsets formed once per atom.  Outputs:
+// valid[p], emin[p], idx[p][3] (first minimum in x-major order); and, when
+// t_at is given, E64 and the absolute scale S = sum |w W P| at t_at[p] (the
+// caller's translation, e.g. the GPU's argmin).  full = 0 skips the window
+// search (only the values at t_at).
+extern "C" void orc_sparse_search_f64(const float* Pused, int N, float h, int natom, const float* coords,
+                                      const int* conf, const float* quats, int npose, const float* charge,
+                                      const float* eps, int num_used, const int* atom_channel, const int* t_at,
+                                      int full, int* valid, double* emin, int* idx, double* e_at, double* s_at) {
+  const size_t G = (size_t)N * N * N;
+  auto one = [&](int p) {
+    std::vector<float> rot((size_t)natom * 3);
+    orc_rotate_batch(1, natom, quats + (size_t)p * 4, coords + (size_t)conf[p] * natom * 3, rot.data());
+    float mn[3], mx[3];
+    orc_min_coors(1, natom, rot.data(), mn);
+    orc_max_coors(1, natom, rot.data(), mx);
+    int win[3], imax[3] = {0, 0, 0};
+    for (int a = 0; a < natom; ++a)
+      for (int k = 0; k < 3; ++k) imax[k] = std::max(imax[k], int((rot[a * 3 + k] - (double)mn[k]) / h));
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) {
+      const float len = mx[k] - mn[k];
+      win[k] = N - int(len / h) - 2;
+      ok = ok && win[k] > 0 && imax[k] + 2 <= N;
+    }
+    valid[p] = ok;
+    emin[p] = INFINITY;
+    idx[p * 3] = idx[p * 3 + 1] = idx[p * 3 + 2] = -1;
+    if (e_at) e_at[p] = NAN;
+    if (s_at) s_at[p] = NAN;
+    if (!ok) return;
+    // per (atom, channel pass, corner): weight w W and grid offset
+    std::vector<double> wt;
+    std::vector<size_t> off;
+    wt.reserve((size_t)natom * 16);
+    off.reserve((size_t)natom * 16);
+    for (int a = 0; a < natom; ++a) {
+      int id[3];
+      double f[3];
+      for (int k = 0; k < 3; ++k) {
+        const double d = (double)rot[a * 3 + k] - (double)mn[k];
+        id[k] = (int)(d / h);
+        f[k] = (d - (double)((float)id[k] * h)) / h;
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        const int c = pass == 0 ? atom_channel[a] : num_used;
+        if (c < 0) continue;
+        const double w = pass == 0 ? (double)std::sqrt(eps[a]) : (double)charge[a];
+        for (int cx = 0; cx < 2; ++cx)
+          for (int cy = 0; cy < 2; ++cy)
+            for (int cz = 0; cz < 2; ++cz) {
+              const double W = (cx ? f[0] : 1 - f[0]) * (cy ? f[1] : 1 - f[1]) * (cz ? f[2] : 1 - f[2]);
+              wt.push_back(w * W);
+              off.push_back((size_t)c * G + ((size_t)(id[0] + cx) * N + (id[1] + cy)) * N + (id[2] + cz));
+            }
+      }
+    }
+    const size_t nt = wt.size();
+    double m = INFINITY;
+    // one window row (i, j) at a time: every term adds to all of the row's
+    // translations, in term order (each E(t) sums its terms in the same order
+    // as a per-translation loop would)
+    std::vector<double> row(win[2]);
+    for (int i = 0; i < (full ? win[0] : 0); ++i)
+      for (int j = 0; j < win[1]; ++j) {
+        std::fill(row.begin(), row.end(), 0.0);
+        const size_t to = ((size_t)i * N + j) * N;
+        for (size_t u = 0; u < nt; ++u) {
+          const float* pp = Pused + off[u] + to;
+          const double w = wt[u];
+          for (int k = 0; k < win[2]; ++k) row[k] += w * (double)pp[k];
+        }
+        for (int k = 0; k < win[2]; ++k)
+          if (row[k] < m) {
+            m = row[k];
+            idx[p * 3] = i;
+            idx[p * 3 + 1] = j;
+            idx[p * 3 + 2] = k;
+          }
+      }
+    emin[p] = m;
+    if (t_at) {
+      const int* t = t_at + (size_t)p * 3;
+      if (t[0] >= 0 && t[1] >= 0 && t[2] >= 0 && t[0] < win[0] && t[1] < win[1] && t[2] < win[2]) {
+        const size_t to = ((size_t)t[0] * N + t[1]) * N + t[2];
+        double e = 0, sabs = 0;
+        for (size_t u = 0; u < nt; ++u) {
+          const double v = wt[u] * (double)Pused[off[u] + to];
+          e += v;
+          sabs += std::fabs(v);
+        }
+        if (e_at) e_at[p] = e;
+        if (s_at) s_at[p] = sabs;
+      }
+    }
+  };
+  const int nth = n_threads(npose);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t)
+    th.emplace_back([&, t] {
+      for (int p = t; p < npose; p += nth) one(p);
+    });
+  for (auto& t : th) t.join();
+}
 
 // ---- Protein potential grids from atoms (the grid builder's definition):
 // LJ channel p: sum_i sqrt(eps_i) ((R/d)^12 - 2 (R/d)^6), R = r_i + probe_p,

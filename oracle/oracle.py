@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_PATH = os.path.join(HERE, "liboracle.so")
+# CDK_ORACLE_LIBRARY: the sanitizer build of the oracle (CPU test runs)
+ORACLE_PATH = os.environ.get("CDK_ORACLE_LIBRARY") or os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libcdkref.so")
 
 FP = C.POINTER(C.c_float)
@@ -47,14 +48,47 @@ _orc.orc_search_pose.restype = C.c_int
 _orc.orc_pose_box.restype = C.c_int
 _orc.orc_quaternion_uniform_sampling.argtypes = [C.c_uint64, C.c_int, C.c_int, FP]
 
-_ref = C.CDLL(REF_PATH) if os.path.exists(REF_PATH) else None
-if _ref is not None:
-    _ref.ref_filter_quaternions.restype = C.c_int
-    _ref.ref_quaternion_uniform_sampling.argtypes = [C.c_uint64, C.c_int, C.c_int, FP]
+_ref_lib = None
+_cpubase_lib = None
+CPUBASE_PATH = os.path.join(HERE, "libcpubase.so")
+CPUBASE_FLAGS = "g++ -O3 -march=x86-64-v3 -ffp-contract=off"
+
+
+def correlate_window_full_timed(Pused, L, win, threads):
+    """correlate_window_full from libcpubase.so (the same source built -O3
+    for AVX2/FMA hosts): the timed CPU baseline."""
+    global _cpubase_lib
+    if _cpubase_lib is None:
+        _cpubase_lib = C.CDLL(CPUBASE_PATH)
+    _cpubase_lib.orc_set_threads(int(threads))
+    P = _f32(Pused)
+    L = _f32(L)
+    Cn, bx, by, bz = L.shape
+    N = P.shape[-1]
+    wx, wy, wz = [int(w) for w in win]
+    E = np.empty((wx, wy, wz), np.float32)
+    _cpubase_lib.orc_correlate_window_full_f32(_fp(P), N, _fp(L), Cn, bx, by, bz, wx, wy, wz, _fp(E))
+    return E
+
+
+def _ref():
+    """The compiled reference subset, loaded on first use only (CPU tests
+    and the golden-vector script ask for it; nothing else maps it)."""
+    global _ref_lib
+    if _ref_lib is None:
+        if not os.path.exists(REF_PATH):
+            raise RuntimeError("oracle/_ref/libcdkref.so missing: run `make -C oracle ref`")
+        lib = C.CDLL(REF_PATH)
+        lib.ref_filter_quaternions.restype = C.c_int
+        lib.ref_quaternion_uniform_sampling.argtypes = [C.c_uint64, C.c_int, C.c_int, FP]
+        lib.ref_read_grids.restype = C.c_int
+        lib.ref_read_grids.argtypes = [C.c_char_p, IP, IP, FP, FP, FP, FP, FP, C.c_int, FP, C.c_longlong]
+        _ref_lib = lib
+    return _ref_lib
 
 
 def have_ref():
-    return _ref is not None
+    return os.path.exists(REF_PATH)
 
 
 # ---- restated functions --------------------------------------------------
@@ -63,7 +97,7 @@ def rotate(quats, coords, lib=None):
     x = _f32(coords).reshape(-1, 3)
     out = np.empty((q.shape[0], x.shape[0], 3), np.float32)
     if lib == "ref":
-        _ref.ref_rotate_batch(q.shape[0], x.shape[0], _fp(q), _fp(x), _fp(out))
+        _ref().ref_rotate_batch(q.shape[0], x.shape[0], _fp(q), _fp(x), _fp(out))
     else:
         _orc.orc_rotate_batch(q.shape[0], x.shape[0], _fp(q), _fp(x), _fp(out))
     return out
@@ -75,8 +109,8 @@ def min_max_coors(rot, lib=None):
     mn = np.empty((nq, 3), np.float32)
     mx = np.empty((nq, 3), np.float32)
     if lib == "ref":
-        _ref.ref_min_coors(nq, na, _fp(r), _fp(mn))
-        _ref.ref_max_coors(nq, na, _fp(r), _fp(mx))
+        _ref().ref_min_coors(nq, na, _fp(r), _fp(mn))
+        _ref().ref_max_coors(nq, na, _fp(r), _fp(mx))
     else:
         _orc.orc_min_coors(nq, na, _fp(r), _fp(mn))
         _orc.orc_max_coors(nq, na, _fp(r), _fp(mx))
@@ -89,7 +123,7 @@ def idx_of_atoms_for_vdw_radius(atom_radii, grid_radii, lib=None):
     nu = C.c_int()
     used = np.zeros(gr.shape[0], np.int32)
     ari = np.zeros(ar.shape[0], np.int32)
-    f = _ref.ref_idx_of_atoms_for_vdw_radius if lib == "ref" else _orc.orc_idx_of_atoms_for_vdw_radius
+    f = _ref().ref_idx_of_atoms_for_vdw_radius if lib == "ref" else _orc.orc_idx_of_atoms_for_vdw_radius
     f(ar.shape[0], _fp(ar), gr.shape[0], _fp(gr), C.byref(nu), _ip(used), _ip(ari))
     return nu.value, used[: nu.value].copy(), ari
 
@@ -119,7 +153,7 @@ def fill_ligand_grid(coors, mincoors, charge, eps, num_used, idx_used, atom_radi
         grid = np.zeros((nq, num_used + 1, X, Y, Z), np.float32)
     g = np.ascontiguousarray(grid, dtype=np.float32)
     if lib == "ref":
-        _ref.ref_fill_ligand_grid(nq, na, _fp(co), _fp(mc), _fp(q), _fp(e), num_used, _ip(used), nv, _ip(offs),
+        _ref().ref_fill_ligand_grid(nq, na, _fp(co), _fp(mc), _fp(q), _fp(e), num_used, _ip(used), nv, _ip(offs),
                                   _ip(atoms), X, Y, Z, C.c_float(spacing), _fp(g))
     else:
         _orc.orc_fill_ligand_grid(nq, na, _fp(co), _fp(mc), _fp(q), _fp(e), num_used, _ip(used), _ip(offs),
@@ -129,7 +163,7 @@ def fill_ligand_grid(coors, mincoors, charge, eps, num_used, idx_used, atom_radi
 
 def quaternion_uniform_sampling(seed, ncall, n, lib=None):
     out = np.empty((ncall, n, 4), np.float32)
-    f = _ref.ref_quaternion_uniform_sampling if lib == "ref" else _orc.orc_quaternion_uniform_sampling
+    f = _ref().ref_quaternion_uniform_sampling if lib == "ref" else _orc.orc_quaternion_uniform_sampling
     f(C.c_uint64(seed), ncall, n, _fp(out))
     return out
 
@@ -139,13 +173,32 @@ def filter_quaternions(coor, quats, lens, lib=None):
     q = _f32(quats).reshape(-1, 4)
     if lib == "ref":
         out = np.empty_like(q)
-        n = _ref.ref_filter_quaternions(_fp(x), x.shape[0], q.shape[0], _fp(q), C.c_float(lens[0]),
+        n = _ref().ref_filter_quaternions(_fp(x), x.shape[0], q.shape[0], _fp(q), C.c_float(lens[0]),
                                         C.c_float(lens[1]), C.c_float(lens[2]), q.shape[0], _fp(out))
         return out[:n].copy()
     kept = np.zeros(q.shape[0], np.int32)
     n = _orc.orc_filter_quaternions(_fp(x), x.shape[0], q.shape[0], _fp(q), C.c_float(lens[0]), C.c_float(lens[1]),
                                     C.c_float(lens[2]), _ip(kept))
     return q[kept[:n]].copy()
+
+
+def read_grids_ref(path, max_values=1 << 22):
+    """ReadGrids.cpp:6-68, the compiled reference reader: (values [G][X][Y][Z],
+    radii, mid, len, spacing, restraint_k)."""
+    ng = C.c_int()
+    dims = np.zeros(3, np.int32)
+    mid = np.zeros(3, np.float32)
+    ln = np.zeros(3, np.float32)
+    sp = C.c_float()
+    rk = C.c_float()
+    radii = np.zeros(64, np.float32)
+    vals = np.zeros(max_values, np.float32)
+    rc = _ref().ref_read_grids(str(path).encode(), C.byref(ng), _ip(dims), _fp(mid), _fp(ln), C.byref(sp),
+                               C.byref(rk), _fp(radii), 64, _fp(vals), max_values)
+    if rc != 0:
+        raise ValueError("grid file larger than the buffers")
+    n = int(ng.value) * int(np.prod(dims))
+    return (vals[:n].reshape(ng.value, *dims).copy(), radii[: ng.value - 1].copy(), mid, ln, sp.value, rk.value)
 
 
 def argmin_first(E):
@@ -248,6 +301,29 @@ class PoseProblem:
                           self.num_used, _ip(used), _ip(self.ari), _ip(box), _ip(win), _fp(minc), _fp(L),
                           C.c_size_t(n))
         return L.reshape(self.num_used + 1, *box), box, win, minc
+
+    def sparse_search(self, coords, conf, quats, t_at=None, full=True):
+        """orc_sparse_search_f64: full-window FP64 argmin of every pose
+        (conformer coords[conf[p]] rotated by quats[p]); with t_at [n][3], E64
+        and the absolute scale S at those translations too (full=False: only
+        those)."""
+        x = _f32(coords).reshape(-1, self.natom, 3)
+        cf = _i32(conf).reshape(-1)
+        q = _f32(quats).reshape(-1, 4)
+        n = q.shape[0]
+        assert cf.shape[0] == n and (cf.max(initial=0) < x.shape[0])
+        valid = np.zeros(n, np.int32)
+        emin = np.zeros(n, np.float64)
+        idx = np.zeros((n, 3), np.int32)
+        e_at = np.full(n, np.nan)
+        s_at = np.full(n, np.nan)
+        ta = None if t_at is None else _i32(t_at).reshape(n, 3)
+        _orc.orc_sparse_search_f64(_fp(self.Pused), self.N, C.c_float(self.h), self.natom, _fp(x), _ip(cf), _fp(q), n,
+                                   _fp(self.charge), _fp(self.eps), self.num_used, _ip(self.atom_channel),
+                                   None if ta is None else _ip(ta), int(bool(full)), _ip(valid), _dp(emin),
+                                   _ip(idx), _dp(e_at),
+                                   _dp(s_at))
+        return dict(valid=valid.astype(bool), emin=emin, index=idx, e_at=e_at, s_at=s_at)
 
     def sparse_energy(self, rot, minc, t):
         r = _f32(rot).reshape(-1, 3)

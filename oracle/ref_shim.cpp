@@ -3,9 +3,9 @@
 // extern "C" entry points over the reference's own pure-math sources
 // (Rotate.cpp, QuaternionMultiply.cpp, GetMinCoors.cpp, GetMaxCoors.cpp,
 // GetIdxOfAtomsForVdwRadius.cpp, FillLigandGrid.cpp,
-// QuaternionUniformSampling.cpp, FilterQuaternions.cpp), compiled where they
-// lie by oracle/Makefile into oracle/_ref/.  Used only to write and check
-// golden vectors; this file contains no reference code.
+// QuaternionUniformSampling.cpp, FilterQuaternions.cpp, ReadGrids.cpp),
+// compiled where they lie by oracle/Makefile into oracle/_ref/.  Used only to
+// write and check golden vectors; this file contains no reference code.
 #include <cstdint>
 #include <cstring>
 #include <random>
@@ -17,6 +17,7 @@
 #include "GetMaxCoors.h"
 #include "GetMinCoors.h"
 #include "QuaternionUniformSampling.h"
+#include "ReadGrids.h"
 #include "Rotate.h"
 
 extern "C" {
@@ -68,6 +69,40 @@ int ref_filter_quaternions(float* coor, int natom, int nq, float* quats, float l
   std::memcpy(out, used, sizeof(float) * 4 * n);
   delete[] used;
   return n;
+}
+
+// ReadGrids.cpp:6-68 into caller buffers: header fields, up to max_radii
+// radii and up to max_values values (returns -1 if either would overflow).
+int ref_read_grids(const char* path, int* num_grids, int* dims, float* mid, float* len, float* spacing,
+                   float* restraint_k, float* radii, int max_radii, float* values, long long max_values) {
+  int ng = 0, x = 0, y = 0, z = 0;
+  float mx = 0, my = 0, mz = 0, lx = 0, ly = 0, lz = 0, h = 0, rk = 0;
+  float* r = 0;
+  float* v = 0;
+  ReadGrids(ng, x, y, z, mx, my, mz, lx, ly, lz, h, rk, r, v, std::string(path));
+  *num_grids = ng;
+  dims[0] = x;
+  dims[1] = y;
+  dims[2] = z;
+  mid[0] = mx;
+  mid[1] = my;
+  mid[2] = mz;
+  len[0] = lx;
+  len[1] = ly;
+  len[2] = lz;
+  *spacing = h;
+  *restraint_k = rk;
+  const long long nv = (long long)x * y * z * ng;
+  int rc = 0;
+  if (ng - 1 > max_radii || nv > max_values) {
+    rc = -1;
+  } else {
+    if (ng > 1) std::memcpy(radii, r, sizeof(float) * (ng - 1));
+    if (nv > 0) std::memcpy(values, v, sizeof(float) * nv);
+  }
+  delete[] r;
+  delete[] v;
+  return rc;
 }
 
 }  // extern "C"

@@ -2,7 +2,10 @@
 sources (Rotate.cpp, QuaternionMultiply.cpp, GetMinCoors.cpp, GetMaxCoors.cpp,
 GetIdxOfAtomsForVdwRadius.cpp, FillLigandGrid.cpp,
 QuaternionUniformSampling.cpp, FilterQuaternions.cpp), compiled by
-`make -C oracle ref` into oracle/_ref/libcdkref.so.
+`make -C oracle ref` into oracle/_ref/libcdkref.so; and grid text files
+tests/golden/grids_*.txt (written here, in the ReadGrids format with assorted
+spacing, line endings and number spellings) with what the reference's
+ReadGrids.cpp reads from them in tests/golden/grids_ref.npz.
 
 Inputs are drawn with numpy's seeded generator, independently of the
 library's own generators.  Run: python tests/golden/make_golden.py
@@ -16,7 +19,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 from oracle import oracle as orc  # noqa: E402
 
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "ref_vectors.npz")
+GRID_OUT = os.path.join(HERE, "grids_ref.npz")
 PROBE = np.array([1.2, 1.5, 1.7, 1.8, 1.9, 2.0, 2.1, 2.2], np.float32)
 
 
@@ -93,6 +98,71 @@ def main():
     d.update(filt_coords=xf, filt_quats=qf, filt_lens=lens, filt_out=orc.filter_quaternions(xf, qf, lens, lib="ref"))
     np.savez_compressed(OUT, **d)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    grid_files()
+
+
+def _num(v, style):
+    """One value in one of several spellings a grid file may hold."""
+    if style == 0:
+        return "%.9g" % v
+    if style == 1:
+        return "%.6E" % v
+    if style == 2:
+        return "%+.7f" % v
+    s = "%.5g" % v
+    return s.replace("0.", ".", 1) if s.startswith("0.") else s.replace("-0.", "-.", 1)
+
+
+def grid_files():
+    """ReadGrids.cpp:21-66 layout: 3 title lines, the header (numGrids X Y Z
+    mid len spacing restraint_k), numGrids-1 radii, then per grid one
+    separator line and X*Y*Z lines "x y z value", z fastest."""
+    rs = np.random.default_rng(20260202)
+    d = {}
+    cases = {
+        # LF, one header line, one radii line
+        "plain": dict(dims=(4, 5, 6), ng=3, eol="\n", sep=" ", trail="", split=False),
+        # CRLF, trailing blanks and tabs, the header over two lines, tab separators
+        "crlf": dict(dims=(3, 4, 5), ng=4, eol="\r\n", sep="\t ", trail="  \t", split=True),
+    }
+    for name, c in cases.items():
+        X, Y, Z = c["dims"]
+        ng = c["ng"]
+        vals = rs.standard_normal((ng, X, Y, Z)) * np.exp(rs.uniform(-12, 6, (ng, X, Y, Z)))
+        vals.reshape(-1)[:3] = [0.0, -0.0, 3.0]
+        vals = vals.astype(np.float32)
+        radii = rs.choice(PROBE, ng - 1, replace=False)
+        eol, sp, tr = c["eol"], c["sep"], c["trail"]
+        head = [ng, X, Y, Z, 1.25, -2.5, 3.75, (X - 1) * 0.5, (Y - 1) * 0.5, (Z - 1) * 0.5, 0.5, 10.0]
+        hs = [str(v) for v in head]
+        lines = ["* grid file " + name + tr, "* second title line", "*"]
+        if c["split"]:
+            lines += [sp.join(hs[:4]) + tr, sp.join(hs[4:]) + tr]
+        else:
+            lines += [sp.join(hs) + tr]
+        lines += [sp.join("%.2f" % r for r in radii) + tr]
+        k = 0
+        for g in range(ng):
+            lines.append("grid %d" % g + tr)
+            for ix in range(X):
+                for iy in range(Y):
+                    for iz in range(Z):
+                        v = float(vals[g, ix, iy, iz])
+                        xyz = sp.join("%.3f" % t for t in (1.25 + ix * 0.5, -2.5 + iy * 0.5, 3.75 + iz * 0.5))
+                        lines.append(xyz + sp + _num(v, k % 4) + tr)
+                        k += 1
+        path = os.path.join(HERE, "grids_%s.txt" % name)
+        with open(path, "w", newline="") as f:
+            f.write(eol.join(lines) + eol)
+        rv, rr, rm, rl, rh, rk = orc.read_grids_ref(path)
+        d[name + "_values"] = rv
+        d[name + "_radii"] = rr
+        d[name + "_mid"] = rm
+        d[name + "_len"] = rl
+        d[name + "_spacing"] = np.array([rh], np.float32)
+        d[name + "_restraint_k"] = np.array([rk], np.float32)
+    np.savez_compressed(GRID_OUT, **d)
+    print("wrote", GRID_OUT, os.path.getsize(GRID_OUT), "bytes")
 
 
 if __name__ == "__main__":

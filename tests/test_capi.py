@@ -92,6 +92,21 @@ def test_grid_file_round_trip(tmp_path):
         cdk.read_grids(bad)
 
 
+@pytest.mark.parametrize("name", ["plain", "crlf"])
+def test_grid_reader_matches_reference_reader(name):
+    """cdk_read_grids on grid files with LF and CRLF endings, trailing blanks
+    and tabs, a header over two lines and assorted number spellings gives
+    what the reference's own ReadGrids.cpp:6-68 read from the same files,
+    bit for bit (signed zeros included)."""
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "grids_ref.npz"))
+    v, radii, mid, ln, sp, rk = cdk.read_grids(os.path.join(ROOT, "tests", "golden", "grids_%s.txt" % name))
+    assert np.array_equal(v.view(np.uint32), gold[name + "_values"].view(np.uint32))
+    assert np.array_equal(radii.view(np.uint32), gold[name + "_radii"].view(np.uint32))
+    assert np.array_equal(mid, gold[name + "_mid"]) and np.array_equal(ln, gold[name + "_len"])
+    assert np.float32(sp) == gold[name + "_spacing"][0] and np.float32(rk) == gold[name + "_restraint_k"][0]
+    assert np.signbit(v.reshape(-1)[1]) and v.reshape(-1)[2] == 3.0
+
+
 def test_write_poses_files_and_energy_rows(tmp_path):
     """conformer_<c>_<i>.pdb files and energy.txt rows as the drivers write
     them (TranRotaConfSearchNoMini.cu:489-502); runs on the host only."""

@@ -205,6 +205,50 @@ def test_server_alone_equals_in_mix(ctx, prots):
     srv.close()
 
 
+def test_workspace_within_budget_with_overlap(prots):
+    """The launch-set working arrays, including the overlapped atom form's
+    second T2 buffer, stay within the context's workspace budget, with the
+    overlap on and off."""
+    say("workspace budget")
+    g, prot0 = prots[1]
+    budget = 1 << 30
+    c = cdk.Context(0, ws_budget_bytes=budget)
+    prot = cdk.Protein(c, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    w = workloads.config2(nconf=2, nquat=1500)
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    try:
+        for ovl, pipeline in ((1, 2), (0, 2), (1, -1), (1, 0)):
+            c.set_option("overlap_z", ovl)
+            c.set_option("pipeline", pipeline)
+            cdk.search(c, prot, lig, L["conformers"], L["quats"])
+            used = c.get_option("workspace_bytes")
+            say("overlap", ovl, "pipeline", pipeline, "workspace bytes", used, "budget", budget)
+            assert 0 < used <= budget
+    finally:
+        lig.close()
+        prot.close()
+        c.close()
+
+
+def test_server_releases_collected_requests(ctx, prots):
+    """Many requests submitted and collected: the server holds none of them
+    afterwards, and a collected id is unknown."""
+    say("server releases requests")
+    srv = cdk.Server(ctx, [prots[1][1]], 16 << 30, 256)
+    L = workloads.ligand(920, 20, 2)
+    q = workloads.quaternion_sets(921, 2, 30)
+    for rnd in range(3):
+        ids = [srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], q) for _ in range(20)]
+        for i in ids:
+            assert srv.result(i) is not None
+        st = srv.stats()
+        assert st["held_requests"] == 0 and st["requests_done"] == 20 * (rnd + 1), st
+    with pytest.raises(cdk.CdkError):
+        srv.result(ids[-1], wait_ms=0)
+    srv.close()
+
+
 def test_server_budget_rejects_oversized(ctx, prots):
     say("server budget")
     ws = 8 << 30  # the context's default workspace budget

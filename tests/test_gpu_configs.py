@@ -1,0 +1,241 @@
+"""The benchmark configurations at their stated sizes on the GPU.
+
+- single-ligand throughput (64^3 x 9, 40-atom ligand): the atom form's three
+  product forms over 1200 poses against the FP64 oracle; the default (exact
+  three-part FP16 splits) is no less accurate than FP32 matrix-core products.
+- uneven batching (64 ligands, 15-60 atoms, 10-100 conformers, 1000
+  quaternions: about 3.5 M poses in one hipGraph-captured launch sequence):
+  every ligand's results bit-identical to its own search, and a sample of
+  poses against the FP64 oracle.
+- serving (64^3 and 96^3 resident, Poisson arrivals while launch sets are in
+  flight): every served result bit-identical to the request searched alone,
+  and a sample against the FP64 oracle.
+- large grid (128^3 at 0.375 A, 50-atom ligand, 50 conformers x 3000
+  quaternions): top-10 poses per conformer, 1% of the rotations checked
+  against the FP64 oracle (energy at the GPU's translation) and 1000 of them
+  over the whole window (argmin).
+
+FP64 checks use the oracle's per-atom gather over the window
+(orc_sparse_search_f64) under the tolerance rules of tests/parity.py."""
+import os
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import cdocker_amd as cdk
+from cdocker_amd import workloads
+from oracle import oracle as orc
+from tests.parity import check_sampled
+
+pytestmark = pytest.mark.gpu
+
+# Oracle worker threads: the GPU box's CPU share.
+orc.set_threads(min(16, os.cpu_count() or 1))
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cdk.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def p64(ctx):
+    g = workloads.protein_grids("p64")
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    yield g, prot
+    prot.close()
+
+
+def flat_sample(rs, nconf, nquat, n):
+    """n distinct (conformer, quaternion) pairs."""
+    k = rs.choice(nconf * nquat, size=min(n, nconf * nquat), replace=False)
+    return k // nquat, k % nquat
+
+
+def test_single_ligand_products_accuracy_1200_poses(ctx, p64):
+    """Single-ligand workload (64^3 x 9, the 40-atom ligand): 12 conformers x
+    100 quaternions = 1200 poses through the atom form with each product form
+    and through the all-FP32 column form.  Every form within the parity
+    tolerance against the FP64 oracle; the default exact three-part split is
+    at least as accurate as the FP32 matrix-core products (largest scaled
+    error and largest plain relative error), and better than the two-part
+    split."""
+    g, prot = p64
+    w = workloads.config2(nconf=12, nquat=100)
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    conf = np.repeat(np.arange(12), 100)
+    quats = L["quats"].reshape(-1, 4)
+    stats = {}
+    try:
+        for label, pipeline, products in (("x3", 2, 2), ("fp32-mfma", 2, 1), ("x2", 2, 0), ("column-fp32", 1, 2)):
+            ctx.set_option("pipeline", pipeline)
+            ctx.set_option("atom_products", products)
+            E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+            assert ctx.get_option("last_pipeline") == pipeline
+            st = check_sampled(prob, L["conformers"], conf, quats, E, idx, mc, "single-ligand " + label,
+                               argmin=label == "x3")
+            say("1200 poses,", label, st)
+            stats[label] = st
+    finally:
+        ctx.set_option("pipeline", -1)
+        ctx.set_option("atom_products", 2)
+        lig.close()
+    assert stats["x3"]["valid"] >= 1000 and stats["x3"]["exact"] + stats["x3"]["near_ties"] == stats["x3"]["valid"]
+    for key in ("max_scaled_err", "max_rel_err_where_asserted"):
+        assert stats["x3"][key] <= stats["fp32-mfma"][key], (key, stats["x3"][key], stats["fp32-mfma"][key])
+        assert stats["x3"][key] <= stats["x2"][key], (key, stats["x3"][key], stats["x2"][key])
+
+
+def test_uneven_batch_full_size(ctx, p64):
+    """64 ligands (15-60 atoms, 10-100 conformers, 1000 quaternions each) in
+    one hipGraph-captured launch sequence: each ligand's energies, indices
+    and min corners bit-identical to the ligand searched alone
+    (TranRotaConfSearchNoMini.cu:297-464 per ligand), and 60 poses of every
+    ligand (3840, about 0.1%) against the FP64 oracle over the whole window."""
+    g, prot = p64
+    w = workloads.config3()
+    ligs = [cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"]) for L in w["ligands"]]
+    b = cdk.Batch(ctx, prot)
+    for lig, L in zip(ligs, w["ligands"]):
+        b.add(lig, L["conformers"], L["quats"])
+    total = b.total_poses()
+    say("uneven batch:", len(ligs), "ligands,", total, "poses")
+    assert total > 3_000_000
+    t0 = time.time()
+    b.run(cdk.ENGINE_FUSED)
+    say("uneven batch ran in %.2f s" % (time.time() - t0))
+    rs = np.random.default_rng(3030)
+    checked = dict(poses=0, valid=0, exact=0, near_ties=0, max_scaled_err=0.0, max_rel_err_where_asserted=0.0)
+    try:
+        for j, (lig, L) in enumerate(zip(ligs, w["ligands"])):
+            E, idx, mc = b.results(j)
+            E1, i1, m1 = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+            assert same(E, E1) and np.array_equal(idx, i1) and same(mc, m1), ("ligand", j)
+            nconf, nquat = E.shape
+            c, q = flat_sample(rs, nconf, nquat, 60)
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+            st = check_sampled(prob, L["conformers"], c, L["quats"][c, q], E[c, q], idx[c, q], mc[c, q],
+                               "uneven batch ligand %d" % j)
+            for k in ("poses", "valid", "exact", "near_ties"):
+                checked[k] += st[k]
+            for k in ("max_scaled_err", "max_rel_err_where_asserted"):
+                checked[k] = max(checked[k], st[k])
+    finally:
+        b.close()
+        for lig in ligs:
+            lig.close()
+    say("uneven batch FP64 sample:", checked)
+    assert checked["poses"] >= 0.001 * total and checked["valid"] >= 0.9 * checked["poses"]
+
+
+def test_serving_poisson_full_size(ctx, p64):
+    """Serving: 64^3 and 96^3 x 9 proteins resident with their spectra, 16
+    requests drawn as the serving configuration says (15-80 atoms, 1-50
+    conformers, 500/2000/5000 quaternions, either protein) submitted on a
+    Poisson clock while launch sets are in flight.  Every served result is
+    bit-identical to the request searched alone; 1000 served poses are
+    checked against the FP64 oracle; every request is released once
+    collected."""
+    g64, prot64 = p64
+    g96 = workloads.protein_grids("p96")
+    prot96 = cdk.Protein(ctx, g96["values"], g96["radii"], g96["spacing"], g96["mid"], g96["length"])
+    grids = [g64, g96]
+    prots = [prot64, prot96]
+    rs = np.random.default_rng(4040)
+    reqs = [workloads.serving_request(rs, 7000 + 10 * k) for k in range(16)]
+    poses = sum(r["conformers"].shape[0] * r["quats"].shape[1] for r in reqs)
+    say("serving:", len(reqs), "requests,", poses, "poses,", [r["protein_index"] for r in reqs])
+    srv = cdk.Server(ctx, prots, 24 << 30, 4096)
+    ids, busy_at_submit = [], []
+    arrivals = np.cumsum(rs.exponential(0.12, len(reqs)))
+    t0 = time.time()
+    for r, at in zip(reqs, arrivals):
+        dt = at - (time.time() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        st = srv.stats()
+        busy_at_submit.append(st["sets_in_flight"] > 0)
+        ids.append(srv.submit(r["protein_index"], r["charge"], r["epsilon"], r["radius"], r["conformers"],
+                              r["quats"]))
+    served = [srv.result(i) for i in ids]
+    st = srv.stats()
+    say("serving stats", st, "submitted while a launch set was in flight:", sum(busy_at_submit))
+    assert st["requests_done"] == len(reqs) and st["poses_done"] == poses
+    assert st["held_requests"] == 0 and st["sets_in_flight"] == 0
+    assert sum(busy_at_submit) >= len(reqs) // 2
+    with pytest.raises(cdk.CdkError):
+        srv.result(ids[0], wait_ms=0)  # released when collected
+    srv.close()
+    checked = dict(poses=0, valid=0, exact=0, near_ties=0)
+    try:
+        for k, (r, s) in enumerate(zip(reqs, served)):
+            p = r["protein_index"]
+            lig = cdk.Ligand(prots[p], r["charge"], r["epsilon"], r["radius"])
+            E, idx, mc = cdk.search(ctx, prots[p], lig, r["conformers"], r["quats"])
+            lig.close()
+            assert same(E, s[0]) and np.array_equal(idx, s[1]) and same(mc, s[2]), ("request", k)
+            assert s[3] > 0
+            nconf, nquat = E.shape
+            c, q = flat_sample(rs, nconf, nquat, 1000 // len(reqs) + 1)
+            g = grids[p]
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], r["charge"], r["epsilon"], r["radius"])
+            st = check_sampled(prob, r["conformers"], c, r["quats"][c, q], E[c, q], idx[c, q], mc[c, q],
+                               "serving request %d" % k)
+            for key in checked:
+                checked[key] += st[key]
+    finally:
+        prot96.close()
+    say("serving FP64 sample:", checked)
+    assert checked["poses"] >= 1000
+
+
+def test_large_grid_topk_and_fp64_sample(ctx):
+    """Large grid: 128^3 x 9 at 0.375 A, 50-atom ligand, 50 conformers x 3000
+    quaternions.  Top-10 poses per conformer (sort_index and the pose
+    rebuild, NoMini:466-487); 1% of the rotations (1500) checked against the
+    FP64 oracle at the GPU's translation, 1000 of them over the whole window."""
+    g = workloads.protein_grids("p128")
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    w = workloads.config5()
+    L = w["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    try:
+        t0 = time.time()
+        E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+        say("large grid: %d poses in %.2f s" % (E.size, time.time() - t0))
+        gmin = prot.grid_min()
+        for c in range(E.shape[0]):
+            rank, pose = cdk.rank_poses(prot, ctx.Rotate(L["quats"][c], L["conformers"][c]), E[c], idx[c], mc[c], 10)
+            assert rank.tolist() == cdk.sort_index(E[c])[:10].tolist()
+            rot = ctx.Rotate(L["quats"][c, rank], L["conformers"][c])
+            shift = gmin - mc[c, rank].astype(np.float64) + idx[c, rank].astype(np.float32) * np.float32(prot.spacing)
+            assert np.allclose(pose, rot.astype(np.float64) + shift[:, None, :], atol=1e-9)
+        rs = np.random.default_rng(5050)
+        c, q = flat_sample(rs, E.shape[0], E.shape[1], E.size // 100)
+        prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+        st = check_sampled(prob, L["conformers"], c, L["quats"][c, q], E[c, q], idx[c, q], mc[c, q],
+                           "large grid 1% energy", argmin=False)
+        say("large grid 1% FP64 energy check:", st)
+        assert st["poses"] == E.size // 100
+        t0 = time.time()
+        st = check_sampled(prob, L["conformers"], c[:1000], L["quats"][c[:1000], q[:1000]], E[c[:1000], q[:1000]],
+                           idx[c[:1000], q[:1000]], mc[c[:1000], q[:1000]], "large grid argmin")
+        say("large grid FP64 argmin over 1000 poses (%.1f s):" % (time.time() - t0), st)
+        assert st["exact"] + st["near_ties"] == st["valid"] and st["valid"] >= 900
+    finally:
+        lig.close()
+        prot.close()

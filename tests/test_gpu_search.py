@@ -254,39 +254,45 @@ def test_atom_form_large_ligand(ctx, p64):
 
 
 @pytest.mark.parametrize("name", ["p64", "p96", "p128"])
-def test_atom_products_split_f16_and_f32(ctx, name):
-    """The atom form's two product forms (split-FP16 matrix cores, the
-    default, and FP32 matrix cores) against the oracle, each within the parity
-    tolerance with a scaled error well under it, and the option in effect
-    (the two forms round differently)."""
+def test_atom_products_all_forms(ctx, name):
+    """The atom form's three product forms (FP16 matrix cores on exact
+    three-part operand splits, the default; two-part splits; FP32 matrix
+    cores) against the oracle, each within the parity tolerance with a scaled
+    error well under it, and the option in effect (the forms round
+    differently).  Ligands with tiny weights (charges down to 1e-6, whose
+    low FP16 parts would be subnormal without the power-of-two scaling) too."""
     g = workloads.protein_grids(name)
     prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
     ctx.set_option("pipeline", 2)
     try:
-        assert ctx.get_option("atom_products") == 0
+        assert ctx.get_option("atom_products") == 2
         with pytest.raises(cdk.CdkError):
-            ctx.set_option("atom_products", 2)
-        assert ctx.get_option("atom_products") == 0
-        for seed, na, nq in ((941, 23, 5), (942, 60, 4)):
+            ctx.set_option("atom_products", 3)
+        assert ctx.get_option("atom_products") == 2
+        for seed, na, nq, tiny in ((941, 23, 5, False), (942, 60, 4, False), (943, 30, 4, True)):
             L = workloads.ligand(seed, na, 1)
+            charge, eps = L["charge"], L["epsilon"]
+            if tiny:
+                charge = (charge * np.float32(1e-4) * np.logspace(-2, 0, na).astype(np.float32)).astype(np.float32)
+                eps = (eps * np.float32(1e-6)).astype(np.float32)
             quats = workloads.quaternion_sets(seed + 1, 1, nq)
-            lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
-            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+            lig = cdk.Ligand(prot, charge, eps, L["radius"])
+            prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], charge, eps, L["radius"])
             res = {}
-            for products in (0, 1):
+            for products in (2, 0, 1):
                 ctx.set_option("atom_products", products)
                 E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats)
                 assert ctx.get_option("last_pipeline") == 2
-                stats = check_search(prob, L["conformers"], quats, E, idx, mc)
-                say(name, "atom products", products, "ligand", na, stats)
+                stats = check_search(prob, L["conformers"], quats, E, idx, mc,
+                                     label="%s products %d ligand %d" % (name, products, seed))
+                say(name, "atom products", products, "ligand", na, "tiny weights" if tiny else "", stats)
                 assert stats.get("exact", 0) >= nq - 1
                 assert stats["max_scaled_err"] <= 1e-5
                 res[products] = E
-            assert not np.array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
-            ctx.set_option("atom_products", 0)
+            assert not np.array_equal(res[2].view(np.uint32), res[1].view(np.uint32))
             lig.close()
     finally:
-        ctx.set_option("atom_products", 0)
+        ctx.set_option("atom_products", 2)
         ctx.set_option("pipeline", -1)
         prot.close()
 

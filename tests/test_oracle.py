@@ -101,6 +101,18 @@ def test_live_reference_fresh_inputs():
     assert np.array_equal(bits(s1), bits(s2))
 
 
+@needs_ref
+def test_live_reference_grid_reader_reproduces_fixture():
+    """The grid-reader fixture is what the compiled ReadGrids.cpp reads."""
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    gold = np.load(os.path.join(here, "grids_ref.npz"))
+    for name in ("plain", "crlf"):
+        v, radii, mid, ln, sp, rk = orc.read_grids_ref(os.path.join(here, "grids_%s.txt" % name))
+        assert np.array_equal(bits(v), bits(gold[name + "_values"]))
+        assert np.array_equal(bits(radii), bits(gold[name + "_radii"]))
+        assert np.array_equal(mid, gold[name + "_mid"]) and sp == gold[name + "_spacing"][0]
+
+
 # ------------------------------------------------------ closed forms
 def rotmat(q):
     w, x, y, z = q
@@ -186,6 +198,58 @@ def test_dense_matches_sparse_fp64():
     prob.charge[0] += np.float32(0.05)
     s_bad = prob.sparse_energy(rot, res["mincoor"], tuple(res["index"]))
     assert abs(E[tuple(res["index"])] - s_bad) > 1e-3
+
+
+def test_sparse_search_matches_dense_fp64_search():
+    """orc_sparse_search_f64 (the full-window gather search used to check
+    argmins at scale) against the dense FP64 correlation's search of the same
+    poses: same validity, same argmin (or a gap within the cells' float
+    rounding), same minimum; E64 and S at a given translation equal the
+    single-translation gather."""
+    prob, x, rs = small_problem(seed=9)
+    confs = np.stack([x, x[::-1].copy()])  # two conformers (the second a relabelling)
+    q = rs.standard_normal((12, 4)).astype(np.float32)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    conf = np.arange(12) % 2
+    t_at = np.zeros((12, 3), np.int32)
+    t_at[:, 0] = 1
+    t_at[:, 2] = 2
+    res = prob.sparse_search(confs, conf, q, t_at)
+    nvalid = 0
+    for p in range(12):
+        d = prob.search_pose(confs[conf[p]], q[p], double=True, want_E=True)
+        assert bool(res["valid"][p]) == d["valid"]
+        if not d["valid"]:
+            continue
+        nvalid += 1
+        E = d["E"]
+        gi = tuple(res["index"][p])
+        assert abs(res["emin"][p] - E.min()) <= 2e-6 * (1 + abs(E.min()))
+        assert E[gi] - E.min() <= 4e-6 * (1 + abs(E.min()))
+        rot = orc.rotate(q[p], confs[conf[p]])[0]
+        e1 = prob.sparse_energy(rot, d["mincoor"], (1, 0, 2))
+        assert abs(res["e_at"][p] - e1) <= 1e-12 * (1 + abs(e1))  # same terms, other summation order
+        assert res["s_at"][p] >= abs(res["e_at"][p])
+    assert nvalid >= 8
+    # planted fault: a transposed translation (k, j, i) is detected
+    p = int(np.flatnonzero(res["valid"])[0])
+    d = prob.search_pose(confs[conf[p]], q[p], double=True, want_E=True)
+    i, j, k = res["index"][p]
+    if (i, j, k) != (k, j, i) and max(i, k) < min(d["E"].shape[0], d["E"].shape[2]):
+        assert d["E"][k, j, i] - d["E"].min() > 1e-6
+
+
+def test_timed_cpu_baseline_build_rounds_as_the_oracle():
+    """libcpubase.so (the timed CPU baseline: the same source at -O3 for
+    AVX2/FMA hosts, contraction off) gives the oracle's FP32 cpuNaive loop
+    bit for bit."""
+    prob, x, rs = small_problem(seed=6)
+    q = rs.standard_normal(4).astype(np.float32)
+    q /= np.linalg.norm(q)
+    L, box, win, minc = prob.pose_box(x, q)
+    a = orc.correlate_window_full(prob.Pused, L, win)
+    b = orc.correlate_window_full_timed(prob.Pused, L, win, 3)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_dense_matches_fft_restatement():
