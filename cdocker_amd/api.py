@@ -460,7 +460,7 @@ class Server:
 
     def result(self, rid, wait_ms=-1):
         """Returns (energy, index, mincoor, latency_s) or None while pending."""
-        nconf, nquat = self.shapes[rid]
+        nconf, nquat = self.shapes.get(rid, (0, 0))  # unknown or collected ids: the library reports it
         e = np.empty((nconf, nquat), np.float32)
         idx = np.empty((nconf, nquat, 3), np.int32)
         mc = np.empty((nconf, nquat, 3), np.float32)

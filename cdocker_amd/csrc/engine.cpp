@@ -479,30 +479,78 @@ void enqueue_reference(Context& ctx, const Protein& prot, Prepared& t, EventTime
   }
 }
 
-// Poses per launch set: the fused working arrays of a set (ligand or product
-// spectra A, correlation T, the second T2 buffer of the overlapped atom form
-// when that form may run, rotated atoms and placements) within ws_budget.
-size_t fused_chunk(Context& ctx, const Protein& prot, const Prepared& t) {
-  const int N = prot.N, ZH = N / 2 + 1;
-  const size_t plane = (size_t)N * ZH;
-  const bool overlap_possible = ctx.overlap_z && !ctx.time_stages && atoms_supported(N) &&
-                                prot.spec_mf.p != nullptr && (ctx.pipeline < 0 || ctx.pipeline == kPipeAtoms);
-  const size_t per_slot = std::max((size_t)t.cmax * t.boxmax * plane * sizeof(float2), atoms_work_bytes(N)) +
-                          (size_t)N * plane * sizeof(float2) * (overlap_possible ? 2 : 1) +
-                          (size_t)t.astride * (3 * sizeof(float) + sizeof(AtomGrid));
-  size_t chunk = ctx.fused_chunk > 0 ? (size_t)ctx.fused_chunk : std::max<size_t>(1, ctx.ws_budget / per_slot);
-  return std::min(chunk, t.slots.size());
+// Whether a fused search on this protein may run the atom form with its
+// z-inverse stage overlapped (two T2 buffers).
+bool overlap_possible(const Context& ctx, const Protein& prot) {
+  return ctx.overlap_z && !ctx.time_stages && atoms_supported(prot.N) && prot.spec_mf.p != nullptr &&
+         (ctx.pipeline < 0 || ctx.pipeline == kPipeAtoms);
 }
 
-void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, size_t chunk) {
+// Bytes per pose of each launch-set working array: rotated atoms, atom
+// placements, A (the plane and column forms' ligand spectra, or the atom
+// form's product spectra at grid edges 96 and 128), T (correlation spectra,
+// the atom form's T2), and the overlapped atom form's second T2 buffer when
+// that form may run.
+void per_pose_bytes(const Context& ctx, const Protein& prot, const Prepared& t, size_t (&s)[5]) {
   const int N = prot.N, ZH = N / 2 + 1;
   const size_t plane = (size_t)N * ZH;
-  ensure_work(ctx, chunk * t.astride * 3 * sizeof(float), ctx.d_rot);
-  ensure_work(ctx, chunk * t.astride * sizeof(AtomGrid), ctx.d_ag);
-  // the plane and column forms' ligand spectra, or the atom form's product
-  // spectra at grid edge 96
-  ensure_work(ctx, chunk * std::max(t.cmax * t.boxmax * plane * sizeof(float2), atoms_work_bytes(N)), ctx.d_A);
-  ensure_work(ctx, chunk * N * plane * sizeof(float2), ctx.d_T);
+  s[0] = (size_t)t.astride * 3 * sizeof(float);
+  s[1] = (size_t)t.astride * sizeof(AtomGrid);
+  s[2] = std::max((size_t)t.cmax * t.boxmax * plane * sizeof(float2), atoms_work_bytes(N));
+  s[3] = (size_t)N * plane * sizeof(float2);
+  s[4] = overlap_possible(ctx, prot) ? s[3] : 0;
+}
+
+std::array<DevBuf*, 5> work_arrays(Context& ctx) { return {&ctx.d_rot, &ctx.d_ag, &ctx.d_A, &ctx.d_T, &ctx.d_T2b}; }
+
+// Poses per launch set: the largest count whose working arrays fit
+// ws_budget together with what the arrays already hold (they grow only, so
+// searches on grids of other sizes reuse them instead of reallocating; the
+// budget then bounds the arrays' total, and a launch set may be smaller than
+// the budget alone would allow).
+size_t fused_chunk(Context& ctx, const Protein& prot, const Prepared& t) {
+  size_t s[5];
+  per_pose_bytes(ctx, prot, t, s);
+  if (ctx.fused_chunk > 0) return std::min((size_t)ctx.fused_chunk, t.slots.size());
+  const auto b = work_arrays(ctx);
+  size_t cap[5];
+  for (int i = 0; i < 5; ++i) cap[i] = b[i]->cap;
+  if (s[4] == 0) cap[4] = 0;  // given back before the others grow (ensure_fused_work)
+  auto total = [&](size_t c) {
+    size_t sum = 0;
+    for (int i = 0; i < 5; ++i) sum += std::max(cap[i], c * s[i]);
+    return sum;
+  };
+  size_t per = 0;
+  for (int i = 0; i < 5; ++i) per += s[i];
+  size_t hi = std::max<size_t>(1, ctx.ws_budget / std::max<size_t>(per, 1));
+  hi = std::min(hi, std::max<size_t>(t.slots.size(), 1));
+  size_t lo = 1;
+  if (total(hi) <= ctx.ws_budget) return hi;
+  // total(c) is non-decreasing in c: the largest c with total(c) <= budget
+  while (lo < hi) {
+    const size_t mid = (lo + hi + 1) / 2;
+    if (total(mid) <= ctx.ws_budget)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// The launch-set working arrays for `chunk` poses (grow only; fused_chunk
+// keeps their total within ws_budget).  The overlapped atom form's second T2
+// buffer is allocated after tuning (ensure_overlap) and given back here when
+// that form cannot run.
+void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, size_t chunk) {
+  size_t s[5];
+  per_pose_bytes(ctx, prot, t, s);
+  if (s[4] == 0 && ctx.d_T2b.cap > 0) {
+    ctx.d_T2b.release();
+    ++ctx.ws_generation;
+  }
+  const auto b = work_arrays(ctx);
+  for (int i = 0; i < 4; ++i) ensure_work(ctx, chunk * s[i], *b[i]);
 }
 
 struct Tuned {
@@ -908,6 +956,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
 
 Prepared::~Prepared() {
   drop_graph();
+  if (h_up) (void)hipHostFree(h_up);
   if (h_keys) (void)hipHostFree(h_keys);
   if (h_meta) (void)hipHostFree(h_meta);
   if (start) (void)hipEventDestroy(start);
@@ -943,13 +992,52 @@ namespace {
 
 // t's tables to the device on the context stream (context lock held).
 void upload_tables(Context& ctx, Prepared& t) {
-  upload(ctx, t.d_ligs, t.ligs);
-  upload(ctx, t.d_chan, t.chan);
-  upload(ctx, t.d_pad, t.pad);
-  upload(ctx, t.d_wv, t.wv);
-  upload(ctx, t.d_wq, t.wq);
-  upload(ctx, t.d_coords, t.coords);
-  upload(ctx, t.d_slots, t.slots);
+  if (t.own_staging) {
+    // Pipelined callers: the tables go through this search's own pinned
+    // staging, so the copies are asynchronous (a copy from pageable memory
+    // would hold the host until the stream's earlier work is done).  The
+    // staging is rewritten only after finish_search of this search.
+    struct Part {
+      DevBuf* dst;
+      const void* src;
+      size_t bytes;
+    };
+    const Part parts[7] = {{&t.d_ligs, t.ligs.data(), t.ligs.size() * sizeof(LigandDesc)},
+                           {&t.d_chan, t.chan.data(), t.chan.size() * sizeof(int)},
+                           {&t.d_pad, t.pad.data(), t.pad.size() * sizeof(int)},
+                           {&t.d_wv, t.wv.data(), t.wv.size() * sizeof(float)},
+                           {&t.d_wq, t.wq.data(), t.wq.size() * sizeof(float)},
+                           {&t.d_coords, t.coords.data(), t.coords.size() * sizeof(float)},
+                           {&t.d_slots, t.slots.data(), t.slots.size() * sizeof(PoseSlot)}};
+    size_t off[7], total = 0;
+    for (int i = 0; i < 7; ++i) {
+      off[i] = total;
+      total += (parts[i].bytes + 255) & ~(size_t)255;
+    }
+    if (total > t.h_up_cap) {
+      if (t.h_up) (void)hipHostFree(t.h_up);
+      t.h_up = nullptr;
+      t.h_up_cap = 0;
+      hip_check(hipHostMalloc(&t.h_up, total), "hipHostMalloc");
+      t.h_up_cap = total;
+    }
+    char* h = static_cast<char*>(t.h_up);
+    for (int i = 0; i < 7; ++i) {
+      parts[i].dst->ensure(std::max<size_t>(parts[i].bytes, 4));
+      if (parts[i].bytes == 0) continue;
+      std::memcpy(h + off[i], parts[i].src, parts[i].bytes);
+      hip_check(hipMemcpyAsync(parts[i].dst->p, h + off[i], parts[i].bytes, hipMemcpyHostToDevice, ctx.stream),
+                "upload");
+    }
+  } else {
+    upload(ctx, t.d_ligs, t.ligs);
+    upload(ctx, t.d_chan, t.chan);
+    upload(ctx, t.d_pad, t.pad);
+    upload(ctx, t.d_wv, t.wv);
+    upload(ctx, t.d_wq, t.wq);
+    upload(ctx, t.d_coords, t.coords);
+    upload(ctx, t.d_slots, t.slots);
+  }
   const size_t total = std::max<size_t>(t.slots.size(), 1);
   t.d_keys.ensure(total * sizeof(unsigned long long));
   t.d_meta.ensure(total * sizeof(RotMeta));
@@ -970,10 +1058,12 @@ namespace {
 // Enqueues t's launch sequence and the copies of its keys and per-pose
 // metadata to host staging (t's own when t.own_staging, else the context's);
 // context lock held.
-void enqueue_search(Context& ctx, Prepared& t, Engine engine, bool use_graph, EventTimer& timer) {
+void enqueue_search(Context& ctx, Prepared& t, Engine engine, bool use_graph, EventTimer& timer,
+                    hipEvent_t start = nullptr) {
   const Protein& prot = *t.prot;
   const size_t total = t.slots.size();
   if (engine == kEngineReference) {
+    if (start) hip_check(hipEventRecord(start, ctx.stream), "hipEventRecord");
     enqueue_reference(ctx, prot, t, timer);
   } else {
     const size_t chunk = fused_chunk(ctx, prot, t);
@@ -988,7 +1078,13 @@ void enqueue_search(Context& ctx, Prepared& t, Engine engine, bool use_graph, Ev
     ctx.last_poses = tu.poses;
     const int ovl = overlap_z_stage(ctx, tu) ? 1 : 0;
     if (ovl) ensure_overlap(ctx, prot, t, chunk);
-    if (use_graph && !ctx.time_stages) {
+    if (start) hip_check(hipEventRecord(start, ctx.stream), "hipEventRecord");
+    // Jobs of more launch sets than kMaxGraphSets run eagerly: a host
+    // segmentation fault inside the search of a 3.28 M-pose uneven batch
+    // captured as one graph of 3 410 launch sets (3 304 had passed) has no
+    // cause found in this code, so graphs stay at sizes that have run.
+    const size_t nsets = (total + chunk - 1) / chunk;
+    if (use_graph && !ctx.time_stages && nsets <= kMaxGraphSets) {
       if (!t.graph || t.graph_gen != ctx.ws_generation || t.graph_chunk != chunk || t.graph_group != tu.mid_group() ||
           t.graph_fwd != tu.fwd || t.graph_inv != tu.inv || t.graph_depth != tu.depth ||
           t.graph_poses != tu.poses || t.graph_pipe != tu.pipe || t.graph_products != ctx.atom_products ||
@@ -1110,13 +1206,14 @@ void submit_search(Context& ctx, Prepared& t) {
   if (!t.done) hip_check(hipEventCreate(&t.done), "hipEventCreate");
   t.own_staging = true;
   upload_tables(ctx, t);
-  hip_check(hipEventRecord(t.start, ctx.stream), "hipEventRecord");
-  if (!t.slots.empty()) {
+  if (t.slots.empty()) {
+    hip_check(hipEventRecord(t.start, ctx.stream), "hipEventRecord");
+  } else {
     EventTimer timer(ctx);  // stages are timed only by the synchronous path
     const bool ts = ctx.time_stages;
     ctx.time_stages = false;
     try {
-      enqueue_search(ctx, t, kEngineFused, false, timer);
+      enqueue_search(ctx, t, kEngineFused, false, timer, t.start);
     } catch (...) {
       ctx.time_stages = ts;
       throw;

@@ -144,11 +144,13 @@ struct Prepared {
   int graph_group = 0, graph_fwd = -1, graph_inv = -1, graph_depth = 0, graph_poses = 0, graph_pipe = -1,
       graph_products = -1, graph_overlap = -1;
   std::vector<PoseResults> results;
-  // Pipelined callers (the server): results are copied into this table's own
-  // pinned staging, and `done` marks the end of its launch sequence on the
-  // context stream (`start` its beginning), so the host can build the next
-  // table while this one runs.
+  // Pipelined callers (the server): the tables are uploaded from, and the
+  // results copied into, this table's own pinned staging, and `done` marks
+  // the end of its launch sequence on the context stream (`start` the first
+  // launch), so the host can build the next table while this one runs.
   bool own_staging = false;
+  void* h_up = nullptr;  // pinned staging of the tables' upload
+  size_t h_up_cap = 0;
   void* h_keys = nullptr;
   void* h_meta = nullptr;
   size_t h_cap = 0;
@@ -223,6 +225,10 @@ void protein_upload(Protein& p, int nch, int N, float h, const float* mid, const
 void protein_build(Protein& p, int natom, const double* xyz, const double* charge, const double* epsilon,
                    const double* radius, int nprobe, const float* probe_radii, int N, float h, const float* mid,
                    const float* len, double lj_cutoff, double elec_cutoff, double lj_cap, double elec_cap);
+
+// Largest job (in launch sets) captured as one hipGraph; larger jobs launch
+// eagerly.
+constexpr size_t kMaxGraphSets = 2048;
 
 // Builds and uploads the tables of a set of jobs (inputs resident in HBM).
 void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Prepared& t);
