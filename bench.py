@@ -258,12 +258,16 @@ def serve_leg(ctx, cdk, workloads, nreq, load, max_batch, seed):
         span = float(np.max(np.asarray(sub) + lat))
         return lat, span, st
 
+    log("serving: plain loop %s poses/s, fused alone %s poses/s; saturated pass" % (
+        [round(v) for v in plain], [round(v) for v in fused]))
     lat0, span0, st0 = run(np.zeros(nreq))
+    log("serving: saturated pass %.1f s for %d poses" % (span0, poses.sum()))
     cap_req = nreq / span0
     rate = load * cap_req
     arrivals = np.cumsum(rs.exponential(1.0 / rate, nreq))
     arrivals -= arrivals[0]
     lat, span, st = run(arrivals)
+    log("serving: Poisson pass %.1f s" % span)
     for p in prots:
         p.close()
     out = {

@@ -215,7 +215,7 @@ int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value) {
     else if (k == "last_mid_poses") *value = c.last_poses;
     else if (k == "max_dyn_lds") *value = (int64_t)c.max_dyn_lds;
     else if (k == "workspace_bytes")
-      *value = (int64_t)(c.d_rot.cap + c.d_ag.cap + c.d_A.cap + c.d_T.cap + c.d_T2b.cap);
+      *value = (int64_t)c.d_pool.cap;
     else throw cdk::Error(CDK_ERR_INVALID, "unknown option " + k);
   });
 }

@@ -415,6 +415,33 @@ void ensure_work(Context& ctx, size_t bytes_needed, DevBuf& b) {
   ++ctx.ws_generation;
 }
 
+// Carves the five launch-set arrays for `chunk` poses (bytes per pose
+// per[i]: rotated atoms, placements, A, T, second T2) from the pool, each
+// 256-byte aligned, growing the pool if needed.
+void carve_work(Context& ctx, size_t chunk, const size_t (&per)[5]) {
+  size_t off[5], end = 0;
+  for (int i = 0; i < 5; ++i) {
+    off[i] = end;
+    end += (chunk * per[i] + 255) & ~(size_t)255;
+  }
+  if (ctx.d_pool.cap < end) {
+    ctx.d_pool.ensure(end);
+    ++ctx.ws_generation;
+  }
+  char* base = static_cast<char*>(ctx.d_pool.p);
+  float* rot = reinterpret_cast<float*>(base + off[0]);
+  AtomGrid* ag = reinterpret_cast<AtomGrid*>(base + off[1]);
+  float2* A = reinterpret_cast<float2*>(base + off[2]);
+  float2* T = reinterpret_cast<float2*>(base + off[3]);
+  float2* T2b = per[4] ? reinterpret_cast<float2*>(base + off[4]) : nullptr;
+  if (rot != ctx.w_rot || ag != ctx.w_ag || A != ctx.w_A || T != ctx.w_T || T2b != ctx.w_T2b) ++ctx.ws_generation;
+  ctx.w_rot = rot;
+  ctx.w_ag = ag;
+  ctx.w_A = A;
+  ctx.w_T = T;
+  ctx.w_T2b = T2b;
+}
+
 // Reference dataflow: one launch sequence per batch of <= B rotations of one
 // conformer (TranRotaConfSearchNoMini.cu:385-464).
 void enqueue_reference(Context& ctx, const Protein& prot, Prepared& t, EventTimer& timer) {
@@ -423,8 +450,8 @@ void enqueue_reference(Context& ctx, const Protein& prot, Prepared& t, EventTime
   const float2* tw = ctx.twiddle(N);
   const int B = std::max(1, ctx.ref_batch);
   const int C = t.cmax;
-  ensure_work(ctx, (size_t)B * t.astride * 3 * sizeof(float), ctx.d_rot);
-  ensure_work(ctx, (size_t)B * t.astride * sizeof(AtomGrid), ctx.d_ag);
+  const size_t per[5] = {(size_t)t.astride * 3 * sizeof(float), (size_t)t.astride * sizeof(AtomGrid), 0, 0, 0};
+  carve_work(ctx, (size_t)B, per);
   ensure_work(ctx, (size_t)B * C * G * sizeof(float), ctx.d_L);
   ensure_work(ctx, (size_t)B * C * H * sizeof(float2), ctx.d_Lhat);
   ensure_work(ctx, (size_t)B * H * sizeof(float2), ctx.d_S);
@@ -448,12 +475,12 @@ void enqueue_reference(Context& ctx, const Protein& prot, Prepared& t, EventTime
     hipEvent_t e;
     e = timer.begin(0);
     hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, N,
-                          ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
+                          ctx.w_rot, ctx.w_ag, d_meta + s, d_keys + s),
               "prep");
     timer.end(e);
     e = timer.begin(1);
     hip_check(launch_spread_full(ctx.stream, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(), t.d_wq.as<float>(),
-                                 ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, N, nch, ctx.d_L.as<float>()),
+                                 ctx.w_ag, t.astride, d_meta + s, N, nch, ctx.d_L.as<float>()),
               "spread");
     timer.end(e);
     e = timer.begin(2);
@@ -501,56 +528,27 @@ void per_pose_bytes(const Context& ctx, const Protein& prot, const Prepared& t, 
   s[4] = overlap_possible(ctx, prot) ? s[3] : 0;
 }
 
-std::array<DevBuf*, 5> work_arrays(Context& ctx) { return {&ctx.d_rot, &ctx.d_ag, &ctx.d_A, &ctx.d_T, &ctx.d_T2b}; }
-
-// Poses per launch set: the largest count whose working arrays fit
-// ws_budget together with what the arrays already hold (they grow only, so
-// searches on grids of other sizes reuse them instead of reallocating; the
-// budget then bounds the arrays' total, and a launch set may be smaller than
-// the budget alone would allow).
+// Poses per launch set: as many as the working arrays of the search allow
+// within ws_budget (or the forced fused_chunk), at most the search's poses.
 size_t fused_chunk(Context& ctx, const Protein& prot, const Prepared& t) {
+  if (ctx.fused_chunk > 0) return std::min((size_t)ctx.fused_chunk, std::max<size_t>(t.slots.size(), 1));
   size_t s[5];
   per_pose_bytes(ctx, prot, t, s);
-  if (ctx.fused_chunk > 0) return std::min((size_t)ctx.fused_chunk, t.slots.size());
-  const auto b = work_arrays(ctx);
-  size_t cap[5];
-  for (int i = 0; i < 5; ++i) cap[i] = b[i]->cap;
-  if (s[4] == 0) cap[4] = 0;  // given back before the others grow (ensure_fused_work)
-  auto total = [&](size_t c) {
-    size_t sum = 0;
-    for (int i = 0; i < 5; ++i) sum += std::max(cap[i], c * s[i]);
-    return sum;
-  };
   size_t per = 0;
   for (int i = 0; i < 5; ++i) per += s[i];
-  size_t hi = std::max<size_t>(1, ctx.ws_budget / std::max<size_t>(per, 1));
-  hi = std::min(hi, std::max<size_t>(t.slots.size(), 1));
-  size_t lo = 1;
-  if (total(hi) <= ctx.ws_budget) return hi;
-  // total(c) is non-decreasing in c: the largest c with total(c) <= budget
-  while (lo < hi) {
-    const size_t mid = (lo + hi + 1) / 2;
-    if (total(mid) <= ctx.ws_budget)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
+  // each array's start is 256-byte aligned: 5 x 256 bytes of slack
+  const size_t room = ctx.ws_budget > 5 * 256 ? ctx.ws_budget - 5 * 256 : 0;
+  const size_t chunk = std::max<size_t>(1, room / std::max<size_t>(per, 1));
+  return std::min(chunk, std::max<size_t>(t.slots.size(), 1));
 }
 
-// The launch-set working arrays for `chunk` poses (grow only; fused_chunk
-// keeps their total within ws_budget).  The overlapped atom form's second T2
-// buffer is allocated after tuning (ensure_overlap) and given back here when
-// that form cannot run.
+// The launch-set working arrays for `chunk` poses carved from the pool
+// (grown when a search needs more; the pool is never larger than the largest
+// search's need, so at most ws_budget).
 void ensure_fused_work(Context& ctx, const Protein& prot, const Prepared& t, size_t chunk) {
   size_t s[5];
   per_pose_bytes(ctx, prot, t, s);
-  if (s[4] == 0 && ctx.d_T2b.cap > 0) {
-    ctx.d_T2b.release();
-    ++ctx.ws_generation;
-  }
-  const auto b = work_arrays(ctx);
-  for (int i = 0; i < 4; ++i) ensure_work(ctx, chunk * s[i], *b[i]);
+  carve_work(ctx, chunk, s);
 }
 
 struct Tuned {
@@ -666,7 +664,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   unsigned long long* d_keys = t.d_keys.as<unsigned long long>();
   const float invG = 1.0f / (float)prot.G();
   hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, t.boxmax,
-                        ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta, d_keys),
+                        ctx.w_rot, ctx.w_ag, d_meta, d_keys),
             "tune prep");
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
@@ -700,19 +698,19 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   };
   auto fwd = [&](int variant) {
     hip_check(launch_fwd_planes(ctx.stream, N, variant, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
-                                t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, t.cmax, t.boxmax,
-                                t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
+                                t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta, t.cmax, t.boxmax,
+                                t.max_chan_atoms, ctx.w_A, tw),
               "tune fwd planes");
   };
   auto mid = [&](int g, int depth, int np) {
     return timed([&] {
-      hip_check(launch_mid(ctx.stream, N, depth, np, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax,
-                           t.boxmax, prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), invG, tw),
+      hip_check(launch_mid(ctx.stream, N, depth, np, sl, d_ligs, d_meta, n, g, ctx.w_A, t.cmax,
+                           t.boxmax, prot.spec_lb.as<float2>(), prot.nch, ctx.w_T, invG, tw),
                 "tune mid");
     });
   };
   auto inv = [&](int variant) {
-    hip_check(launch_inv_argmin(ctx.stream, N, variant, d_meta, n, N - 2, ctx.d_T.as<float2>(), d_keys, tw),
+    hip_check(launch_inv_argmin(ctx.stream, N, variant, d_meta, n, N - 2, ctx.w_T, d_keys, tw),
               "tune inverse");
   };
   float plane_ms = 0.f, col_ms = 0.f, atom_ms = 0.f;
@@ -776,14 +774,14 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
     if (need_col || (need_pipe && cols)) {
       col_ms += timed([&] {
         hip_check(launch_fwd_z(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
-                               t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, t.cmax, t.boxmax,
-                               t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
+                               t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta, t.cmax, t.boxmax,
+                               t.max_chan_atoms, ctx.w_A, tw),
                   "tune column forward");
       });
       auto midc = [&](int g) {
         return timed([&] {
-          hip_check(launch_mid_kz(ctx.stream, N, sl, d_ligs, d_meta, n, g, ctx.d_A.as<float2>(), t.cmax, t.boxmax,
-                                  prot.spec_kz.as<float2>(), ctx.d_T.as<float2>(), invG, tw),
+          hip_check(launch_mid_kz(ctx.stream, N, sl, d_ligs, d_meta, n, g, ctx.w_A, t.cmax, t.boxmax,
+                                  prot.spec_kz.as<float2>(), ctx.w_T, invG, tw),
                     "tune column mid");
         });
       };
@@ -794,15 +792,15 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
         col_ms += midc(tu.col_group);
       }
       col_ms += timed([&] {
-        hip_check(launch_inv_z(ctx.stream, N, d_meta, n, ctx.d_T.as<float2>(), d_keys, tw), "tune column inverse");
+        hip_check(launch_inv_z(ctx.stream, N, d_meta, n, ctx.w_T, d_keys, tw), "tune column inverse");
       });
     }
     if (need_atom || (need_pipe && atoms)) {
       auto mida = [&](int g) {
         return timed([&] {
           hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_pad.as<int>(), t.d_wv.as<float>(),
-                                     t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta, n, g,
-                                     prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), ctx.d_T.as<float2>(), invG,
+                                     t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta, n, g,
+                                     prot.spec_mf.as<float2>(), ctx.w_A, ctx.w_T, invG,
                                      tw, ctx.atom_products),
                     "tune atom mid");
         });
@@ -814,7 +812,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
         atom_ms += mida(tu.atom_group);
       }
       atom_ms += timed([&] {
-        hip_check(launch_inv_z(ctx.stream, N, d_meta, n, ctx.d_T.as<float2>(), d_keys, tw), "tune atom inverse");
+        hip_check(launch_inv_z(ctx.stream, N, d_meta, n, ctx.w_T, d_keys, tw), "tune atom inverse");
       });
     }
     if (need_pipe) {
@@ -846,11 +844,10 @@ bool overlap_z_stage(const Context& ctx, const Tuned& tu) {
   return tu.pipe == kPipeAtoms && ctx.overlap_z && !ctx.time_stages;
 }
 
-// The second T2 buffer and the ordering events of an overlapped search, made
-// before any capture.
-void ensure_overlap(Context& ctx, const Protein& prot, const Prepared& t, size_t chunk) {
-  const int N = prot.N;
-  ensure_work(ctx, chunk * N * (size_t)N * (N / 2 + 1) * sizeof(float2), ctx.d_T2b);
+// The ordering events of an overlapped search (its second T2 buffer is
+// carved with the other arrays), made before any capture.
+void ensure_overlap(Context& ctx, const Prepared& t, size_t chunk) {
+  if (!ctx.w_T2b) throw Error(kErrInvalid, "overlapped z-inverse stage without its second buffer");
   const size_t sets = (t.slots.size() + chunk - 1) / chunk;
   while (ctx.set_events.size() < 2 * sets) {
     hipEvent_t e = nullptr;
@@ -881,18 +878,18 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
     hipEvent_t e;
     e = timer.begin(0);
     hip_check(launch_prep(ctx.stream, sl, n, d_ligs, t.d_coords.as<float>(), N, prot.h, t.astride, t.boxmax,
-                          ctx.d_rot.as<float>(), ctx.d_ag.as<AtomGrid>(), d_meta + s, d_keys + s),
+                          ctx.w_rot, ctx.w_ag, d_meta + s, d_keys + s),
               "prep");
     timer.end(e);
     if (tu.pipe == kPipeAtoms) {
       // overlapped: T2 alternates between two buffers; set si's mid stage
       // waits for set si - 2's z-inverse, the last reader of its buffer
-      float2* T2 = ovl && (si & 1) ? ctx.d_T2b.as<float2>() : ctx.d_T.as<float2>();
+      float2* T2 = ovl && (si & 1) ? ctx.w_T2b : ctx.w_T;
       if (ovl && si >= 2) hip_check(hipStreamWaitEvent(ctx.stream, ctx.set_events[2 * (si - 2) + 1], 0), "wait");
       e = timer.begin(2);
       hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_pad.as<int>(), t.d_wv.as<float>(),
-                                 t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, n, tu.atom_group,
-                                 prot.spec_mf.as<float2>(), ctx.d_A.as<float2>(), T2, 1.0f / (float)G, tw,
+                                 t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta + s, n, tu.atom_group,
+                                 prot.spec_mf.as<float2>(), ctx.w_A, T2, 1.0f / (float)G, tw,
                                  ctx.atom_products),
                 "atom mid");
       timer.end(e);
@@ -912,35 +909,35 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
     if (tu.pipe == kPipeColumns) {
       e = timer.begin(1);
       hip_check(launch_fwd_z(ctx.stream, N, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
-                             t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, t.cmax, t.boxmax,
-                             t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
+                             t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta + s, t.cmax, t.boxmax,
+                             t.max_chan_atoms, ctx.w_A, tw),
                 "column forward");
       timer.end(e);
       e = timer.begin(2);
-      hip_check(launch_mid_kz(ctx.stream, N, sl, d_ligs, d_meta + s, n, tu.col_group, ctx.d_A.as<float2>(), t.cmax,
-                              t.boxmax, prot.spec_kz.as<float2>(), ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+      hip_check(launch_mid_kz(ctx.stream, N, sl, d_ligs, d_meta + s, n, tu.col_group, ctx.w_A, t.cmax,
+                              t.boxmax, prot.spec_kz.as<float2>(), ctx.w_T, 1.0f / (float)G, tw),
                 "column mid");
       timer.end(e);
       e = timer.begin(3);
-      hip_check(launch_inv_z(ctx.stream, N, d_meta + s, n, ctx.d_T.as<float2>(), d_keys + s, tw),
+      hip_check(launch_inv_z(ctx.stream, N, d_meta + s, n, ctx.w_T, d_keys + s, tw),
                 "column inverse + argmin");
       timer.end(e);
       continue;
     }
     e = timer.begin(1);
     hip_check(launch_fwd_planes(ctx.stream, N, tu.fwd, sl, n, d_ligs, t.d_chan.as<int>(), t.d_wv.as<float>(),
-                                t.d_wq.as<float>(), ctx.d_ag.as<AtomGrid>(), t.astride, d_meta + s, t.cmax, t.boxmax,
-                                t.max_chan_atoms, ctx.d_A.as<float2>(), tw),
+                                t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta + s, t.cmax, t.boxmax,
+                                t.max_chan_atoms, ctx.w_A, tw),
               "fwd planes");
     timer.end(e);
     e = timer.begin(2);
-    hip_check(launch_mid(ctx.stream, N, tu.depth, tu.poses, sl, d_ligs, d_meta + s, n, group, ctx.d_A.as<float2>(), t.cmax,
+    hip_check(launch_mid(ctx.stream, N, tu.depth, tu.poses, sl, d_ligs, d_meta + s, n, group, ctx.w_A, t.cmax,
                          t.boxmax,
-                         prot.spec_lb.as<float2>(), prot.nch, ctx.d_T.as<float2>(), 1.0f / (float)G, tw),
+                         prot.spec_lb.as<float2>(), prot.nch, ctx.w_T, 1.0f / (float)G, tw),
               "mid");
     timer.end(e);
     e = timer.begin(3);
-    hip_check(launch_inv_argmin(ctx.stream, N, tu.inv, d_meta + s, n, N - 2, ctx.d_T.as<float2>(), d_keys + s, tw),
+    hip_check(launch_inv_argmin(ctx.stream, N, tu.inv, d_meta + s, n, N - 2, ctx.w_T, d_keys + s, tw),
               "inverse + argmin");
     timer.end(e);
   }
@@ -1077,7 +1074,7 @@ void enqueue_search(Context& ctx, Prepared& t, Engine engine, bool use_graph, Ev
     ctx.last_depth = tu.depth;
     ctx.last_poses = tu.poses;
     const int ovl = overlap_z_stage(ctx, tu) ? 1 : 0;
-    if (ovl) ensure_overlap(ctx, prot, t, chunk);
+    if (ovl) ensure_overlap(ctx, t, chunk);
     if (start) hip_check(hipEventRecord(start, ctx.stream), "hipEventRecord");
     // Jobs of more launch sets than kMaxGraphSets run eagerly: a host
     // segmentation fault inside the search of a 3.28 M-pose uneven batch

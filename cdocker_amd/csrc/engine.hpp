@@ -202,10 +202,20 @@ struct Context {
   int last_fwd = -1, last_inv = -1, last_group = 0, last_depth = 0, last_poses = 0,
       last_pipe = -1;
   StageTimes stage_times;
-  // per-launch-set working arrays; ws_generation counts reallocations so a
-  // captured graph that baked in the old addresses is rebuilt
-  DevBuf d_rot, d_ag;
-  DevBuf d_A, d_T, d_T2b, d_L, d_Lhat, d_S, d_E;
+  // Launch-set working arrays: one pool, never above ws_budget (unless
+  // fused_chunk is forced), carved per search into the rotated atoms, atom
+  // placements, A (ligand or product spectra), T (correlation spectra, the
+  // atom form's T2) and the overlapped atom form's second T2 buffer.
+  // ws_generation counts reallocations and re-carvings, so a captured graph
+  // that baked in the old addresses is rebuilt.
+  DevBuf d_pool;
+  float* w_rot = nullptr;
+  AtomGrid* w_ag = nullptr;
+  float2* w_A = nullptr;
+  float2* w_T = nullptr;
+  float2* w_T2b = nullptr;
+  // the reference dataflow's batch arrays
+  DevBuf d_L, d_Lhat, d_S, d_E;
   uint64_t ws_generation = 1;
   // pinned host staging
   void* h_keys = nullptr;
