@@ -110,6 +110,7 @@ _SIGS = {
     "cdk_rotate_host": [I, I, FP, FP, FP],
     "cdk_min_max_coors": [I, I, FP, FP, FP],
     "cdk_quaternion_uniform_sampling_mt": [P, I, FP],
+    "cdk_argmin_window_noalloc": [P, I, I, P, P, P, P, P],
     "cdk_rng_create": [U64, PP],
     "cdk_rng_destroy": [P],
     "cdk_quaternion_uniform_sampling": [P, I, FP],

@@ -13,6 +13,7 @@
 #include "engine.hpp"
 #include "host_inputs.hpp"
 #include "kernels.hpp"
+#include "noalloc.hpp"
 #include "server.hpp"
 
 struct cdk_context {
@@ -800,6 +801,23 @@ int cdk_argmin_window(cdk_context* ctx, int nq, int n, const float* grids, const
         index[q * 3 + 2] = (int)(lin % (unsigned)n);
       }
     }
+  });
+}
+
+int cdk_argmin_window_noalloc(cdk_context* ctx, int nq, int n, const float* grids, const int* windows,
+                              float* energy, int* index, void* stream) {
+  return guard([&] {
+    require(ctx != nullptr && n > 0 && nq >= 0, "bad arguments");
+    require(nq == 0 || (grids && windows && energy && index), "NULL buffer");
+    if (nq > cdk::kNoAllocMaxPoses)
+      throw cdk::Error(CDK_ERR_INVALID, "cdk_argmin_window_noalloc: nq exceeds the fixed maximum of " +
+                                            std::to_string(cdk::kNoAllocMaxPoses) + " poses per call");
+    cdk::Context& c = *ctx->c;
+    std::lock_guard<std::mutex> lk(c.mu);
+    cdk::hip_check(hipSetDevice(c.device), "hipSetDevice");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+    cdk::hip_check(cdk::argmin_window_noalloc(c.device, st, nq, n, grids, windows, energy, index), "argmin");
+    if (!stream) cdk::hip_check(hipStreamSynchronize(st), "argmin sync");
   });
 }
 

@@ -10,6 +10,7 @@
 #include "conformers.hpp"
 #include "gridbuild.hpp"
 #include "kernels.hpp"
+#include "noalloc.hpp"
 #include "rescore.hpp"
 
 namespace cdk {
@@ -48,6 +49,8 @@ Context::Context(int dev) : device(dev) {
   hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
   max_dyn_lds = (size_t)lds;
   hip_check(configure_fwd_reg(max_dyn_lds), "configure forward kernels");
+  // the device address of this device's fixed no-alloc array, once per device
+  hip_check(noalloc_init(dev), "no-alloc scratch address");
 }
 
 Context::~Context() {

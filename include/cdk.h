@@ -284,6 +284,19 @@ int cdk_sum_grids(cdk_context* ctx, int nq, int nch, int64_t H, int64_t G, const
 int cdk_argmin_window(cdk_context* ctx, int nq, int n, const float* grids, const int* windows, float* energy,
                       int* index);
 
+/* No reference counterpart: cdk_argmin_window for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Every pointer is device memory: grids
+ * [nq][n][n][n], windows [nq][3] in, energy [nq] and index [nq][3] out.  The
+ * per-pose window metadata and argmin keys live in a fixed per-device array
+ * (72 B per pose, at most 16384 poses per call: CDK_ERR_INVALID beyond, with
+ * nothing launched); the call allocates nothing.  stream: a hipStream_t to
+ * enqueue on (the call returns without waiting), or NULL for the context
+ * stream (the call returns when the outputs are written).  Calls that overlap
+ * in time on one device share that array and race. */
+int cdk_argmin_window_noalloc(cdk_context* ctx, int nq, int n, const float* grids, const int* windows,
+                              float* energy, int* index, void* stream);
+
 /* ---- host-side input producers --------------------------------------- */
 /* std::mt19937_64 generator, as the reference's drivers use (NoMini:203-204,
  * here with an explicit seed). */
