@@ -168,16 +168,17 @@ def profiled_traffic(kernel, poses_per_launch):
     return None
 
 
-def profiled_issue(kernel):
+def profiled_issue(kernel, products):
     """The SIMD-cycle fractions (vector-instruction issue, matrix-core busy)
-    of `kernel` from the committed rocprofv3 SQ-counter summary of the
-    single-ligand workload; None if absent."""
+    of `kernel` with the atom form's product form `products` from the
+    committed rocprofv3 SQ-counter summary of the single-ligand workload;
+    None if absent."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for ln in f:
                 if ln.startswith("issue-per-simd-cycle:"):
                     kv = dict(t.split("=") for t in ln.split(":", 1)[1].split())
-                    if kv.get("kernel") == kernel:
+                    if kv.get("kernel") == kernel and kv.get("products") == str(products):
                         return {"valu_issue": float(kv["valu_issue"]), "mfma_busy": float(kv["mfma_busy"]),
                                 "source": "profiles/traffic_single_ligand_64.md (rocprofv3 SQ counters)"}
     except OSError:
@@ -536,9 +537,11 @@ def run_single(args, cdk, workloads):
     if tr is not None:
         roof["traffic"] = round(tr)
         roof["traffic_source"] = "profiles/traffic_single_ligand_64.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per pose)"
-    iss = profiled_issue(roof["kernel"]) if args.protein == "p64" and roof.get("products", "").startswith("split") else None
+    iss = (profiled_issue(roof["kernel"], ctx.get_option("atom_products"))
+           if args.protein == "p64" and roof["kernel"] == "mid_atoms" else None)
     if iss is not None:
-        # what binds the kernel: vector-instruction issue, not the matrix cores
+        # what binds the kernel: vector-instruction issue plus matrix-core
+        # busy cycles, which barely overlap
         roof["issue"] = iss
 
     pc = max(1, min(args.plain_conformers, args.conformers))

@@ -578,7 +578,13 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
   const bool reg_ok = inv_reg_supported(N);
   const bool col_ok = col_supported(N) && prot.spec_kz.p != nullptr;
   const bool atoms_ok = atoms_supported(N) && prot.spec_mf.p != nullptr;
-  const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)chunk};
+  // The plane form's schedule is kept per (N, channels, box bound, launch
+  // size rounded up to a power of two): a server's launch sets vary in size
+  // from set to set, and a key per exact size would re-run the tuner on
+  // nearly every set.
+  size_t chunk_class = 1;
+  while (chunk_class < chunk) chunk_class <<= 1;
+  const std::array<int, 4> gkey = {N, t.cmax, t.boxmax, (int)std::min<size_t>(chunk_class, 1 << 30)};
   // The column and atom forms' group choice is kept per (N, channels): their
   // best group hardly moves with the box bound or the launch size, and a
   // server sees a new (box bound, chunk) pair on many launch sets.
@@ -759,6 +765,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
               tu.poses = np;
             }
           }
+        if (ctx.tuned_group.size() >= kMaxTunedShapes) ctx.tuned_group.erase(ctx.tuned_group.begin());
         ctx.tuned_group[gkey] = {tu.group, tu.depth, tu.poses};
         plane_ms += best;
       } else {

@@ -195,7 +195,8 @@ struct Context {
   std::map<int, int> tuned_pipe;                     // N -> pipeline form
   std::map<int, int> tuned_fwd;                      // N -> variant
   std::map<int, int> tuned_inv;                      // N -> variant
-  // (N, cmax, boxmax, chunk) -> (group, depth, poses per protein load)
+  // (N, cmax, boxmax, chunk rounded up to a power of two) -> (group, depth,
+  // poses per protein load)
   std::map<std::array<int, 4>, std::array<int, 3>> tuned_group;
   std::map<std::array<int, 4>, int> tuned_col_group;   // (N, cmax, 0, 0) -> column form group
   std::map<std::array<int, 4>, int> tuned_atom_group;  // (N, cmax, 0, 0) -> atom form group
@@ -239,6 +240,9 @@ void protein_build(Protein& p, int natom, const double* xyz, const double* charg
 // Largest job (in launch sets) captured as one hipGraph; larger jobs launch
 // eagerly.
 constexpr size_t kMaxGraphSets = 2048;
+// Plane-form schedules remembered per context; beyond this many the map drops
+// its smallest key before adding one.
+constexpr size_t kMaxTunedShapes = 256;
 
 // Builds and uploads the tables of a set of jobs (inputs resident in HBM).
 void prepare_search(Context& ctx, const Protein& prot, const std::vector<Job>& jobs, Prepared& t);

@@ -208,7 +208,8 @@ def test_server_alone_equals_in_mix(ctx, prots):
 def test_workspace_within_budget_with_overlap(prots):
     """The launch-set working arrays, including the overlapped atom form's
     second T2 buffer, stay within the context's workspace budget, with the
-    overlap on and off."""
+    overlap on and off, and across searches that alternate the 64^3, 96^3 and
+    128^3 grids in one context (one pool, re-carved per search)."""
     say("workspace budget")
     g, prot0 = prots[1]
     budget = 1 << 30
@@ -217,6 +218,7 @@ def test_workspace_within_budget_with_overlap(prots):
     w = workloads.config2(nconf=2, nquat=1500)
     L = w["ligands"][0]
     lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    big = []
     try:
         for ovl, pipeline in ((1, 2), (0, 2), (1, -1), (1, 0)):
             c.set_option("overlap_z", ovl)
@@ -225,7 +227,22 @@ def test_workspace_within_budget_with_overlap(prots):
             used = c.get_option("workspace_bytes")
             say("overlap", ovl, "pipeline", pipeline, "workspace bytes", used, "budget", budget)
             assert 0 < used <= budget
+        c.set_option("overlap_z", 1)
+        c.set_option("pipeline", -1)
+        for name in ("p96", "p128"):
+            gb = workloads.protein_grids(name)
+            pb = cdk.Protein(c, gb["values"], gb["radii"], gb["spacing"], gb["mid"], gb["length"])
+            big.append((pb, cdk.Ligand(pb, L["charge"], L["epsilon"], L["radius"])))
+        order = [(big[1], "p128"), ((prot, lig), "p64"), (big[0], "p96"), (big[1], "p128"), ((prot, lig), "p64")]
+        for (pp, ll), name in order:
+            E, _, _ = cdk.search(c, pp, ll, L["conformers"][:1], L["quats"][:1, :400])
+            used = c.get_option("workspace_bytes")
+            say(name, "workspace bytes", used, "budget", budget, "valid", int(np.isfinite(E).sum()))
+            assert 0 < used <= budget and np.isfinite(E).sum() > 0
     finally:
+        for pb, lb in big:
+            lb.close()
+            pb.close()
         lig.close()
         prot.close()
         c.close()
