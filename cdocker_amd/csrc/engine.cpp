@@ -412,8 +412,17 @@ struct EventTimer {
 
 namespace {
 
+// Before a working array is reallocated: the context's queued work (a
+// server's earlier launch set may still run on either stream) is done with
+// the old memory.
+void drain_streams(Context& ctx) {
+  hip_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize");
+  hip_check(hipStreamSynchronize(ctx.stream_z), "hipStreamSynchronize");
+}
+
 void ensure_work(Context& ctx, size_t bytes_needed, DevBuf& b) {
   if (b.cap >= bytes_needed) return;
+  drain_streams(ctx);
   b.ensure(bytes_needed);
   ++ctx.ws_generation;
 }
@@ -428,6 +437,7 @@ void carve_work(Context& ctx, size_t chunk, const size_t (&per)[5]) {
     end += (chunk * per[i] + 255) & ~(size_t)255;
   }
   if (ctx.d_pool.cap < end) {
+    drain_streams(ctx);
     ctx.d_pool.ensure(end);
     ++ctx.ws_generation;
   }

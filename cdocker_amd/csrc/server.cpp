@@ -94,15 +94,23 @@ int64_t Server::submit(int prot, int natom, const float* charge, const float* ep
 int Server::result(int64_t id, int wait_ms, float* energy, int* index, float* mincoor, double* latency_s,
                    std::string* err) {
   std::unique_lock<std::mutex> lk(m_);
+  // The request is looked up again after every wake-up: another caller may
+  // have collected (and released) it meanwhile.
   auto it = reqs_.find(id);
-  if (it == reqs_.end()) throw Error(kErrInvalid, "unknown request id (never submitted, or already collected)");
+  if (it == reqs_.end() || it->second->collected)
+    throw Error(kErrInvalid, "unknown request id (never submitted, or already collected)");
   Request* r = it->second.get();
-  auto finished = [&] { return r->state == Request::kDone || r->state == Request::kFailed; };
+  auto finished = [&] {
+    it = reqs_.find(id);
+    r = it == reqs_.end() ? nullptr : it->second.get();
+    return !r || r->collected || r->state == Request::kDone || r->state == Request::kFailed;
+  };
   if (wait_ms < 0)
     cv_done_.wait(lk, finished);
   else if (wait_ms > 0)
     cv_done_.wait_for(lk, std::chrono::milliseconds(wait_ms), finished);
   if (!finished()) return kErrPending;
+  if (!r || r->collected) throw Error(kErrInvalid, "request collected by another caller while this one waited");
   int code = kOk;
   if (r->state == Request::kFailed) {
     if (err) *err = r->err;
@@ -120,7 +128,7 @@ int Server::result(int64_t id, int wait_ms, float* energy, int* index, float* mi
   for (const Flight& f : fl_)
     if (f.live && std::find(f.owners.begin(), f.owners.end(), r) != f.owners.end()) pinned = true;
   if (pinned)
-    r->served_s = -1.0;
+    r->collected = true;
   else
     reqs_.erase(it);
   return code;
@@ -237,7 +245,7 @@ void Server::complete(Flight& f, int code, const std::string& msg) {
   for (size_t i = 0; i < f.jobs.size(); ++i) {
     Request* r = f.owners[i];
     const size_t b = f.jobs[i].pose_begin, n = f.jobs[i].npose();
-    if (r->served_s < 0) continue;  // collected while this set was in flight
+    if (r->collected) continue;  // collected while this set was in flight
     if (code == kOk && poses > 0) r->served_s += gpu_s * (double)n / (double)poses - f.est_s[i];
     if (code == kOk && r->state == Request::kActive) {
       const PoseResults& o = f.prep.results[i];
@@ -262,7 +270,7 @@ void Server::complete(Flight& f, int code, const std::string& msg) {
   --in_flight_;
   // requests collected while this set was in flight, and now in no other set
   for (Request* r : f.owners) {
-    if (r->served_s >= 0) continue;
+    if (!r->collected) continue;
     bool other = false;
     for (const Flight& g : fl_)
       if (g.live && std::find(g.owners.begin(), g.owners.end(), r) != g.owners.end()) other = true;

@@ -42,6 +42,7 @@ struct Request {
   size_t npose = 0, next = 0, done = 0;
   size_t bytes = 0;
   double served_s = 0;  // GPU seconds attributed so far (estimated at launch, measured at completion)
+  bool collected = false;  // returned by result() while a launch set in flight still held it
   PoseResults res;
   std::chrono::steady_clock::time_point t_submit, t_done;
   enum State { kQueued, kActive, kDone, kFailed } state = kQueued;

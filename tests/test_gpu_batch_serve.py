@@ -2,6 +2,8 @@
 results must be identical to the per-ligand search, alone or in the mix."""
 import time
 
+import threading
+
 import numpy as np
 import pytest
 
@@ -263,6 +265,26 @@ def test_server_releases_collected_requests(ctx, prots):
         assert st["held_requests"] == 0 and st["requests_done"] == 20 * (rnd + 1), st
     with pytest.raises(cdk.CdkError):
         srv.result(ids[-1], wait_ms=0)
+    # two callers waiting on the same request: one gets its results, the
+    # other an error (the request was released under it), neither crashes
+    for rnd in range(3):
+        rid = srv.submit(0, L["charge"], L["epsilon"], L["radius"], L["conformers"], q)
+        got = []
+
+        def collect():
+            try:
+                got.append(("ok", srv.result(rid)))
+            except cdk.CdkError as e:
+                got.append(("err", str(e)))
+
+        ths = [threading.Thread(target=collect) for _ in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join(60)
+        kinds = sorted(k for k, _ in got)
+        assert kinds == ["err", "ok"], got
+    assert srv.stats()["held_requests"] == 0
     srv.close()
 
 
