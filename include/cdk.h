@@ -85,8 +85,9 @@ int cdk_context_set_option(cdk_context* ctx, const char* key, int64_t value);
  * "last_pipeline", "last_mid_group", "last_mid_depth" and "last_mid_poses"
  * (what the autotuner chose for the last fused search), "max_dyn_lds" (the
  * device's dynamic LDS limit per workgroup, bytes) and "workspace_bytes"
- * (device bytes the fused engine's launch-set working arrays hold now; never
- * above "ws_budget_bytes").  No reference counterpart. */
+ * (device bytes of the fused engine's launch-set working arrays: one pool,
+ * carved per search; never above "ws_budget_bytes" unless "fused_chunk"
+ * forces larger launch sets).  No reference counterpart. */
 int cdk_context_get_option(cdk_context* ctx, const char* key, int64_t* value);
 /* Per-stage device time (HIP events on the context stream) accumulated while
  * "time_stages" is on: ms[8], launches[8].  Stage ids: reference engine
