@@ -110,6 +110,15 @@ def test_fft_known_answers(ctx):
     k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n // 2 + 1), indexing="ij")
     ph = np.exp(-2j * np.pi * (3 * k[0] + 5 * k[1] + 7 * k[2]) / n)
     assert np.abs(X[0] - ph).max() < 2e-5
+    # Parseval on the half spectrum: sum x^2 = (1/G) sum_k w_k |X_k|^2, w = 1
+    # on the kz = 0 and kz = N/2 planes (their own mirrors) and 2 elsewhere
+    x = np.random.default_rng(41).standard_normal((1, n, n, n)).astype(np.float32)
+    X = ctx.fft_r2c_3d(x).astype(np.complex128)
+    wk = np.full(n // 2 + 1, 2.0)
+    wk[0] = wk[n // 2] = 1.0
+    lhs = float((x.astype(np.float64) ** 2).sum())
+    rhs = float((np.abs(X[0]) ** 2 * wk[None, None, :]).sum()) / n ** 3
+    assert abs(lhs - rhs) <= 1e-5 * lhs, (lhs, rhs)
 
 
 def test_conj_mult_and_sum_grids(ctx):

@@ -95,6 +95,85 @@ def test_config2_subset_vs_oracle(ctx, p64):
         assert stats.get("exact", 0) >= 20
 
 
+RAMP = np.array([[0.25, -0.5, 0.125, 1.0], [-0.125, 0.25, 0.5, -2.0], [0.5, 0.125, -0.25, 0.5],
+                 [0.0625, -0.25, 0.375, 0.0], [-0.375, 0.5, 0.0625, 3.0], [0.125, 0.125, -0.5, -1.0],
+                 [0.25, 0.375, 0.25, 0.25], [-0.5, -0.0625, 0.125, 2.0], [0.375, -0.125, -0.375, -0.5]])
+
+
+@pytest.mark.parametrize("N", [64, 96])
+def test_linear_ramp_protein_closed_form(ctx, N):
+    """Protein channels that are linear ramps P_c(i, j, k) = a_c i + b_c j +
+    g_c k + d_c: trilinear spreading reproduces them exactly, so a pose's
+    energy at translation t is sum_c sum_{a in c} w_a P_c(u_a + t), u_a the
+    atom's continuous grid position, linear in t.  Every engine form (the
+    reference dataflow, the plane, column and atom forms, all three product
+    forms) must give that energy at its translation, and the argmin must be
+    the window corner the slopes' signs pick.  No oracle involved."""
+    h = 0.5
+    i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    vals = np.stack([(a * i + b * j + g * k + d).astype(np.float32) for a, b, g, d in RAMP])
+    radii = np.array([1.2, 1.5, 1.7, 1.8, 1.9, 2.0, 2.1, 2.2], np.float32)
+    mid = np.zeros(3, np.float32)
+    length = np.full(3, N * h, np.float32)
+    prot = cdk.Protein(ctx, vals, radii, h, mid, length)
+    L = workloads.ligand(971, 30, 2)
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    nu, used, ari = lig.channels()
+    quats = workloads.quaternion_sets(972, 2, 6)
+    forms = [("reference", cdk.ENGINE_REFERENCE, -1, 2), ("planes", cdk.ENGINE_FUSED, 0, 2),
+             ("atoms x3", cdk.ENGINE_FUSED, 2, 2), ("atoms x2", cdk.ENGINE_FUSED, 2, 0),
+             ("atoms fp32", cdk.ENGINE_FUSED, 2, 1)]
+    if N == 64:
+        forms.append(("columns", cdk.ENGINE_FUSED, 1, 2))
+    # channel of each atom: used vdW channels in used order, elec last
+    chan = np.full(len(ari), -1)
+    for c, r in enumerate(used):
+        chan[ari == r] = c
+    sq = np.sqrt(L["epsilon"].astype(np.float64))
+    q64 = L["charge"].astype(np.float64)
+    try:
+        for label, engine, pipe, products in forms:
+            ctx.set_option("pipeline", pipe)
+            ctx.set_option("atom_products", products)
+            E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], quats, engine=engine)
+            if engine == cdk.ENGINE_FUSED:
+                assert ctx.get_option("last_pipeline") == pipe
+            worst = 0.0
+            for c in range(E.shape[0]):
+                for qi in range(E.shape[1]):
+                    rot32 = orc.rotate(quats[c, qi], L["conformers"][c])[0]
+                    rot = rot32.astype(np.float64)
+                    u = (rot - mc[c, qi].astype(np.float64)) / h
+                    # window in FP32 as NoMini:446-450 (h = 0.5: the division is exact)
+                    span = (rot32.max(axis=0) - rot32.min(axis=0)).astype(np.float32)
+                    win = N - (span / np.float32(h)).astype(np.int64) - 2
+                    slope = np.zeros(3)
+                    E_t = 0.0
+                    for ch in range(nu + 1):
+                        sel = chan == ch if ch < nu else np.ones(len(ari), bool)
+                        w = sq[sel] if ch < nu else q64
+                        coef = RAMP[used[ch]] if ch < nu else RAMP[8]
+                        W = w.sum()
+                        Wu = (w[:, None] * u[sel]).sum(axis=0)
+                        slope += coef[:3] * W
+                        E_t += float(coef[:3] @ (Wu + idx[c, qi] * W) + coef[3] * W)
+                    scale = np.abs(slope).sum() * N + abs(E_t)
+                    err = abs(float(E[c, qi]) - E_t) / scale
+                    worst = max(worst, err)
+                    assert err <= 1e-5, (label, c, qi, float(E[c, qi]), E_t)
+                    corner = [0 if slope[d] > 0 else win[d] - 1 for d in range(3)]
+                    for d in range(3):
+                        # a slope well above the FP32 transforms' noise picks the corner
+                        if abs(slope[d]) > 1e-4 * scale:
+                            assert idx[c, qi, d] == corner[d], (label, c, qi, d, list(idx[c, qi]), corner)
+            say("ramp protein", N, label, "max scaled error %.2e" % worst)
+    finally:
+        ctx.set_option("pipeline", -1)
+        ctx.set_option("atom_products", 2)
+        lig.close()
+        prot.close()
+
+
 def test_shared_quaternions_and_empty_windows(ctx, p48):
     say("shared quats / empty windows")
     g, prot = p48

@@ -4,7 +4,7 @@
   (tests/golden/ref_vectors.npz): bit-exact;
 * the compiled reference subset live, on fresh inputs, when it is built;
 * closed forms and invariants (rotation matrix, spreading mass and first
-  moment, constant-grid and single-atom correlations);
+  moment, constant-grid, linear-ramp and single-atom correlations);
 * an independent FP64 formulation (per-atom trilinear gather) and an FFT
   restatement of the reference's correlation in numpy;
 * planted faults that those checks must catch.
@@ -310,6 +310,52 @@ def test_constant_grid_known_answer():
         expect += k[c] * np.sqrt(prob.eps[prob.ari == prob.idx_used[c]]).astype(np.float64).sum()
     expect += k[-1] * prob.charge.astype(np.float64).sum()
     assert np.allclose(res["E"], expect, rtol=1e-6, atol=1e-6)
+
+
+def test_linear_ramp_grids_closed_form():
+    """Every protein channel a different linear ramp P_c(i, j, k) = a_c i +
+    b_c j + g_c k + d_c (dyadic coefficients: exact in FP32).  Trilinear
+    spreading reproduces a linear function exactly, so the FP64 correlation
+    at translation t is sum_c sum_{a in c} w_a P_c(u_a + t) with u_a = (p_a -
+    m) / h the atom's continuous grid position: linear in t with slopes
+    sum_c (a_c, b_c, g_c) W_c.  A transposed axis, a wrong sign of t or a
+    dropped channel each change it."""
+    prob, x, rs = small_problem(seed=11)
+    N = prob.N
+    C = prob.Pused.shape[0]
+    coef = np.array([[0.25, -0.5, 0.125, 1.0], [-0.125, 0.25, 0.5, -2.0], [0.5, 0.125, -0.25, 0.5],
+                     [0.0625, -0.25, 0.375, 0.0], [-0.375, 0.5, 0.0625, 3.0], [0.125, 0.125, -0.5, -1.0],
+                     [0.25, 0.375, 0.25, 0.25], [-0.5, -0.0625, 0.125, 2.0], [0.375, -0.125, -0.375, -0.5]])[:C]
+    i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    prob.Pused = np.ascontiguousarray(np.stack(
+        [(a * i + b * j + g * k + d).astype(np.float32) for a, b, g, d in coef]))
+    q = rs.standard_normal(4).astype(np.float32)
+    q /= np.linalg.norm(q)
+    res = prob.search_pose(x, q, double=True, want_E=True)
+    assert res["valid"]
+    rot = orc.rotate(q, x)[0].astype(np.float64)
+    u = (rot - res["mincoor"].astype(np.float64)) / prob.h
+    ti, tj, tk = np.meshgrid(*[np.arange(n) for n in res["window"]], indexing="ij")
+
+    def closed_form(axes):
+        """sum over channels of the ramp at the atoms' positions plus t; axes
+        permutes which coefficient goes with which grid axis."""
+        E = np.zeros(tuple(res["window"]))
+        for c in range(C):
+            elec = c == C - 1
+            sel = np.ones(prob.natom, bool) if elec else prob.atom_channel == c
+            w = prob.charge.astype(np.float64) if elec else np.sqrt(prob.eps.astype(np.float64))
+            W = w[sel].sum()
+            Wu = (w[sel, None] * u[sel]).sum(axis=0)
+            s = coef[c][list(axes)]
+            E += s[0] * (Wu[0] + ti * W) + s[1] * (Wu[1] + tj * W) + s[2] * (Wu[2] + tk * W) + coef[c][3] * W
+        return E
+
+    E = closed_form((0, 1, 2))
+    scale = np.abs(E).max()
+    assert np.abs(res["E"] - E).max() <= 1e-6 * scale, np.abs(res["E"] - E).max() / scale
+    # the x and z axes swapped (a transposed operand) does not fit
+    assert np.abs(res["E"] - closed_form((2, 1, 0))).max() > 1e-3 * scale
 
 
 def test_single_atom_correlation_is_window_of_grid():
