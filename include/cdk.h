@@ -245,7 +245,8 @@ int cdk_server_submit(cdk_server* s, int protein, int natom, const float* charge
 /* wait_ms < 0 blocks until done; 0 polls.  Returns CDK_ERR_PENDING if not
  * finished.  latency_s: submit -> results available on the host.  The call
  * that returns a finished request's results (or its error) releases the
- * request: its id is unknown afterwards (CDK_ERR_INVALID). */
+ * request: its id is unknown afterwards (CDK_ERR_INVALID); of several threads
+ * collecting one id, one gets the results and the others CDK_ERR_INVALID. */
 int cdk_server_result(cdk_server* s, int64_t request_id, int wait_ms, float* energy, int* index, float* mincoor,
                       double* latency_s);
 /* The first n of: requests done, poses done, launch sets, mean poses per
