@@ -1580,6 +1580,39 @@ __device__ __forceinline__ void col_inverse(float2 (&acc)[N / (4 * RegShape<N>::
   }
 }
 
+// Workgroup w of a launch over G units (pose groups, or (pose group, block
+// row) pairs) times the N/2 + 1 = 8 Q + 1 kz planes.  Workgroups go to the 8
+// XCDs round-robin (w % 8), so XCD x runs kz = x + 8 j (j < Q) of every unit
+// -- its L2 holds its own eighth of the protein planes -- and the kz = N/2
+// plane of every eighth unit (unit % 8 == x): every XCD gets the same work.
+// (Padding the kz planes to a multiple of 8 instead left XCD 0 with Q + 1
+// planes per unit and the others with Q.)  Launch kz_unit_grid(Q, G)
+// workgroups; false for those past the last unit.
+__host__ __device__ __forceinline__ unsigned kz_unit_grid(int Q, int G) {
+  return 8u * (unsigned)(8 * Q + 1) * (unsigned)((G + 7) / 8);
+}
+
+__device__ __forceinline__ bool kz_unit_of(int w, int Q, int G, int& kz, int& unit) {
+  const int x = w & 7, m = w >> 3;
+  const int per = 8 * Q + 1;
+  const int b = m / per, r = m - b * per;
+  int g8, j;
+  if (r < Q * x) {
+    g8 = r / Q;
+    j = r - g8 * Q;
+  } else if (r <= Q * x + Q) {
+    g8 = x;
+    j = r - Q * x;
+  } else {
+    const int r2 = r - Q * x - Q - 1;
+    g8 = x + 1 + r2 / Q;
+    j = r2 - (r2 / Q) * Q;
+  }
+  unit = 8 * b + g8;
+  kz = j == Q ? 8 * Q : x + 8 * j;
+  return unit < G;
+}
+
 // Column form, mid: one workgroup per (kz, group of poses).  For each pose
 // and channel: y transforms of the box's x lines into the LDS plane [x][ky],
 // x transforms of all ky lines out of it, product with conj and the protein
@@ -1609,15 +1642,13 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
   constexpr int LPP = 4 * LPW;     // lines per pass
   static_assert(N % LPP == 0, "whole passes of lines");
   constexpr int NPASS = N / LPP;
-  constexpr int KZX = (ZH + 7) / 8;
+  static_assert(ZH % 8 == 1, "kz planes: 8 Q + 1");
   __shared__ float2 sbuf[4][RS::SLICE];
   constexpr int PROWS = HALF ? N / 2 : N;  // plane rows
   __shared__ float2 pl[PROWS * N];
   __shared__ float2 stw[L * V];
-  const int w = blockIdx.x;
-  const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
-  const int sg = (w >> 3) / KZX;
-  if (kz >= ZH) return;
+  int kz, sg;
+  if (!kz_unit_of(blockIdx.x, ZH / 8, (nslots + group - 1) / group, kz, sg)) return;
   // twiddles from LDS: the registers go to the accumulators
   for (int t = threadIdx.x; t < L * V; t += blockDim.x) stw[t] = twg[(t / V) * (t % V)];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -2137,7 +2168,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   static_assert(N == 64 && V == 8 && L == 8, "atom form: 64 points, 8 lanes x 8 values per line");
   constexpr int LPP = 4 * LPW;
   constexpr int NPASS = N / LPP;
-  constexpr int KZX = (ZH + 7) / 8;
+  static_assert(ZH % 8 == 1, "kz planes: 8 Q + 1");
   constexpr int ENT = 512;  // channel atoms staged at a time
   // The staged atoms, the plane's contents and the inverse transforms'
   // scratch are never live together: all three share the plane's memory.
@@ -2150,10 +2181,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   __shared__ float2 stwr[L * V];  // register FFT twiddles
   float4* ef = reinterpret_cast<float4*>(scratch);                      // (w Z(kz), fx w Z(kz))
   int4* ei = reinterpret_cast<int4*>(scratch + ENT * sizeof(float4));  // (8 ix, 8 iy, fy bits, 0)
-  const int w = blockIdx.x;
-  const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
-  const int sg = (w >> 3) / KZX;
-  if (kz >= ZH) return;
+  int kz, sg;
+  if (!kz_unit_of(blockIdx.x, ZH / 8, (nslots + group - 1) / group, kz, sg)) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
   for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
@@ -2260,17 +2289,15 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
   constexpr int N = P::N, ZH = P::ZH;
   static_assert(N == 96 || N == 128, "the two-kernel atom form: grid edges 96 and 128");
   constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks per plane
-  constexpr int KZX = (ZH + 7) / 8;
+  static_assert(ZH % 8 == 1, "kz planes: 8 Q + 1");
   constexpr int ENT = 512;
   __shared__ float2 stw[N];
   __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
   __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0; times 8 at 128)
   __shared__ uint4 abuf[2 * 4 * 64];  // the shared A operands (the row's wavefronts share kx)
-  const int w = blockIdx.x;
-  const int kz = (w & 7) + 8 * ((w >> 3) % KZX);
-  const int rest = (w >> 3) / KZX;
-  const int brow = rest % NB, sg = rest / NB;
-  if (kz >= ZH) return;
+  int kz, unit;
+  if (!kz_unit_of(blockIdx.x, ZH / 8, NB * ((nslots + group - 1) / group), kz, unit)) return;
+  const int brow = unit % NB, sg = unit / NB;
   const int tid = threadIdx.x;
   for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -2748,8 +2775,7 @@ hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const Lig
                          float invG, const float2* tw) {
   if (nslots <= 0) return hipSuccess;
   if (!col_supported(N) || group <= 0) return hipErrorInvalidValue;
-  const int kzx = (N / 2 + 1 + 7) / 8;
-  const dim3 grid((unsigned)(8 * kzx) * ceil_div(nslots, group));
+  const dim3 grid(kz_unit_grid(N / 16, (int)ceil_div(nslots, group)));
   CDK_DISPATCH_N(N, {
     if constexpr (PL::N == 64) {
       // poses whose box fits in N/2 along x go through the half-plane form,
@@ -2797,8 +2823,7 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
   if (nslots <= 0) return hipSuccess;
   if (!atoms_supported(N) || group <= 0 || (products != kAtomF16 && products != kAtomF32 && products != kAtomF16x3))
     return hipErrorInvalidValue;
-  const int kzx = (N / 2 + 1 + 7) / 8;
-  const dim3 grid((unsigned)(8 * kzx) * ceil_div(nslots, group));
+  const dim3 grid(kz_unit_grid(N / 16, (int)ceil_div(nslots, group)));
   CDK_DISPATCH_N(N, {
     if constexpr (PL::N == 64)
       hipLaunchKernelGGL((products == kAtomF16x3 ? k_mid_atoms<PL, kAtomF16x3>
@@ -2808,7 +2833,7 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
                          P3, T2, invG, tw);
     if constexpr (PL::N == 96 || PL::N == 128) {
       constexpr int NB = PL::N / 32, NLW = PL::N / RegShape<PL::N>::LPW;
-      const dim3 gf((unsigned)(8 * kzx * NB) * ceil_div(nslots, group));
+      const dim3 gf(kz_unit_grid(N / 16, NB * (int)ceil_div(nslots, group)));
       hipLaunchKernelGGL((products == kAtomF16x3 ? k_atoms_fwd<PL, kAtomF16x3>
                           : products == kAtomF16 ? k_atoms_fwd<PL, kAtomF16>
                                                  : k_atoms_fwd<PL, kAtomF32>),
