@@ -815,6 +815,37 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
   }
 }
 
+// Workgroup w of a launch over G units (pose groups, or (pose group, block
+// row) pairs) times M = 8 Q + R items (kz planes, or line groups of the
+// spectra).  Workgroups go to the 8 XCDs round-robin (w % 8), so XCD x runs
+// items [Q x, Q x + Q) of every unit -- its L2 holds its own eighth of the
+// protein data -- and, of each block of 8 units, R of the 8 R leftover items
+// (item 8 Q + i of unit g8 goes to XCD (g8 R + i) % 8): every XCD gets the
+// same work.  (Padding the items to a multiple of 8 instead left some XCDs a
+// whole item per unit more than others, and those set the kernel's time.)
+// Launch xcd_unit_grid(Q, R, G) workgroups; false for those past the last
+// unit.
+__host__ __device__ __forceinline__ unsigned xcd_unit_grid(int Q, int R, int G) {
+  return 8u * (unsigned)(8 * Q + R) * (unsigned)((G + 7) / 8);
+}
+
+__device__ __forceinline__ bool xcd_unit_of(int w, int Q, int R, int G, int& item, int& unit) {
+  const int x = w & 7, m = w >> 3;
+  const int per = 8 * Q + R;
+  const int b = m / per, r = m - b * per;
+  int g8;
+  if (r < 8 * Q) {
+    g8 = r / Q;
+    item = Q * x + (r - g8 * Q);
+  } else {
+    const int t = x + 8 * (r - 8 * Q);
+    g8 = t / R;
+    item = 8 * Q + (t - g8 * R);
+  }
+  unit = 8 * b + g8;
+  return unit < G;
+}
+
 // Stage B: x FFT of every used channel, conj-multiply with the protein
 // spectra, channel sum, 1/G, inverse x FFT, for the (ky, kz) lines of the
 // spectrum.  Each wavefront owns 64/L consecutive lines (flattened
@@ -826,10 +857,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_planes_reg(const PoseSlot* __res
 // items; a ring of DEPTH register buffers (DEPTH tuned, 1 for long lines)
 // keeps the next items' loads in flight while one item is transformed.  When the
 // pose's box is at most N/2 the upper half of every line is zero: those rows
-// are not loaded and the first DFT stage is pruned.  Workgroup w runs on
-// XCD w % 8 and the line groups are split so each XCD touches one eighth of
-// the protein spectra, which then stays in that XCD's L2.  Reads
+// are not loaded and the first DFT stage is pruned.  Workgroups map to
+// (line group, pose group) by xcd_unit_of, so each XCD touches about one
+// eighth of the protein spectra, which then stays in that XCD's L2.  Reads
 // A[slot][c][x][line] for x < box, writes T[slot][x'][line] for x' < window.
+
 #ifdef CDK_MID_WAVES
 #define CDK_MID_ATTR __attribute__((amdgpu_waves_per_eu(CDK_MID_WAVES)))
 #else
@@ -847,17 +879,14 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid(const PoseSlot* __r
   constexpr int NL = N * ZH;             // lines in a spectrum
   constexpr int LPB = 4 * LPW;           // lines per workgroup
   constexpr int NLG = (NL + LPB - 1) / LPB;
-  constexpr int PERX = (NLG + 7) / 8;    // line groups per XCD
   // Long lines read their twiddles from LDS (registers go to the data).
   constexpr bool TW_LDS = V >= 16;
   // BOXHALF: every pose's box fits in N/2, so only V/2 ligand rows exist.
   constexpr int VA = BOXHALF ? V / 2 : V;
   __shared__ float2 sbuf[4][RS::SLICE];
   __shared__ float2 stw[TW_LDS ? L * V : 1];
-  const int w = blockIdx.x;
-  const int lg = (w & 7) * PERX + ((w >> 3) % PERX);
-  const int sg = (w >> 3) / PERX;
-  if (lg >= NLG) return;
+  int lg, sg;
+  if (!xcd_unit_of(blockIdx.x, NLG / 8, NLG % 8, (nslots + group - 1) / group, lg, sg)) return;
   if constexpr (TW_LDS) {
     for (int t = threadIdx.x; t < L * V; t += blockDim.x) stw[t] = twg[(t / V) * (t % V)];
     __syncthreads();
@@ -1023,13 +1052,10 @@ __global__ __launch_bounds__(kBlock) CDK_MID_ATTR void k_mid_np(const PoseSlot* 
   constexpr int NL = N * ZH;
   constexpr int LPB = 4 * LPW;
   constexpr int NLG = (NL + LPB - 1) / LPB;
-  constexpr int PERX = (NLG + 7) / 8;
   constexpr int VA = BOXHALF ? V / 2 : V;
   __shared__ float2 sbuf[4][RS::SLICE];
-  const int w = blockIdx.x;
-  const int lg = (w & 7) * PERX + ((w >> 3) % PERX);
-  const int sg = (w >> 3) / PERX;
-  if (lg >= NLG) return;
+  int lg, sg;
+  if (!xcd_unit_of(blockIdx.x, NLG / 8, NLG % 8, (nslots + group - 1) / group, lg, sg)) return;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int ln = lane % LPW, n2 = lane / LPW;
@@ -1580,38 +1606,6 @@ __device__ __forceinline__ void col_inverse(float2 (&acc)[N / (4 * RegShape<N>::
   }
 }
 
-// Workgroup w of a launch over G units (pose groups, or (pose group, block
-// row) pairs) times the N/2 + 1 = 8 Q + 1 kz planes.  Workgroups go to the 8
-// XCDs round-robin (w % 8), so XCD x runs kz = x + 8 j (j < Q) of every unit
-// -- its L2 holds its own eighth of the protein planes -- and the kz = N/2
-// plane of every eighth unit (unit % 8 == x): every XCD gets the same work.
-// (Padding the kz planes to a multiple of 8 instead left XCD 0 with Q + 1
-// planes per unit and the others with Q.)  Launch kz_unit_grid(Q, G)
-// workgroups; false for those past the last unit.
-__host__ __device__ __forceinline__ unsigned kz_unit_grid(int Q, int G) {
-  return 8u * (unsigned)(8 * Q + 1) * (unsigned)((G + 7) / 8);
-}
-
-__device__ __forceinline__ bool kz_unit_of(int w, int Q, int G, int& kz, int& unit) {
-  const int x = w & 7, m = w >> 3;
-  const int per = 8 * Q + 1;
-  const int b = m / per, r = m - b * per;
-  int g8, j;
-  if (r < Q * x) {
-    g8 = r / Q;
-    j = r - g8 * Q;
-  } else if (r <= Q * x + Q) {
-    g8 = x;
-    j = r - Q * x;
-  } else {
-    const int r2 = r - Q * x - Q - 1;
-    g8 = x + 1 + r2 / Q;
-    j = r2 - (r2 / Q) * Q;
-  }
-  unit = 8 * b + g8;
-  kz = j == Q ? 8 * Q : x + 8 * j;
-  return unit < G;
-}
 
 // Column form, mid: one workgroup per (kz, group of poses).  For each pose
 // and channel: y transforms of the box's x lines into the LDS plane [x][ky],
@@ -1642,13 +1636,12 @@ __global__ __launch_bounds__(kBlock) void k_mid_kz(const PoseSlot* __restrict__ 
   constexpr int LPP = 4 * LPW;     // lines per pass
   static_assert(N % LPP == 0, "whole passes of lines");
   constexpr int NPASS = N / LPP;
-  static_assert(ZH % 8 == 1, "kz planes: 8 Q + 1");
   __shared__ float2 sbuf[4][RS::SLICE];
   constexpr int PROWS = HALF ? N / 2 : N;  // plane rows
   __shared__ float2 pl[PROWS * N];
   __shared__ float2 stw[L * V];
   int kz, sg;
-  if (!kz_unit_of(blockIdx.x, ZH / 8, (nslots + group - 1) / group, kz, sg)) return;
+  if (!xcd_unit_of(blockIdx.x, ZH / 8, ZH % 8, (nslots + group - 1) / group, kz, sg)) return;
   // twiddles from LDS: the registers go to the accumulators
   for (int t = threadIdx.x; t < L * V; t += blockDim.x) stw[t] = twg[(t / V) * (t % V)];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -2168,7 +2161,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   static_assert(N == 64 && V == 8 && L == 8, "atom form: 64 points, 8 lanes x 8 values per line");
   constexpr int LPP = 4 * LPW;
   constexpr int NPASS = N / LPP;
-  static_assert(ZH % 8 == 1, "kz planes: 8 Q + 1");
   constexpr int ENT = 512;  // channel atoms staged at a time
   // The staged atoms, the plane's contents and the inverse transforms'
   // scratch are never live together: all three share the plane's memory.
@@ -2182,7 +2174,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   float4* ef = reinterpret_cast<float4*>(scratch);                      // (w Z(kz), fx w Z(kz))
   int4* ei = reinterpret_cast<int4*>(scratch + ENT * sizeof(float4));  // (8 ix, 8 iy, fy bits, 0)
   int kz, sg;
-  if (!kz_unit_of(blockIdx.x, ZH / 8, (nslots + group - 1) / group, kz, sg)) return;
+  if (!xcd_unit_of(blockIdx.x, ZH / 8, ZH % 8, (nslots + group - 1) / group, kz, sg)) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
   for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
@@ -2289,14 +2281,13 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
   constexpr int N = P::N, ZH = P::ZH;
   static_assert(N == 96 || N == 128, "the two-kernel atom form: grid edges 96 and 128");
   constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks per plane
-  static_assert(ZH % 8 == 1, "kz planes: 8 Q + 1");
   constexpr int ENT = 512;
   __shared__ float2 stw[N];
   __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
   __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0; times 8 at 128)
   __shared__ uint4 abuf[2 * 4 * 64];  // the shared A operands (the row's wavefronts share kx)
   int kz, unit;
-  if (!kz_unit_of(blockIdx.x, ZH / 8, NB * ((nslots + group - 1) / group), kz, unit)) return;
+  if (!xcd_unit_of(blockIdx.x, ZH / 8, ZH % 8, NB * ((nslots + group - 1) / group), kz, unit)) return;
   const int brow = unit % NB, sg = unit / NB;
   const int tid = threadIdx.x;
   for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
@@ -2775,7 +2766,7 @@ hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const Lig
                          float invG, const float2* tw) {
   if (nslots <= 0) return hipSuccess;
   if (!col_supported(N) || group <= 0) return hipErrorInvalidValue;
-  const dim3 grid(kz_unit_grid(N / 16, (int)ceil_div(nslots, group)));
+  const dim3 grid(xcd_unit_grid(N / 16, 1, (int)ceil_div(nslots, group)));
   CDK_DISPATCH_N(N, {
     if constexpr (PL::N == 64) {
       // poses whose box fits in N/2 along x go through the half-plane form,
@@ -2823,7 +2814,7 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
   if (nslots <= 0) return hipSuccess;
   if (!atoms_supported(N) || group <= 0 || (products != kAtomF16 && products != kAtomF32 && products != kAtomF16x3))
     return hipErrorInvalidValue;
-  const dim3 grid(kz_unit_grid(N / 16, (int)ceil_div(nslots, group)));
+  const dim3 grid(xcd_unit_grid(N / 16, 1, (int)ceil_div(nslots, group)));
   CDK_DISPATCH_N(N, {
     if constexpr (PL::N == 64)
       hipLaunchKernelGGL((products == kAtomF16x3 ? k_mid_atoms<PL, kAtomF16x3>
@@ -2833,7 +2824,7 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
                          P3, T2, invG, tw);
     if constexpr (PL::N == 96 || PL::N == 128) {
       constexpr int NB = PL::N / 32, NLW = PL::N / RegShape<PL::N>::LPW;
-      const dim3 gf(kz_unit_grid(N / 16, NB * (int)ceil_div(nslots, group)));
+      const dim3 gf(xcd_unit_grid(N / 16, 1, NB * (int)ceil_div(nslots, group)));
       hipLaunchKernelGGL((products == kAtomF16x3 ? k_atoms_fwd<PL, kAtomF16x3>
                           : products == kAtomF16 ? k_atoms_fwd<PL, kAtomF16>
                                                  : k_atoms_fwd<PL, kAtomF32>),
@@ -2905,8 +2896,7 @@ int mid_line_groups(int N) {
   const int L = (N == 64 || N == 128) ? 8 : 4;
   const int lpb = 4 * (64 / L);
   const int NL = N * (N / 2 + 1);
-  const int NLG = (NL + lpb - 1) / lpb;
-  return ((NLG + 7) / 8) * 8;
+  return (NL + lpb - 1) / lpb;
 }
 
 namespace {
@@ -2935,8 +2925,8 @@ hipError_t launch_mid(hipStream_t st, int N, int depth, int np, const PoseSlot* 
   (void)npch;
   if (nslots <= 0) return hipSuccess;
   if (group <= 0 || np < 1 || np > mid_max_poses(N) || (np & (np - 1)) != 0) return hipErrorInvalidValue;
-  const unsigned nsg = ceil_div(nslots, group);
-  const dim3 grid((unsigned)mid_line_groups(N) * nsg);
+  const int nlg = mid_line_groups(N);
+  const dim3 grid(xcd_unit_grid(nlg / 8, nlg % 8, (int)ceil_div(nslots, group)));
   const bool half = 2 * boxmax <= N;  // every valid pose's box fits in N/2
   if (np > 1) {
     CDK_DISPATCH_N(N, {

@@ -7,6 +7,8 @@
 namespace cdk {
 
 bool grid_edge_supported(int N);
+// Line groups (four wavefronts of transform lines each) of an N-point
+// half spectrum, the plane form's mid-stage work items per pose group.
 int mid_line_groups(int N);
 // Copies [nch][N][N*(N/2+1)] spectra into the fused engine's line-blocked
 // layout (blocks of 64/L lines, rows contiguous within a block).
