@@ -174,6 +174,61 @@ def test_linear_ramp_protein_closed_form(ctx, N):
         prot.close()
 
 
+def test_edge_sizes(ctx, p64):
+    """Edge sizes: a ligand of the maximum 512 atoms (compact, so its box fits
+    the 64^3 grid) through the atom form and the reference dataflow against
+    the FP64 oracle over the whole window; 513 atoms refused; searches with no
+    rotations or no conformers return empty results; a batch whose middle job
+    has no rotations keeps the other jobs' results."""
+    from tests.parity import check_sampled
+    g, prot = p64
+    rs = np.random.default_rng(981)
+    na = 512
+    L = workloads.ligand(981, na, 1)
+    d = rs.normal(size=(na, 3))
+    x = (7.0 * d / np.linalg.norm(d, axis=1, keepdims=True) * rs.uniform(0, 1, (na, 1)) ** (1 / 3)).astype(np.float32)
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    quats = workloads.quaternion_sets(982, 1, 3)
+    try:
+        for engine, pipe in ((cdk.ENGINE_FUSED, 2), (cdk.ENGINE_REFERENCE, -1)):
+            ctx.set_option("pipeline", pipe)
+            E, idx, mc = cdk.search(ctx, prot, lig, x[None], quats, engine=engine)
+            st = check_sampled(prob, x[None], np.zeros(3, np.int32), quats[0], E[0], idx[0], mc[0],
+                               "512 atoms engine %d" % engine)
+            say("512-atom ligand, engine", engine, st)
+            assert st["valid"] == 3 and st["exact"] + st["near_ties"] == 3
+        # no rotations, no conformers
+        E, idx, mc = cdk.search(ctx, prot, lig, x[None], np.zeros((1, 0, 4), np.float32))
+        assert E.shape == (1, 0) and idx.shape == (1, 0, 3)
+        E, idx, mc = cdk.search(ctx, prot, lig, np.zeros((0, na, 3), np.float32), np.zeros((0, 5, 4), np.float32))
+        assert E.shape == (0, 5)
+    finally:
+        ctx.set_option("pipeline", -1)
+        lig.close()
+    with pytest.raises(cdk.CdkError):
+        L2 = workloads.ligand(983, 513, 1)
+        cdk.Ligand(prot, L2["charge"], L2["epsilon"], L2["radius"])
+    # a batch with an empty middle job
+    Ls = [workloads.ligand(984 + i, 20, 2) for i in range(3)]
+    ligs = [cdk.Ligand(prot, Li["charge"], Li["epsilon"], Li["radius"]) for Li in Ls]
+    qs = [workloads.quaternion_sets(990 + i, 2, n) for i, n in enumerate((5, 0, 7))]
+    try:
+        b = cdk.Batch(ctx, prot)
+        for lg, Li, q in zip(ligs, Ls, qs):
+            b.add(lg, Li["conformers"], q)
+        b.run(cdk.ENGINE_FUSED)
+        for j in (0, 2):
+            E, idx, mc = b.results(j)
+            E1, i1, m1 = cdk.search(ctx, prot, ligs[j], Ls[j]["conformers"], qs[j])
+            assert np.array_equal(E.view(np.uint32), E1.view(np.uint32)) and np.array_equal(idx, i1)
+        assert b.results(1)[0].shape == (2, 0)
+        b.close()
+    finally:
+        for lg in ligs:
+            lg.close()
+
+
 def test_shared_quaternions_and_empty_windows(ctx, p48):
     say("shared quats / empty windows")
     g, prot = p48
