@@ -2466,7 +2466,15 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
   static_assert(N == 64 || N == 96 || N == 128, "column and atom forms: grid edges 64, 96 and 128");
   constexpr int H = L / 2;        // row pairs per L rows
   constexpr int LPB = 4 * LPW;    // row pairs per workgroup
-  __shared__ float2 sbuf[4][RS::SLICE];
+  // Lines of 16 and more values per lane (96, 128) take their twiddles from
+  // LDS, so the registers go to the data and more wavefronts fit a SIMD; at
+  // 96 the transposes' scratch (15 KB per wavefront) would then cap the
+  // kernel at two workgroups per CU: two wavefronts transform at a time,
+  // on half the scratch.
+  constexpr bool TW_LDS = V >= 16;
+  constexpr int TURNS = N == 96 ? 2 : 1;
+  __shared__ float2 sbuf[4 / TURNS][RS::SLICE];
+  __shared__ float2 stwr[TW_LDS ? L * V : 1];
   const int s = blockIdx.y;
   const RotMeta m = meta[s];
   if (!m.valid) return;
@@ -2476,8 +2484,15 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
   if ((int)blockIdx.x * LPB >= nl) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ln = lane % LPW, n2 = lane / LPW;
-  float2 twr[V];
-  reg_twiddles<N>(twg, n2, twr);
+  float2 twreg[TW_LDS ? 1 : V];
+  const float2* twr = twreg;
+  if constexpr (TW_LDS) {
+    for (int t = threadIdx.x; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
+    __syncthreads();
+    twr = stwr + n2 * V;
+  } else {
+    reg_twiddles<N>(twg, n2, twreg);
+  }
   const int l = blockIdx.x * LPB + wave * LPW + ln;
   const bool ok = l < nl;
   const int mm = l / (wx * H), r = l - mm * (wx * H);
@@ -2503,7 +2518,15 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
     }
     z[n1] = make_float2(A.x - B.y, A.y + B.x);
   }
-  reg_fft<N, 1>(z, twr, sbuf[wave], ln, n2);
+  if constexpr (TURNS == 1) {
+    reg_fft<N, 1>(z, twr, sbuf[wave], ln, n2);
+  } else {
+#pragma unroll
+    for (int t = 0; t < TURNS; ++t) {
+      if (wave / (4 / TURNS) == t) reg_fft<N, 1>(z, twr, sbuf[wave % (4 / TURNS)], ln, n2);
+      __syncthreads();  // this turn's scratch reads are done before the next turn writes it
+    }
+  }
   unsigned long long best = ~0ull;
   if (ok) {
 #pragma unroll
