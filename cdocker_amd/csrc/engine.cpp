@@ -527,16 +527,16 @@ bool overlap_possible(const Context& ctx, const Protein& prot) {
 }
 
 // Bytes per pose of each launch-set working array: rotated atoms, atom
-// placements, A (the plane and column forms' ligand spectra, or the atom
-// form's product spectra at grid edges 96 and 128), T (correlation spectra,
-// the atom form's T2), and the overlapped atom form's second T2 buffer when
-// that form may run.
+// placements, A (the plane and column forms' ligand spectra), T (correlation
+// spectra, the atom form's T2: at grid edges 96 and 128 the window's y rows
+// of the product spectra), and the overlapped atom form's second T2 buffer
+// when that form may run.
 void per_pose_bytes(const Context& ctx, const Protein& prot, const Prepared& t, size_t (&s)[5]) {
   const int N = prot.N, ZH = N / 2 + 1;
   const size_t plane = (size_t)N * ZH;
   s[0] = (size_t)t.astride * 3 * sizeof(float);
   s[1] = (size_t)t.astride * sizeof(AtomGrid);
-  s[2] = std::max((size_t)t.cmax * t.boxmax * plane * sizeof(float2), atoms_work_bytes(N));
+  s[2] = (size_t)t.cmax * t.boxmax * plane * sizeof(float2);
   s[3] = (size_t)N * plane * sizeof(float2);
   s[4] = overlap_possible(ctx, prot) ? s[3] : 0;
 }
@@ -820,8 +820,7 @@ Tuned tune_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk) {
         return timed([&] {
           hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_pad.as<int>(), t.d_wv.as<float>(),
                                      t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta, n, g,
-                                     prot.spec_mf.as<float2>(), ctx.w_A, ctx.w_T, invG,
-                                     tw, ctx.atom_products),
+                                     prot.spec_mf.as<float2>(), ctx.w_T, invG, tw, ctx.atom_products),
                     "tune atom mid");
         });
       };
@@ -909,8 +908,7 @@ void enqueue_fused(Context& ctx, const Protein& prot, Prepared& t, size_t chunk,
       e = timer.begin(2);
       hip_check(launch_mid_atoms(ctx.stream, N, sl, d_ligs, t.d_pad.as<int>(), t.d_wv.as<float>(),
                                  t.d_wq.as<float>(), ctx.w_ag, t.astride, d_meta + s, n, tu.atom_group,
-                                 prot.spec_mf.as<float2>(), ctx.w_A, T2, 1.0f / (float)G, tw,
-                                 ctx.atom_products),
+                                 prot.spec_mf.as<float2>(), T2, 1.0f / (float)G, tw, ctx.atom_products),
                 "atom mid");
       timer.end(e);
       if (ovl) {

@@ -61,9 +61,11 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 }
 
 // Block-wide min of a key, then one atomicMin into `dst` (order independent,
-// so the result does not depend on workgroup scheduling).
+// so the result does not depend on workgroup scheduling).  At most NW
+// wavefronts per workgroup.
+template <int NW = kBlock / 64>
 __device__ __forceinline__ void block_min_commit(unsigned long long v, unsigned long long* dst) {
-  __shared__ unsigned long long red[kBlock / 64];
+  __shared__ unsigned long long red[NW];
   v = wave_min_u64(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) red[wid] = v;
@@ -2245,29 +2247,55 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   }
 }
 
-// Atom form at grid edges 96 and 128, in two kernels so that each runs at its
-// own occupancy (the 96- and 128-point inverse transforms hold 24 and 16
-// values per lane and need the whole plane in LDS).
+// Atom form at grid edges 96 and 128, in two kernels (a 96- or 128-point
+// line holds 24 or 16 values per lane, so a whole plane does not fit one
+// workgroup's registers as at 64).
 //
 // k_atoms_fwd: one workgroup per (kz, row of 32 x 32 kx blocks, group of
 // poses), N / 32 wavefronts, one block each: the same matrix-core ligand
 // spectra and product as k_mid_atoms (at 96 the table index reduced mod 96
 // exactly by a multiply and shift: (t * 43691) >> 22 = t / 96 for t <= 95 *
-// 95).  The pose's product spectrum goes to E[slot][kz][block][r/2][lane][2]
-// (the matrix-core result layout: 1 KB per wavefront store).
+// 95).  The row's wavefronts own the same kx rows and share the A operands.
+// Per pose the row's product spectrum (32 kx rows x N ky, times 1/G) goes
+// into LDS tiles of 16 kx rows, the inverse y transforms run as LDS
+// Stockham passes (few registers: the products keep the register file),
+// and the window's y rows go to R[slot][y][kz][kx].
 //
-// k_atoms_inv: one workgroup per (pose, kz), N / (lines per wavefront)
-// wavefronts (6 at 96, 16 at 128): the blocks into a plane of odd row stride
-// N + 1 in dynamic LDS (consecutive lanes then fall on distinct 8-byte bank
-// slots reading along either axis), register inverse transforms over x (ky
-// lines) and y (x' lines) half of the wavefronts at a time with their scratch
-// in the plane, window rows into T2.
+// k_atoms_xz: one workgroup per (pose, window row y), N / 2 / (lines per
+// wavefront) wavefronts (3 at 96, 8 at 128): the inverse x transforms of the
+// row's N/2 + 1 kz lines (the kz = 0 and N/2 lines as one complex line
+// A + iB: both planes are Hermitian in (kx, ky), so after the x and y
+// transforms both are real, as the z C2R takes them), the window's x'
+// values through LDS into z lines, the z C2R of two x' lines at a time as
+// one complex line (two-for-one, imaginary parts of the kz = 0 and N/2 bins
+// dropped as a C2R transform does), the window minimum, one atomicMin per
+// workgroup.  The product spectra make one HBM round trip, of the window's
+// y rows only.
 template <>
 __device__ __forceinline__ unsigned atom_tw_bytes<96>(unsigned k, unsigned i) {
   const unsigned t = mul_u24(k, i);
   const unsigned q = mul_u24(t, 43691u) >> 22;
   return (t - 96u * q) * 8u;
 }
+
+template <int N>
+struct AtomRows {
+  using RS = RegShape<N>;
+  static constexpr int ENT = 512;           // staged channel-list entries
+  // Row stride of the LDS tiles (float2), odd: consecutive lines (the
+  // Stockham passes' consecutive threads, and the x-z kernel's lines ln of
+  // one 8-byte access) fall on distinct bank pairs.
+  static constexpr int TS = N + 1;
+  static constexpr int STAGE = ENT * 4 + 2 * 4 * 64 * 2;  // ef, ei and the shared operands (float2)
+  static constexpr int TILES = 2 * 16 * TS;               // two tiles of 16 lines (Stockham ping-pong)
+  static constexpr int U = STAGE > TILES ? STAGE : TILES;
+  static_assert(U % 2 == 0, "float4 storage");
+  // k_atoms_xz
+  static constexpr int XW = N / 2 / RS::LPW;     // wavefronts
+  static constexpr int XRW = N == 96 ? 2 : XW;   // of them transforming at a time
+  static constexpr int XT = (N / 2 + 1) * TS;    // [kz][x'] tile
+  static constexpr int XU = XT > XRW * RS::SLICE ? XT : XRW * RS::SLICE;
+};
 
 template <class P, int PR>
 __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ slots,
@@ -2276,16 +2304,21 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
                                                      const float* __restrict__ wv, const float* __restrict__ wq,
                                                      const AtomGrid* __restrict__ ag, int astride,
                                                      const RotMeta* __restrict__ meta, int nslots, int group,
-                                                     const float2* __restrict__ P3, float2* __restrict__ Eout,
-                                                     const float2* __restrict__ twg) {
+                                                     const float2* __restrict__ P3, float2* __restrict__ R,
+                                                     float invG, const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
   static_assert(N == 96 || N == 128, "the two-kernel atom form: grid edges 96 and 128");
+  using AR = AtomRows<N>;
   constexpr int NB = N / 32, NW = NB * NB;  // 32 x 32 blocks per plane
-  constexpr int ENT = 512;
+  constexpr int ENT = AR::ENT;
   __shared__ float2 stw[N];
-  __shared__ float4 ef[ENT];  // (w Z(kz), fx w Z(kz))
-  __shared__ int4 ei[ENT];    // (ix, iy, fy bits, 0; times 8 at 128)
-  __shared__ uint4 abuf[2 * 4 * 64];  // the shared A operands (the row's wavefronts share kx)
+  // The staged atoms and shared operands and the transform tiles are never
+  // live together.
+  __shared__ float4 ubuf4[AR::U / 2];
+  float2* ubuf = reinterpret_cast<float2*>(ubuf4);
+  float4* ef = ubuf4;                               // (w Z(kz), fx w Z(kz))
+  int4* ei = reinterpret_cast<int4*>(ubuf4 + ENT);  // (ix, iy, fy bits, 0; times 8 at 128)
+  uint4* abuf = reinterpret_cast<uint4*>(ubuf4 + 2 * ENT);  // [2][4][64] shared A operands
   int kz, unit;
   if (!xcd_unit_of(blockIdx.x, ZH / 8, ZH % 8, NB * ((nslots + group - 1) / group), kz, unit)) return;
   const int brow = unit % NB, sg = unit / NB;
@@ -2331,131 +2364,157 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
         }
       }
     });
-    if constexpr (PR != kAtomF32) {
-      // the split products' scales out (powers of two: exact)
-      const float un = __int_as_float(uni(__float_as_int(Ld.aunscale))) * (1.f / kAtomYScale);
+    // 1/G and the split products' scales out (powers of two: exact) in one factor
+    float sc = invG;
+    if constexpr (PR != kAtomF32)
+      sc = invG * (__int_as_float(uni(__float_as_int(Ld.aunscale))) * (1.f / kAtomYScale));
+    const int wy = uni(meta[s].win[1]);
+    float2* rb = R + (size_t)s * ((size_t)ZH * N * N) + (size_t)kz * N + 32 * brow;
+    // the row's 32 kx rows in two halves of 16 ky lines: into the tile,
+    // inverse y transforms through LDS, window rows out (consecutive threads
+    // on consecutive kx)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) E[r] = make_float2(E[r].x * un, E[r].y * un);
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();  // the staged atoms and shared operands, or the previous half, are done with
+      {
+        float2* tr = ubuf + kyB;
+#pragma unroll
+        for (int r = 8 * h; r < 8 * h + 8; ++r)
+          tr[(8 * (r >> 2) + 4 * part + (r & 3) - 16 * h) * AR::TS] = c_scale(E[r], sc);
+      }
+      __syncthreads();
+      const float2* res = lds_fft<P, 1, true, 16>(ubuf, ubuf + 16 * AR::TS, 16, 1, AR::TS, stw);
+      for (int t = tid; t < 16 * wy; t += blockDim.x) {
+        const int line = t & 15, y = t >> 4;
+        rb[(size_t)y * (ZH * N) + 16 * h + line] = res[line * AR::TS + y];
+      }
     }
-    float2* eb = Eout + (((size_t)s * ZH + kz) * NW + blk) * (8 * 128) + lane * 2;
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      *reinterpret_cast<float4*>(eb + q * 128) = make_float4(E[2 * q].x, E[2 * q].y, E[2 * q + 1].x, E[2 * q + 1].y);
+    // (the next pose's staging starts with a barrier, after these reads)
   }
-}
-
-// float2 of the inverse kernel's dynamic LDS: the plane (row stride N + 1),
-// at 128 enlarged to every wavefront's transform scratch.
-template <int N>
-constexpr int atoms_inv_lds_floats() {
-  constexpr int plane = N * (N + 1);
-  if constexpr (N == 128) {
-    constexpr int scr = (N / RegShape<N>::LPW) * RegShape<N>::SLICE;
-    return plane > scr ? plane : scr;
-  }
-  return plane;
 }
 
 template <class P>
-__global__ __launch_bounds__(1024) void k_atoms_inv(const RotMeta* __restrict__ meta,
-                                                     const float2* __restrict__ Ein, float2* __restrict__ T2,
-                                                     float invG, const float2* __restrict__ twg) {
+__global__ __launch_bounds__(64 * AtomRows<P::N>::XW) void k_atoms_xz(const RotMeta* __restrict__ meta,
+                                                                      const float2* __restrict__ R,
+                                                                      unsigned long long* __restrict__ keys,
+                                                                      const float2* __restrict__ twg) {
   constexpr int N = P::N, ZH = P::ZH;
   static_assert(N == 96 || N == 128, "the two-kernel atom form: grid edges 96 and 128");
+  using AR = AtomRows<N>;
   using RS = RegShape<N>;
   constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
-  constexpr int NB = N / 32, NW = NB * NB;
-  constexpr int S = N + 1;      // plane row stride
-  constexpr int NLW = N / LPW;  // wavefronts, each carrying LPW transform lines
-  // wavefronts transforming at a time: all at 128 (one workgroup per CU
-  // either way, so the LDS holds every wavefront's scratch), half at 96 (two
-  // workgroups per CU fit only with the plane-sized LDS)
-  constexpr int RW = N == 128 ? NLW : NLW / 2;
-  static_assert(RW * RS::SLICE <= atoms_inv_lds_floats<N>(), "scratch fits the LDS");
-  extern __shared__ float2 pl[];  // [N * S], then the transforms' scratch
+  constexpr int TS = AR::TS, XRW = AR::XRW, TURNS = (AR::XW + AR::XRW - 1) / AR::XRW;
+  extern __shared__ float2 tb[];  // [ZH][TS] (kz, x'), or the transforms' scratch
   __shared__ float2 stwr[L * V];
-  const int kz = blockIdx.x, s = blockIdx.y;
-  if (!meta[s].valid) return;
-  const int wx = meta[s].win[0], wy = meta[s].win[1];
+  const int y = blockIdx.x, s = blockIdx.y;
+  const RotMeta m = meta[s];
+  if (!m.valid || y >= m.win[1]) return;
+  const int wx = m.win[0], wz = m.win[2];
   const int tid = threadIdx.x;
   for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
-  const int part = lane >> 5;
-  // the blocks into the plane [kx][ky]
-  for (int b = wave; b < NW; b += NLW) {
-    const float2* eb = Ein + (((size_t)s * ZH + kz) * NW + b) * (8 * 128) + lane * 2;
-    const int kx0 = 32 * (b / NB), ky = 32 * (b % NB) + (lane & 31);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const float4 f = *reinterpret_cast<const float4*>(eb + q * 128);
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = 2 * q + h;
-        const int kx = kx0 + 8 * (r >> 2) + 4 * part + (r & 3);
-        pl[kx * S + ky] = h ? make_float2(f.z, f.w) : make_float2(f.x, f.y);
-      }
-    }
-  }
-  __syncthreads();
-  const float2* twr = stwr + (lane / LPW) * V;
   int ln = lane % LPW, n2 = lane / LPW;
   asm volatile("" : "+v"(ln), "+v"(n2));
-  // ky lines: value j at kx = n2 + 4j
+  const float2* twr = stwr + n2 * V;
+  // line t = LPW * wave + ln: kz = t, and for t = 0 the kz = 0 and N/2 lines
+  // as one; value j at kx = n2 + L j
+  const int t = LPW * wave + ln;
+  const float2* rr = R + (size_t)s * ((size_t)ZH * N * N) + (size_t)y * (ZH * N) + n2;
   float2 v[V];
   {
-    const int ky = wave * LPW + ln;
+    const float2* a = rr + t * N;
 #pragma unroll
-    for (int j = 0; j < V; ++j) v[j] = pl[(n2 + L * j) * S + ky];
-  }
-  __syncthreads();  // the plane is free: the transforms' scratch
+    for (int j = 0; j < V; ++j) v[j] = a[L * j];
+    if (t == 0) {
+      const float2* b = rr + (N / 2) * N;
 #pragma unroll
-  for (int rr = 0; rr < NLW / RW; ++rr) {
-    if (wave / RW == rr) {
-#pragma unroll
-      for (int i = 0; i < V; ++i) v[i] = c_scale(v[i], invG);
-      reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
-    }
-    __syncthreads();
-  }
-  // x' rows back into the plane [x'][ky], then x' lines (value j at y' = n2 + 4j)
-  {
-    const int ky = wave * LPW + ln;
-#pragma unroll
-    for (int m = 0; m < V; ++m) pl[(n2 + L * m) * S + ky] = v[m];
-  }
-  __syncthreads();
-  const int xp = wave * LPW + ln;
-  const bool xl = wave * LPW < wx;
-  if (xl) {
-#pragma unroll
-    for (int j = 0; j < V; ++j) v[j] = pl[xp * S + n2 + L * j];
-  }
-  __syncthreads();  // the plane is free: the transforms' scratch
-  float2* tb = T2 + ((size_t)s * ZH + kz) * (size_t)(N * N);
-#pragma unroll
-  for (int rr = 0; rr < NLW / RW; ++rr) {
-    if (xl && wave / RW == rr) {
-      reg_fft<N, 1>(v, twr, pl + (wave % RW) * RS::SLICE, ln, n2);
-      if (xp < wx) {
-#pragma unroll
-        for (int q = 0; q < V; ++q)
-          if (n2 + L * q < wy) tb[(q * N + xp) * L + n2] = v[q];
+      for (int j = 0; j < V; ++j) {
+        const float2 bj = b[L * j];
+        v[j] = make_float2(v[j].x - bj.y, v[j].y + bj.x);
       }
     }
-    __syncthreads();
   }
+  __syncthreads();  // stwr
+#pragma unroll
+  for (int r = 0; r < TURNS; ++r) {
+    if (wave / XRW == r) reg_fft<N, 1>(v, twr, tb + (wave % XRW) * RS::SLICE, ln, n2);
+    __syncthreads();  // this turn's scratch reads are done (the [kz][x'] tile overlays the scratch)
+  }
+  // the window's x' values into [kz][x']
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    const int xp = n2 + L * j;
+    if (xp < wx) {
+      if (t == 0) {
+        tb[xp] = make_float2(v[j].x, 0.f);
+        tb[(N / 2) * TS + xp] = make_float2(v[j].y, 0.f);
+      } else {
+        tb[t * TS + xp] = v[j];
+      }
+    }
+  }
+  __syncthreads();
+  // z lines: x' pair (2p, 2p + 1), p = t, as one complex line A + iB; value j
+  // at kz = n2 + L j over the full range (kz > N/2 from the conjugates)
+  const int p = t;
+  const bool act = 2 * p < wx, bact = 2 * p + 1 < wx;
+  float2 z[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    const int k = n2 + L * j;
+    const int kk = k <= N / 2 ? k : N - k;
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (act) A = tb[kk * TS + 2 * p];
+    if (bact) B = tb[kk * TS + 2 * p + 1];
+    if (k == 0 || k == N / 2) {
+      A.y = 0.f;
+      B.y = 0.f;
+    } else if (k > N / 2) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z[j] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();  // the tile is free: the transforms' scratch
+#pragma unroll
+  for (int r = 0; r < TURNS; ++r) {
+    if (wave / XRW == r) reg_fft<N, 1>(z, twr, tb + (wave % XRW) * RS::SLICE, ln, n2);
+    if (r + 1 < TURNS) __syncthreads();
+  }
+  unsigned long long best = ~0ull;
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const int zz = n2 + L * q;
+      if (zz < wz) {
+        const unsigned lin0 = (unsigned)((2 * p * N + y) * N + zz);
+        const unsigned long long k0 = energy_key(z[q].x, lin0);
+        best = k0 < best ? k0 : best;
+        if (bact) {
+          const unsigned long long k1 = energy_key(z[q].y, lin0 + (unsigned)(N * N));
+          best = k1 < best ? k1 : best;
+        }
+      }
+    }
+  }
+  block_min_commit<AR::XW>(best, keys + s);
 }
 
-size_t atoms_inv_lds_bytes(int N) {
-  return (size_t)(N == 128 ? atoms_inv_lds_floats<128>() : N * (N + 1)) * sizeof(float2);
+size_t atoms_xz_lds_bytes(int N) {
+  return (size_t)(N == 96 ? AtomRows<96>::XU : N == 128 ? AtomRows<128>::XU : 0) * sizeof(float2);
 }
 
-// Column form, last stage: one workgroup per (pose, block of 64/L * 4 row
-// pairs).  Row pair (x', y0 = L*m + 2h, y0 + 1) is one complex line A + iB
-// of the z C2R (two-for-one, imaginary parts of the kz = 0 and N/2 bins
-// dropped as a C2R transform does); the window minimum of the two real rows
-// goes to the pose's key.
+// k_atoms_xz's static LDS: the twiddles (N float2) and the key reduction.
+static size_t atoms_xz_static_lds(int N) {
+  return (size_t)N * sizeof(float2) + 16 * sizeof(unsigned long long);
+}
+
+// Column form (and the atom form at 64), last stage: one workgroup per
+// (pose, block of 64/L * 4 row pairs).  Row pair (x', y0 = L*m + 2h, y0 + 1)
+// is one complex line A + iB of the z C2R (two-for-one, imaginary parts of
+// the kz = 0 and N/2 bins dropped as a C2R transform does); the window
+// minimum of the two real rows goes to the pose's key.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ meta, const float2* __restrict__ T2,
                                                   unsigned long long* __restrict__ keys,
@@ -2463,18 +2522,10 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
   constexpr int N = P::N, ZH = P::ZH;
   using RS = RegShape<N>;
   constexpr int L = RS::L, V = RS::V, LPW = RS::LPW;
-  static_assert(N == 64 || N == 96 || N == 128, "column and atom forms: grid edges 64, 96 and 128");
+  static_assert(N == 64, "column form and 64-point atom form");
   constexpr int H = L / 2;        // row pairs per L rows
   constexpr int LPB = 4 * LPW;    // row pairs per workgroup
-  // Lines of 16 and more values per lane (96, 128) take their twiddles from
-  // LDS, so the registers go to the data and more wavefronts fit a SIMD; at
-  // 96 the transposes' scratch (15 KB per wavefront) would then cap the
-  // kernel at two workgroups per CU: two wavefronts transform at a time,
-  // on half the scratch.
-  constexpr bool TW_LDS = V >= 16;
-  constexpr int TURNS = N == 96 ? 2 : 1;
-  __shared__ float2 sbuf[4 / TURNS][RS::SLICE];
-  __shared__ float2 stwr[TW_LDS ? L * V : 1];
+  __shared__ float2 sbuf[4][RS::SLICE];
   const int s = blockIdx.y;
   const RotMeta m = meta[s];
   if (!m.valid) return;
@@ -2484,15 +2535,8 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
   if ((int)blockIdx.x * LPB >= nl) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ln = lane % LPW, n2 = lane / LPW;
-  float2 twreg[TW_LDS ? 1 : V];
-  const float2* twr = twreg;
-  if constexpr (TW_LDS) {
-    for (int t = threadIdx.x; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
-    __syncthreads();
-    twr = stwr + n2 * V;
-  } else {
-    reg_twiddles<N>(twg, n2, twreg);
-  }
+  float2 twr[V];
+  reg_twiddles<N>(twg, n2, twr);
   const int l = blockIdx.x * LPB + wave * LPW + ln;
   const bool ok = l < nl;
   const int mm = l / (wx * H), r = l - mm * (wx * H);
@@ -2518,15 +2562,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_z(const RotMeta* __restrict__ me
     }
     z[n1] = make_float2(A.x - B.y, A.y + B.x);
   }
-  if constexpr (TURNS == 1) {
-    reg_fft<N, 1>(z, twr, sbuf[wave], ln, n2);
-  } else {
-#pragma unroll
-    for (int t = 0; t < TURNS; ++t) {
-      if (wave / (4 / TURNS) == t) reg_fft<N, 1>(z, twr, sbuf[wave % (4 / TURNS)], ln, n2);
-      __syncthreads();  // this turn's scratch reads are done before the next turn writes it
-    }
-  }
+  reg_fft<N, 1>(z, twr, sbuf[wave], ln, n2);
   unsigned long long best = ~0ull;
   if (ok) {
 #pragma unroll
@@ -2741,9 +2777,9 @@ hipError_t configure_fwd_reg(size_t max_dyn_lds) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
       }
       if constexpr (PL::N == 96 || PL::N == 128) {
-        if (e == hipSuccess && atoms_inv_lds_bytes(PL::N) + PL::N * sizeof(float2) <= max_dyn_lds)
-          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_atoms_inv<PL>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)atoms_inv_lds_bytes(PL::N));
+        if (e == hipSuccess && atoms_xz_lds_bytes(PL::N) + atoms_xz_static_lds(PL::N) <= max_dyn_lds)
+          e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_atoms_xz<PL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)atoms_xz_lds_bytes(PL::N));
       }
     });
     if (e != hipSuccess) return e;
@@ -2809,31 +2845,31 @@ hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, 
   if (nslots <= 0) return hipSuccess;
   if (!col_supported(N) && !atoms_supported(N)) return hipErrorInvalidValue;
   CDK_DISPATCH_N(N, {
-    if constexpr (PL::N == 64 || PL::N == 96 || PL::N == 128) {
+    // windows are at most N - 2 wide (boxes are at least 3 cells)
+    if constexpr (PL::N == 64) {
       using RS = RegShape<PL::N>;
-      // windows are at most N - 2 wide (boxes are at least 3 cells)
       const int w = PL::N - 2;
       const int pairs = ((w + RS::L - 1) / RS::L) * w * (RS::L / 2);
       const unsigned gx = ceil_div(pairs, 4 * RS::LPW);
       hipLaunchKernelGGL(k_inv_z<PL>, dim3(gx, nslots), dim3(kBlock), 0, st, meta, T2, keys, tw);
     }
+    if constexpr (PL::N == 96 || PL::N == 128)
+      hipLaunchKernelGGL(k_atoms_xz<PL>, dim3(PL::N - 2, nslots), dim3(64 * AtomRows<PL::N>::XW),
+                         atoms_xz_lds_bytes(PL::N), st, meta, T2, keys, tw);
   });
   return hipGetLastError();
 }
 
-size_t atoms_work_bytes(int N) {
-  return N == 96 || N == 128 ? (size_t)N * N * (N / 2 + 1) * sizeof(float2) : 0;
-}
-
 bool atoms_supported(int N) {
-  // the inverse kernel's plane (dynamic) and register-FFT twiddles (static)
-  return N == 64 || ((N == 96 || N == 128) && atoms_inv_lds_bytes(N) + N * sizeof(float2) <= g_max_dyn_lds);
+  // the x-z kernel's tile or scratch (dynamic) and its twiddles and key
+  // reduction (static)
+  return N == 64 || ((N == 96 || N == 128) && atoms_xz_lds_bytes(N) + atoms_xz_static_lds(N) <= g_max_dyn_lds);
 }
 
 hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
                             const int* pad_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
-                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* work, float2* T2,
-                            float invG, const float2* tw, int products) {
+                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* T2, float invG,
+                            const float2* tw, int products) {
   if (nslots <= 0) return hipSuccess;
   if (!atoms_supported(N) || group <= 0 || (products != kAtomF16 && products != kAtomF32 && products != kAtomF16x3))
     return hipErrorInvalidValue;
@@ -2846,15 +2882,13 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
                          grid, dim3(kBlock), 0, st, slots, ligs, pad_atoms, wv, wq, ag, astride, meta, nslots, group,
                          P3, T2, invG, tw);
     if constexpr (PL::N == 96 || PL::N == 128) {
-      constexpr int NB = PL::N / 32, NLW = PL::N / RegShape<PL::N>::LPW;
+      constexpr int NB = PL::N / 32;
       const dim3 gf(xcd_unit_grid(N / 16, 1, NB * (int)ceil_div(nslots, group)));
       hipLaunchKernelGGL((products == kAtomF16x3 ? k_atoms_fwd<PL, kAtomF16x3>
                           : products == kAtomF16 ? k_atoms_fwd<PL, kAtomF16>
                                                  : k_atoms_fwd<PL, kAtomF32>),
                          gf, dim3(64 * NB), 0, st, slots, ligs, pad_atoms, wv, wq, ag, astride, meta, nslots, group,
-                         P3, work, tw);
-      hipLaunchKernelGGL(k_atoms_inv<PL>, dim3(PL::ZH, nslots), dim3(64 * NLW), atoms_inv_lds_bytes(PL::N), st, meta,
-                         work, T2, invG, tw);
+                         P3, T2, invG, tw);
     }
   });
   return hipGetLastError();

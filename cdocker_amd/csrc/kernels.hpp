@@ -71,11 +71,15 @@ hipError_t launch_mid_kz(hipStream_t st, int N, const PoseSlot* slots, const Lig
 hipError_t launch_inv_z(hipStream_t st, int N, const RotMeta* meta, int nslots, const float2* T2,
                         unsigned long long* keys, const float2* tw);
 hipError_t launch_to_kz_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
-// Atom form (grid edges 64 and 96): the mid stage builds each channel's ligand
-// spectrum on a kz plane as a sum of per-atom outer products on the matrix
-// cores (no ligand grids, no forward stage); then as the column form, with
-// k_inv_z last.  P3 is the protein spectra in the matrix-core result layout
-// (launch_to_mfma_blocks).
+// Atom form (grid edges 64, 96 and 128): the mid stage builds each channel's
+// ligand spectrum on a kz plane as a sum of per-atom outer products on the
+// matrix cores (no ligand grids, no forward stage).  At 64 it then runs as
+// the column form (the inverse y and x transforms in the mid kernel, window
+// rows into T2, k_inv_z last); at 96 and 128 the mid stage runs the inverse
+// y transforms and writes the window's y rows of the product spectra into T2
+// (at most N (N/2 + 1) N float2 per pose), and launch_inv_z runs the inverse
+// x and z transforms and the argmin per (pose, y row).  P3 is the protein
+// spectra in the matrix-core result layout (launch_to_mfma_blocks).
 constexpr int kPipeAtoms = 2;
 bool atoms_supported(int N);
 // Atom form ligand-spectrum products: FP16 matrix cores on operands split
@@ -89,13 +93,10 @@ bool atoms_supported(int N);
 constexpr int kAtomF16 = 0, kAtomF32 = 1, kAtomF16x3 = 2;
 constexpr float kAtomYScale = 4096.f;
 // pad_atoms: the ligands' padded channel lists (LigandDesc::pad_atoms).
-// work: at grid edge 96 the product spectra between the two kernels,
-// atoms_work_bytes(N) per pose (unused at 64).
-size_t atoms_work_bytes(int N);
 hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const LigandDesc* ligs,
                             const int* pad_atoms, const float* wv, const float* wq, const AtomGrid* ag, int astride,
-                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* work, float2* T2,
-                            float invG, const float2* tw, int products);
+                            const RotMeta* meta, int nslots, int group, const float2* P3, float2* T2, float invG,
+                            const float2* tw, int products);
 hipError_t launch_to_mfma_blocks(hipStream_t st, int N, const float2* in, float2* out, int nch);
 // Inverse plane stage variants: one workgroup per (pose, window plane) with
 // block-wide LDS FFTs, or one workgroup per (pose, 4 window planes) with
