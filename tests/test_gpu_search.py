@@ -465,9 +465,10 @@ def test_atom_form_channel_padding_edges(ctx, name):
 @pytest.mark.parametrize("name", ["p96", "p128"])
 def test_atom_form_96_128_vs_oracle(ctx, name):
     """Atom form at 96^3 and 128^3 (nine or sixteen 32 x 32 matrix-core blocks
-    per kz plane, at 96 the twiddle index reduced mod 96; the product spectra
-    through HBM to a second kernel's 96- or 128-point inverse transforms)
-    against the oracle, and the same bits for every mid-stage group size.  The
+    per kz plane, at 96 the twiddle index reduced mod 96; the inverse y
+    transforms in the forward kernel, the window's y rows through HBM to the
+    x-z kernel's 96- or 128-point transforms and argmin) against the oracle,
+    and the same bits for every mid-stage group size.  The
     300-atom ligand's padded channel lists exceed the 512 staged entries, so
     the forward kernel stages them one channel at a time."""
     g = workloads.protein_grids(name)
