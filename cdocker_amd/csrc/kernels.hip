@@ -848,6 +848,31 @@ __device__ __forceinline__ bool xcd_unit_of(int w, int Q, int R, int G, int& ite
   return unit < G;
 }
 
+// Plane-major form for kernels whose items share large per-item data (the
+// protein spectra of a kz plane): XCD x runs its planes Qx ... Qx + Q - 1
+// one after another, all G units of a plane before the next, then its share
+// of the R leftover planes' items (spread over the XCDs in turn).  The
+// workgroups resident on an XCD at a time then cover one or two planes, whose
+// spectra its L2 holds.
+__host__ __device__ __forceinline__ unsigned xcd_plane_grid(int Q, int R, int G) {
+  return 8u * (unsigned)(Q * G + (R * G + 7) / 8);
+}
+__device__ __forceinline__ bool xcd_plane_of(int w, int Q, int R, int G, int& item, int& unit) {
+  const int x = w & 7, m = w >> 3;
+  if (m < Q * G) {
+    const int p = m / G;
+    item = Q * x + p;
+    unit = m - p * G;
+    return true;
+  }
+  const int j = (m - Q * G) * 8 + x;
+  if (j >= R * G) return false;
+  const int i = j / G;
+  item = 8 * Q + i;
+  unit = j - i * G;
+  return true;
+}
+
 // Stage B: x FFT of every used channel, conj-multiply with the protein
 // spectra, channel sum, 1/G, inverse x FFT, for the (ky, kz) lines of the
 // spectrum.  Each wavefront owns 64/L consecutive lines (flattened
@@ -2176,7 +2201,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   float4* ef = reinterpret_cast<float4*>(scratch);                      // (w Z(kz), fx w Z(kz))
   int4* ei = reinterpret_cast<int4*>(scratch + ENT * sizeof(float4));  // (8 ix, 8 iy, fy bits, 0)
   int kz, sg;
-  if (!xcd_unit_of(blockIdx.x, ZH / 8, ZH % 8, (nslots + group - 1) / group, kz, sg)) return;
+  if (!xcd_plane_of(blockIdx.x, ZH / 8, ZH % 8, (nslots + group - 1) / group, kz, sg)) return;
   const int tid = threadIdx.x;
   for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
   for (int t = tid; t < L * V; t += blockDim.x) stwr[t] = twg[(t / V) * (t % V)];
@@ -2320,7 +2345,7 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
   int4* ei = reinterpret_cast<int4*>(ubuf4 + ENT);  // (ix, iy, fy bits, 0; times 8 at 128)
   uint4* abuf = reinterpret_cast<uint4*>(ubuf4 + 2 * ENT);  // [2][4][64] shared A operands
   int kz, unit;
-  if (!xcd_unit_of(blockIdx.x, ZH / 8, ZH % 8, NB * ((nslots + group - 1) / group), kz, unit)) return;
+  if (!xcd_plane_of(blockIdx.x, ZH / 8, ZH % 8, NB * ((nslots + group - 1) / group), kz, unit)) return;
   const int brow = unit % NB, sg = unit / NB;
   const int tid = threadIdx.x;
   for (int t = tid; t < N; t += blockDim.x) stw[t] = twg[t];
@@ -2873,7 +2898,7 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
   if (nslots <= 0) return hipSuccess;
   if (!atoms_supported(N) || group <= 0 || (products != kAtomF16 && products != kAtomF32 && products != kAtomF16x3))
     return hipErrorInvalidValue;
-  const dim3 grid(xcd_unit_grid(N / 16, 1, (int)ceil_div(nslots, group)));
+  const dim3 grid(xcd_plane_grid((N / 2 + 1) / 8, (N / 2 + 1) % 8, (int)ceil_div(nslots, group)));
   CDK_DISPATCH_N(N, {
     if constexpr (PL::N == 64)
       hipLaunchKernelGGL((products == kAtomF16x3 ? k_mid_atoms<PL, kAtomF16x3>
@@ -2883,7 +2908,7 @@ hipError_t launch_mid_atoms(hipStream_t st, int N, const PoseSlot* slots, const 
                          P3, T2, invG, tw);
     if constexpr (PL::N == 96 || PL::N == 128) {
       constexpr int NB = PL::N / 32;
-      const dim3 gf(xcd_unit_grid(N / 16, 1, NB * (int)ceil_div(nslots, group)));
+      const dim3 gf(xcd_plane_grid(PL::ZH / 8, PL::ZH % 8, NB * (int)ceil_div(nslots, group)));
       hipLaunchKernelGGL((products == kAtomF16x3 ? k_atoms_fwd<PL, kAtomF16x3>
                           : products == kAtomF16 ? k_atoms_fwd<PL, kAtomF16>
                                                  : k_atoms_fwd<PL, kAtomF32>),
