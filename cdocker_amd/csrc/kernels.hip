@@ -49,6 +49,12 @@ __device__ __forceinline__ unsigned long long energy_key(float e, unsigned lin) 
   return ((unsigned long long)u << 32) | (unsigned long long)lin;
 }
 
+// The key's high word alone: u ^ (sign ? ~0 : 0x80000000).
+__device__ __forceinline__ unsigned energy_bits(float e) {
+  const unsigned u = __float_as_uint(e);
+  return u ^ ((unsigned)((int)u >> 31) | 0x80000000u);
+}
+
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -2320,6 +2326,9 @@ struct AtomRows {
   static constexpr int XRW = N == 96 ? 2 : XW;   // of them transforming at a time
   static constexpr int XT = (N / 2 + 1) * TS;    // [kz][x'] tile
   static constexpr int XU = XT > XRW * RS::SLICE ? XT : XRW * RS::SLICE;
+  // wavefronts per SIMD the tile's LDS allows (160 KB per CU): the compiler
+  // may spend registers up to that occupancy on loads in flight
+  static constexpr int XOCC = (160 * 1024 / (XU * 8 + 2048)) * XW / 4;
 };
 
 template <class P, int PR>
@@ -2419,7 +2428,8 @@ __global__ __launch_bounds__(256) void k_atoms_fwd(const PoseSlot* __restrict__ 
 }
 
 template <class P>
-__global__ __launch_bounds__(64 * AtomRows<P::N>::XW) void k_atoms_xz(const RotMeta* __restrict__ meta,
+__global__ __launch_bounds__(64 * AtomRows<P::N>::XW) __attribute__((amdgpu_waves_per_eu(1, AtomRows<P::N>::XOCC)))
+void k_atoms_xz(const RotMeta* __restrict__ meta,
                                                                       const float2* __restrict__ R,
                                                                       unsigned long long* __restrict__ keys,
                                                                       const float2* __restrict__ twg) {
@@ -2446,18 +2456,20 @@ __global__ __launch_bounds__(64 * AtomRows<P::N>::XW) void k_atoms_xz(const RotM
   // as one; value j at kx = n2 + L j
   const int t = LPW * wave + ln;
   const float2* rr = R + (size_t)s * ((size_t)ZH * N * N) + (size_t)y * (ZH * N) + n2;
-  float2 v[V];
+  float2 v[V], bv[V];
   {
+    // the kz = N/2 row's loads first, so none of them waits on another
+    if (t == 0) {
+      const float2* b = rr + (N / 2) * N;
+#pragma unroll
+      for (int j = 0; j < V; ++j) bv[j] = b[L * j];
+    }
     const float2* a = rr + t * N;
 #pragma unroll
     for (int j = 0; j < V; ++j) v[j] = a[L * j];
     if (t == 0) {
-      const float2* b = rr + (N / 2) * N;
 #pragma unroll
-      for (int j = 0; j < V; ++j) {
-        const float2 bj = b[L * j];
-        v[j] = make_float2(v[j].x - bj.y, v[j].y + bj.x);
-      }
+      for (int j = 0; j < V; ++j) v[j] = make_float2(v[j].x - bv[j].y, v[j].y + bv[j].x);
     }
   }
   __syncthreads();  // stwr
@@ -2466,32 +2478,28 @@ __global__ __launch_bounds__(64 * AtomRows<P::N>::XW) void k_atoms_xz(const RotM
     if (wave / XRW == r) reg_fft<N, 1>(v, twr, tb + (wave % XRW) * RS::SLICE, ln, n2);
     __syncthreads();  // this turn's scratch reads are done (the [kz][x'] tile overlays the scratch)
   }
-  // the window's x' values into [kz][x']
+  // every x' into [kz][x'] (the columns past the window are read as the
+  // partners of the window's last pair); line 0 holds kz = 0 in its real
+  // and kz = N/2 in its imaginary parts
 #pragma unroll
-  for (int j = 0; j < V; ++j) {
-    const int xp = n2 + L * j;
-    if (xp < wx) {
-      if (t == 0) {
-        tb[xp] = make_float2(v[j].x, 0.f);
-        tb[(N / 2) * TS + xp] = make_float2(v[j].y, 0.f);
-      } else {
-        tb[t * TS + xp] = v[j];
-      }
-    }
+  for (int j = 0; j < V; ++j) tb[t * TS + n2 + L * j] = v[j];
+  if (t == 0) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) tb[(N / 2) * TS + n2 + L * j] = make_float2(v[j].y, 0.f);
   }
   __syncthreads();
   // z lines: x' pair (2p, 2p + 1), p = t, as one complex line A + iB; value j
-  // at kz = n2 + L j over the full range (kz > N/2 from the conjugates)
+  // at kz = n2 + L j over the full range (kz > N/2 from the conjugates).  A
+  // wavefront whose pairs all lie past the window skips its transforms.
   const int p = t;
   const bool act = 2 * p < wx, bact = 2 * p + 1 < wx;
+  const bool wact = 2 * LPW * wave < wx;
   float2 z[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     const int k = n2 + L * j;
     const int kk = k <= N / 2 ? k : N - k;
-    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
-    if (act) A = tb[kk * TS + 2 * p];
-    if (bact) B = tb[kk * TS + 2 * p + 1];
+    float2 A = tb[kk * TS + 2 * p], B = tb[kk * TS + 2 * p + 1];
     if (k == 0 || k == N / 2) {
       A.y = 0.f;
       B.y = 0.f;
@@ -2504,24 +2512,29 @@ __global__ __launch_bounds__(64 * AtomRows<P::N>::XW) void k_atoms_xz(const RotM
   __syncthreads();  // the tile is free: the transforms' scratch
 #pragma unroll
   for (int r = 0; r < TURNS; ++r) {
-    if (wave / XRW == r) reg_fft<N, 1>(z, twr, tb + (wave % XRW) * RS::SLICE, ln, n2);
+    if (wact && wave / XRW == r) reg_fft<N, 1>(z, twr, tb + (wave % XRW) * RS::SLICE, ln, n2);
     if (r + 1 < TURNS) __syncthreads();
   }
-  unsigned long long best = ~0ull;
-  if (act) {
+  // the lane's window minimum: values in increasing translation index (line
+  // 2p's z values, then line 2p + 1's); the first strict minimum of the
+  // ordered energy bits (the keys' high word) and its position, then one key
+  unsigned bu = 0xffffffffu;
+  int bq = -1;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const bool lok = h ? bact : act;
 #pragma unroll
     for (int q = 0; q < V; ++q) {
-      const int zz = n2 + L * q;
-      if (zz < wz) {
-        const unsigned lin0 = (unsigned)((2 * p * N + y) * N + zz);
-        const unsigned long long k0 = energy_key(z[q].x, lin0);
-        best = k0 < best ? k0 : best;
-        if (bact) {
-          const unsigned long long k1 = energy_key(z[q].y, lin0 + (unsigned)(N * N));
-          best = k1 < best ? k1 : best;
-        }
-      }
+      const unsigned u = energy_bits(h ? z[q].y : z[q].x);
+      const bool take = lok && n2 + L * q < wz && (u < bu || bq < 0);
+      bu = take ? u : bu;
+      bq = take ? h * V + q : bq;
     }
+  }
+  unsigned long long best = ~0ull;
+  if (bq >= 0) {
+    const int h = bq >= V ? 1 : 0, q = bq - h * V;
+    best = ((unsigned long long)bu << 32) | (unsigned)(((2 * p + h) * N + y) * N + n2 + L * q);
   }
   block_min_commit<AR::XW>(best, keys + s);
 }
