@@ -509,6 +509,60 @@ def line(args, metric, value, total, workload, extra_config, roof, kernels, cpu,
     print(json.dumps(out), flush=True)
 
 
+def callers_leg(ctx, cdk, workloads, prot, lig, L, E, idx, mc, nlow=10):
+    """The search's callers on either side, each timed end to end through
+    the C-ABI with host arrays in and out (staging, device buffers and PCIe
+    included), best of a few calls: retention of the top-10 poses per
+    conformer with their rebuilt coordinates (NoMini:466-487), FP64 grid
+    rescoring of those poses with and without the gradient, rigid-body
+    refinement of them (50 steps), the grid handoff to the host minimiser
+    (every channel, x fastest, AddGridForcesToOpenMMSystem.cpp:30-50) and
+    random-torsion conformer generation (GeneRandomConformations.cpp:14-111,
+    steric filter)."""
+    def best(f, reps=3):
+        b, r = 1e30, None
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = f()
+            b = min(b, time.perf_counter() - t)
+        return b, r
+
+    out = {}
+    nconf = E.shape[0]
+    t = time.perf_counter()
+    kept = []
+    for c in range(nconf):
+        rot = ctx.Rotate(L["quats"][c], L["conformers"][c])
+        _, pose = cdk.rank_poses(prot, rot, E[c], idx[c], mc[c], nlow)
+        kept.append(pose)
+    dt = time.perf_counter() - t
+    poses = np.concatenate(kept)
+    n = poses.shape[0]
+    out["retention"] = {"conformers": nconf, "poses_kept": n, "s": round(dt, 4),
+                        "conformers_per_s": round(nconf / dt, 1)}
+    dt, (e, _, ins) = best(lambda: cdk.rescore_poses(prot, lig, poses))
+    out["rescore"] = {"poses": n, "atoms": lig.natom, "s": round(dt, 5), "poses_per_s": round(n / dt, 1),
+                      "on_grid": int((ins == lig.natom).sum())}
+    dt, _ = best(lambda: cdk.rescore_poses(prot, lig, poses, want_grad=True))
+    out["rescore_with_gradient"] = {"poses": n, "s": round(dt, 5), "poses_per_s": round(n / dt, 1)}
+    dt, (_, e2) = best(lambda: cdk.refine_poses(prot, lig, poses, steps=50), reps=2)
+    out["refine"] = {"poses": n, "steps": 50, "s": round(dt, 5), "poses_per_s": round(n / dt, 1),
+                     "mean_energy_drop": round(float(np.mean(e - e2)), 4)}
+    nch = prot.num_grids
+    dt, _ = best(lambda: [cdk.grid_for_openmm(prot, ch) for ch in range(nch)])
+    nbytes = nch * prot.n ** 3 * 8
+    out["grid_for_openmm"] = {"channels": nch, "bytes_out": nbytes, "s": round(dt, 5),
+                              "GBps_to_host": round(nbytes / dt / 1e9, 2)}
+    x0, _, _, _ = cdk.synth_ligand(4242, 40)
+    rot, mv, ex = workloads.chain_topology(40)
+    ntrial = 20000
+    dt, (confs, _) = best(lambda: cdk.generate_conformers(ctx, x0, rot, mv, ex, ntrial, 11, 1.2, max_keep=1000),
+                          reps=2)
+    out["conformers"] = {"atoms": 40, "rotors": int(rot.shape[0]), "trials": ntrial, "kept": int(confs.shape[0]),
+                         "s": round(dt, 5), "trials_per_s": round(ntrial / dt, 1)}
+    return out
+
+
 def run_single(args, cdk, workloads):
     t0 = time.time()
     g = workloads.protein_grids(args.protein)
@@ -565,6 +619,14 @@ def run_single(args, cdk, workloads):
                     "max_abs_diff_vs_host_recipe": diff}
     log("grid builder", grid_builder)
 
+    callers = None
+    if args.callers:
+        try:
+            callers = callers_leg(ctx, cdk, workloads, prot, lig, L, E, idx, mc)
+        except Exception as ex:  # a caller's failure must not lose the search's line
+            callers = {"error": "%s: %s" % (type(ex).__name__, ex)}
+        log("callers", json.dumps(callers))
+
     serving = None
     if args.serve_requests > 0:
         serving = serve_leg(ctx, cdk, workloads, args.serve_requests, args.serve_load, args.serve_batch, 7100)
@@ -579,7 +641,8 @@ def run_single(args, cdk, workloads):
          roof, kernels, cpu,
          pipeline_equiv={"bytes_per_pose": reference_bytes(N, C), "GBps": round(value * reference_bytes(N, C) / 1e9, 1),
                          "note": "poses/s x B(N,C) of the unfused reference dataflow"},
-         reference_engine=ref, fp64_check=f64, serving=serving, grid_builder=grid_builder)
+         reference_engine=ref, fp64_check=f64, serving=serving, grid_builder=grid_builder,
+         callers=callers)
     batch.close()
     lig.close()
     prot.close()
@@ -747,6 +810,8 @@ def main():
     ap.add_argument("--fwd", type=int, default=-1, help="forward plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--inv", type=int, default=-1, help="inverse plane form: 0 block, 1 register, -1 autotuned")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--callers", type=int, default=1,
+                    help="time the callers either side of the search (retention, rescoring, refinement, grid handoff, conformers)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="Poisson rate as a fraction of capacity")
     ap.add_argument("--serve-batch", type=int, default=4096, help="poses per server launch set")
     ap.add_argument("--large-full-checks", type=int, default=6, help="full FP64 searches in the large workload")

@@ -33,6 +33,18 @@ def ligand(seed, natom, nconf):
     return dict(charge=q, epsilon=e, radius=r, conformers=confs)
 
 
+def chain_topology(natom):
+    """Torsion topology of the synthetic ligands (a chain bonded i-(i+1)):
+    rotor (b-1, b, b+1, b+2) for each inner bond b -> b+1 turns atoms b+2..
+    (the bonds the synthetic conformers twist); 1-2, 1-3 and 1-4 pairs
+    excluded from the clash test.  Inputs for generate_conformers."""
+    bonds = np.arange(1, max(natom - 2, 1))
+    rot = np.stack([bonds - 1, bonds, bonds + 1, bonds + 2], axis=1).astype(np.int32).reshape(-1, 4)
+    mv = (np.arange(natom)[None, :] >= rot[:, 2:3] + 1).astype(np.uint8)
+    sep = np.abs(np.arange(natom)[:, None] - np.arange(natom)[None, :])
+    return rot, mv, (sep <= 3).astype(np.uint8)
+
+
 def quaternion_sets(seed, nconf, nquat):
     """[nconf][nquat][4]: successive QuaternionUniformSampling draws on one mt19937_64."""
     rng = api.Rng(seed)
