@@ -471,6 +471,33 @@ def cpu_leg(g, L, nsample, threads, N, budget_s=20.0):
     return cpu
 
 
+def cpu_callers(cdk, workloads, g, L, E, idx, mc, nsample=2000, ntrial=2000):
+    """Beside the device figures of the callers object: the oracle's FP64
+    restatements on one host thread, for the grid energy of explicit search
+    poses (the rescoring's energy: trilinear grid values at the atoms,
+    orc_sparse_energy_f64, over the first valid poses of conformer 0) and
+    for random-torsion conformer trials of the same 40-atom chain
+    (orc_conformers)."""
+    from oracle import oracle as orc
+    prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+    qs = np.flatnonzero(np.isfinite(E[0]))[:nsample]
+    rot = orc.rotate(L["quats"][0, qs], L["conformers"][0])
+    ts = time.perf_counter()
+    for k, q in enumerate(qs):
+        prob.sparse_energy(rot[k], mc[0, q], idx[0, q])
+    dt = time.perf_counter() - ts
+    out = {"cores": 1, "kind": "port",
+           "rescore_energy": {"poses": int(qs.size), "s": round(dt, 4), "poses_per_s": round(qs.size / dt, 1),
+                              "note": "one oracle call per pose from Python"}}
+    x0, _, _, _ = cdk.synth_ligand(4242, 40)
+    rotors, mv, ex = workloads.chain_topology(40)
+    ts = time.perf_counter()
+    orc.conformers(x0.astype(np.float64), rotors, mv, ex, ntrial, 11, 1.2)
+    dt = time.perf_counter() - ts
+    out["conformers"] = {"trials": ntrial, "s": round(dt, 4), "trials_per_s": round(ntrial / dt, 1)}
+    return out
+
+
 def arithmetic_note(roof):
     prod = str((roof or {}).get("products", ""))
     if prod.startswith("split-f16 x3"):
@@ -606,6 +633,12 @@ def run_single(args, cdk, workloads):
     ref["speedup_fused_vs_roofline"] = round(value / ref["roofline_poses_per_s"], 2)
     f64 = fp64_check(args, g, L, E, idx, mc, args.fp64_sample, args.fp64_full) if args.fp64_sample > 0 else None
     cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
+    if cpu is not None and args.callers:
+        try:
+            cpu["callers"] = cpu_callers(cdk, workloads, g, L, E, idx, mc)
+        except Exception as ex:  # a baseline's failure must not lose the search's line
+            cpu["callers"] = {"error": "%s: %s" % (type(ex).__name__, ex)}
+        log("cpu callers", cpu["callers"])
 
     # The same protein's grids built on the device from its atoms.
     natom_p, box_p, _, _ = workloads.PROTEINS[args.protein]
