@@ -471,13 +471,14 @@ def cpu_leg(g, L, nsample, threads, N, budget_s=20.0):
     return cpu
 
 
-def cpu_callers(cdk, workloads, g, L, E, idx, mc, nsample=2000, ntrial=2000):
+def cpu_callers(cdk, workloads, protein, g, L, E, idx, mc, nsample=2000, ntrial=2000):
     """Beside the device figures of the callers object: the oracle's FP64
     restatements on one host thread, for the grid energy of explicit search
     poses (the rescoring's energy: trilinear grid values at the atoms,
     orc_sparse_energy_f64, over the first valid poses of conformer 0) and
     for random-torsion conformer trials of the same 40-atom chain
-    (orc_conformers)."""
+    (orc_conformers), and for the protein's potential grids built from its
+    atoms (orc_build_grids, brute force)."""
     from oracle import oracle as orc
     prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
     qs = np.flatnonzero(np.isfinite(E[0]))[:nsample]
@@ -495,6 +496,14 @@ def cpu_callers(cdk, workloads, g, L, E, idx, mc, nsample=2000, ntrial=2000):
     orc.conformers(x0.astype(np.float64), rotors, mv, ex, ntrial, 11, 1.2)
     dt = time.perf_counter() - ts
     out["conformers"] = {"trials": ntrial, "s": round(dt, 4), "trials_per_s": round(ntrial / dt, 1)}
+    # the protein's potential grids from its atoms (orc_build_grids, brute force FP64)
+    natom_p, box_p, n, h = workloads.PROTEINS[protein]
+    xyz, q, e, r = cdk.synth_protein_atoms(workloads.PROTEIN_SEED[protein], natom_p, box_p)
+    gmin = np.asarray(g["mid"], np.float64) - np.asarray(g["length"], np.float64) / 2
+    ts = time.perf_counter()
+    orc.build_grids(xyz, q, e, r, g["radii"], n, gmin, h)
+    dt = time.perf_counter() - ts
+    out["grid_builder"] = {"atoms": natom_p, "grids": "%d^3 x %d" % (n, len(g["radii"]) + 1), "s": round(dt, 3)}
     return out
 
 
@@ -635,7 +644,7 @@ def run_single(args, cdk, workloads):
     cpu = cpu_leg(g, L, args.cpu_sample, args.cpu_threads, N) if args.cpu_sample > 0 else None
     if cpu is not None and args.callers:
         try:
-            cpu["callers"] = cpu_callers(cdk, workloads, g, L, E, idx, mc)
+            cpu["callers"] = cpu_callers(cdk, workloads, args.protein, g, L, E, idx, mc)
         except Exception as ex:  # a baseline's failure must not lose the search's line
             cpu["callers"] = {"error": "%s: %s" % (type(ex).__name__, ex)}
         log("cpu callers", cpu["callers"])
