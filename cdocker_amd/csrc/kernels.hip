@@ -2045,11 +2045,20 @@ __device__ __forceinline__ void atom_channel(const float4* ef, const int4* ei, c
     // zeroing moves); the fused 64-point kernel, at its register limit, runs
     // faster with one copy of the step.
     if (N == 64 || k0 >= kend) {
+      // 64-bit moves: the zeroing takes half the instructions the compiler's
+      // own 32-bit copies of one zero register take here
+      typedef double zd8 __attribute__((ext_vector_type(8)));
+      zd8 zr, zi;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        cr[r] = 0.f;
-        ci[r] = 0.f;
+      for (int r = 0; r < 8; ++r) {
+        double a, b;
+        asm volatile("v_mov_b64 %0, 0" : "=v"(a));
+        asm volatile("v_mov_b64 %0, 0" : "=v"(b));
+        zr[r] = a;
+        zi[r] = b;
       }
+      cr = __builtin_bit_cast(mfma_acc16, zr);
+      ci = __builtin_bit_cast(mfma_acc16, zi);
     }
     if constexpr (N == 64) {
       for (int k = k0; k < kend; k += 8) step(k, std::false_type{});
