@@ -78,6 +78,37 @@ def test_synthetic_generators_are_deterministic_and_valid():
     assert np.array_equal(radii, PROBE) and np.allclose(ln, 31 * 0.5)
 
 
+def test_chain_topology_turns_the_synthetic_torsions():
+    """workloads.chain_topology, the conformer-generation inputs the bench
+    uses for the synthetic chain ligands: each rotor's four atoms are
+    consecutive chain atoms, a rotor about bond b -> b+1 moves exactly atoms
+    b+2.., turning it keeps every bond length and the far side rigid, and the
+    exclusions are the 1-2, 1-3 and 1-4 pairs (the same as the tests'
+    oracle helper)."""
+    from cdocker_amd import workloads
+    for n in (4, 5, 12, 40):
+        rot, mv, ex = workloads.chain_topology(n)
+        r2, m2, e2 = orc.chain_rotors(n)
+        assert np.array_equal(rot, r2) and np.array_equal(mv, m2) and np.array_equal(ex, e2)
+        assert rot.shape == (n - 3, 4) and (np.diff(rot, axis=1) == 1).all()
+    x, _, _, _ = cdk.synth_ligand(5, 12)
+    rot, mv, _ = workloads.chain_topology(12)
+    x = x.astype(np.float64)
+    for (a, b, c, d), m in zip(rot, mv):
+        axis = (x[c] - x[b]) / np.linalg.norm(x[c] - x[b])
+        th = 0.7
+        k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+        R = np.eye(3) + np.sin(th) * k + (1 - np.cos(th)) * (k @ k)
+        y = x.copy()
+        y[m.astype(bool)] = (x[m.astype(bool)] - x[c]) @ R.T + x[c]
+        assert np.allclose(np.linalg.norm(y[1:] - y[:-1], axis=1), np.linalg.norm(x[1:] - x[:-1], axis=1))
+        moved = np.flatnonzero(np.linalg.norm(y - x, axis=1) > 1e-9)
+        assert moved.min() >= c + 1  # atoms up to c stay put
+        far = m.astype(bool)
+        dfar = np.linalg.norm(y[far][:, None] - y[far][None], axis=2)
+        assert np.allclose(dfar, np.linalg.norm(x[far][:, None] - x[far][None], axis=2))
+
+
 def test_grid_file_round_trip(tmp_path):
     rs = np.random.default_rng(0)
     vals = rs.standard_normal((3, 4, 5, 6)).astype(np.float32)
