@@ -198,3 +198,44 @@ def test_conformers_on_device_match_oracle(ctx):
     want = np.flatnonzero(ok)[:50]
     assert np.array_equal(trial, want)
     assert np.abs(conf - ref[want]).max() < 1e-9
+
+
+def test_callers_at_bench_sizes(ctx):
+    """The callers at the sizes the bench times them: the top 10 poses of each
+    of 10 conformers of the 64^3 single-ligand search rescored (equal to the
+    oracle's FP64 gather at the search's translation, and to the search's own
+    energy), refined downhill, and 20,000 conformer trials of a 40-atom chain
+    whose first 1,000 kept trials equal the oracle's."""
+    g = workloads.protein_grids("p64")
+    prot = cdk.Protein(ctx, g["values"], g["radii"], g["spacing"], g["mid"], g["length"])
+    L = workloads.config2(nconf=10, nquat=200)["ligands"][0]
+    lig = cdk.Ligand(prot, L["charge"], L["epsilon"], L["radius"])
+    try:
+        E, idx, mc = cdk.search(ctx, prot, lig, L["conformers"], L["quats"])
+        prob = orc.PoseProblem(g["values"], g["radii"], g["spacing"], L["charge"], L["epsilon"], L["radius"])
+        poses, e64, e32 = [], [], []
+        for c in range(E.shape[0]):
+            rot = ctx.Rotate(L["quats"][c], L["conformers"][c])
+            rank, pose = cdk.rank_poses(prot, rot, E[c], idx[c], mc[c], 10)
+            poses.append(pose)
+            e64 += [prob.sparse_energy(rot[q], mc[c, q], idx[c, q]) for q in rank]
+            e32 += [float(E[c, q]) for q in rank]
+        poses = np.concatenate(poses)
+        e64 = np.asarray(e64)
+        say("callers: %d retained poses" % poses.shape[0])
+        e, _, ins = cdk.rescore_poses(prot, lig, poses)
+        assert poses.shape[0] == 100 and (ins == lig.natom).all()
+        assert np.abs(e - e64).max() / np.maximum(np.abs(e64), 1.0).max() < 1e-9
+        assert (np.abs(e - np.asarray(e32)) <= 1e-4 * np.maximum(np.abs(e64), 10.0)).all()
+        x, e2 = cdk.refine_poses(prot, lig, poses, steps=50)
+        assert (e2 <= e + 1e-9).all() and e2.mean() < e.mean()
+        x0, _, _, _ = cdk.synth_ligand(4242, 40)
+        rot, mv, ex = workloads.chain_topology(40)
+        conf, trial = cdk.generate_conformers(ctx, x0, rot, mv, ex, 20000, 11, 1.2, max_keep=1000)
+        ref, ok = orc.conformers(x0.astype(np.float64), rot, mv, ex, 20000, 11, 1.2)
+        want = np.flatnonzero(ok)[:1000]
+        say("callers: %d conformer trials kept of 20000" % int(ok.sum()))
+        assert np.array_equal(trial, want) and np.abs(conf - ref[want]).max() < 1e-9
+    finally:
+        lig.close()
+        prot.close()
